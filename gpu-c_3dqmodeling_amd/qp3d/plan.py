@@ -1,0 +1,119 @@
+"""Python mirror of the solver boundary: modeling3d() (3dPureqPModeling_single_shot.h:4-6)
+and the device-resident plan API, both through the C ABI of libqp3d.so."""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import Config, STORAGE, check, fptr, load
+
+
+def _tables(s):
+    return [s.dlr, s.ddlr, s.dtb, s.ddtb, s.dfb, s.ddfb, s.c, s.c2]
+
+
+def modeling3d(s, record=None, snapflag=0, ntsnap=1, snap_file=None):
+    """Run one shot through the drop-in entry point qp3d_modeling3d (same arguments as the
+    reference's modeling3d). Returns the record [nt][hx][hy]."""
+    lib = load()
+    if record is None:
+        record = np.zeros((s.nt, s.hx, s.hy), np.float32)
+    sx, sy, _ = s.src_cell
+    arrs = [s.vp, s.eps, s.dele, s.source, s.wavelet, record] + _tables(s)
+    arrs = [np.ascontiguousarray(a, np.float32) for a in arrs]
+    rc = lib.qp3d_modeling3d(s.nx, s.ny, s.nz, s.nt, ntsnap, s.dx, s.dy, s.dz, s.dt, s.pml, snapflag, sx, sy, s.sz,
+                             *[fptr(a.reshape(-1)) for a in arrs],
+                             snap_file.encode() if snap_file else None)
+    check(rc)
+    return arrs[5].reshape(s.nt, s.hx, s.hy)
+
+
+class Plan:
+    """Device-resident forward solver for one grid (qp3d_plan_* in include/qp3d.h)."""
+
+    def __init__(self, s, storage="fp32", use_graph=False, snapflag=0, ntsnap=0, zchunk=0, device=0):
+        self.lib = load()
+        self.s = s
+        cfg = Config(s.nx, s.ny, s.nz, s.pml, s.nt, s.dx, s.dy, s.dz, s.dt, s.sz, STORAGE[storage],
+                     int(use_graph), int(snapflag), int(ntsnap), int(zchunk), int(device))
+        h = C.c_void_p()
+        check(self.lib.qp3d_plan_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.upload(s)
+
+    def upload(self, s):
+        arrs = [np.ascontiguousarray(a, np.float32).reshape(-1)
+                for a in [s.vp, s.eps, s.dele, s.source, s.wavelet] + _tables(s)]
+        check(self.lib.qp3d_plan_upload(self.h, *[fptr(a) for a in arrs]))
+        self.s = s
+
+    def reset(self):
+        check(self.lib.qp3d_plan_reset(self.h))
+
+    def run(self, k0=0, k1=None):
+        check(self.lib.qp3d_plan_run(self.h, k0, self.s.nt if k1 is None else k1))
+
+    def sync(self):
+        check(self.lib.qp3d_plan_sync(self.h))
+
+    def forward(self):
+        self.reset()
+        self.run()
+        self.sync()
+        return self.record()
+
+    def record(self):
+        s = self.s
+        out = np.zeros((s.nt, s.hx, s.hy), np.float32)
+        check(self.lib.qp3d_plan_record(self.h, fptr(out.reshape(-1))))
+        return out
+
+    def field(self, which=0):
+        s = self.s
+        out = np.zeros((s.nz, s.nx, s.ny), np.float32)
+        check(self.lib.qp3d_plan_field(self.h, int(which), fptr(out.reshape(-1))))
+        return out
+
+    def snapshots(self):
+        s = self.s
+        res = []
+        for i in range(self.lib.qp3d_plan_snapshot_count(self.h)):
+            out = np.zeros((s.hz, s.hx, s.hy), np.float32)
+            k = C.c_int()
+            check(self.lib.qp3d_plan_snapshot(self.h, i, C.byref(k), fptr(out.reshape(-1))))
+            res.append((k.value, out))
+        return res
+
+    def step_maxima(self):
+        out = np.zeros(self.s.nt, np.float32)
+        check(self.lib.qp3d_plan_step_maxima(self.h, fptr(out)))
+        return out
+
+    def set_instrument(self, on=True):
+        check(self.lib.qp3d_plan_set_instrument(self.h, int(on)))
+
+    def kernel_times(self):
+        a, b, n = C.c_double(), C.c_double(), C.c_long()
+        check(self.lib.qp3d_plan_kernel_times(self.h, C.byref(a), C.byref(b), C.byref(n)))
+        return a.value, b.value, n.value
+
+    def step_bytes(self):
+        a, b = C.c_double(), C.c_double()
+        check(self.lib.qp3d_plan_step_bytes(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if self.h:
+            self.lib.qp3d_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -1,0 +1,134 @@
+/*
+ * qp3d — MI355X-native 3D pure-qP VTI forward modeling (C ABI).
+ *
+ * Grid layout everywhere: padded fields [nz][nx][ny], y unit-stride, linear index
+ * z*nx*ny + x*ny + y, with nx = hx + 2*pml, ny = hy + 2*pml, nz = Nz0 + 2*pml
+ * (array_new.c:2, GPU_kernel.cu:551, Main.c:125). Every pointer argument below is host
+ * memory owned by the caller unless the comment says otherwise.
+ */
+#ifndef QP3D_H
+#define QP3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the legacy void entry point records them in qp3d_last_error) ---- */
+#define QP3D_OK 0
+#define QP3D_ERR_ARG (-1)       /* invalid dimensions, pml, window or pointer */
+#define QP3D_ERR_IO (-2)        /* missing or short input file, unwritable output */
+#define QP3D_ERR_ALLOC (-3)     /* host or device allocation failed */
+#define QP3D_ERR_HIP (-4)       /* a HIP runtime call failed */
+#define QP3D_ERR_NONFINITE (-5) /* the wavefield became inf/NaN (CFL violation) */
+#define QP3D_ERR_STATE (-6)     /* call order violated (e.g. run before upload) */
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char *qp3d_last_error(void);
+/* Internal helper shared by the host sources of the library (printf-style). */
+void qp3d_set_error(const char *fmt, ...);
+
+/* ---- drop-in solver entry point -------------------------------------------------------
+ * Replaces void modeling3d(...) of 3dPureqPModeling_single_shot.h:4-6 (GPU implementation
+ * 3dPureqPModeling_single_shot.cu:26-447, CPU twin modeling3D.c:17-153) with the same
+ * argument list and meaning:
+ *   vp, epsilon, delta, source  padded model and spatial source [nz][nx][ny];
+ *   wavelet[nt]                  source time function;
+ *   record                       out: [nt][nx-2pml][ny-2pml], record[k] = u(sz) after step k;
+ *   dlr..ddfb[pml]               PML profiles (qp3d_pmlcoff); c, c2[36] FD tables (qp3d_fdcoff*);
+ *   snapflag, ntsnap, snap_file  when snapflag == 1 and k % ntsnap == 0, writes
+ *                                <snap_file>k.dat (interior cube) and <snap_file>_{xz,yz,xy}k.dat.
+ * sx, sy are unused (the source position is carried by `source`), as in the reference.
+ * Runs on HIP device 0, synchronously; failures are reported by qp3d_last_error(). */
+void modeling3d(int nx, int ny, int nz, int nt, int ntsnap, float dx, float dy, float dz, float dt, int pml,
+                int snapflag, int sx, int sy, int sz, float *vp, float *epsilon, float *delta, float *source,
+                float *wavelet, float *record, float *dlr, float *ddlr, float *dtb, float *ddtb, float *dfb,
+                float *ddfb, float *c, float *c2, const char *snap_file);
+
+/* Same as modeling3d() but returns a QP3D_* status. No reference counterpart: the
+ * reference's entry point returns void and checks nothing. */
+int qp3d_modeling3d(int nx, int ny, int nz, int nt, int ntsnap, float dx, float dy, float dz, float dt, int pml,
+                    int snapflag, int sx, int sy, int sz, const float *vp, const float *epsilon,
+                    const float *delta, const float *source, const float *wavelet, float *record,
+                    const float *dlr, const float *ddlr, const float *dtb, const float *ddtb, const float *dfb,
+                    const float *ddfb, const float *c, const float *c2, const char *snap_file);
+
+/* ---- host setup math (bit-identical to the reference's tables) ---- */
+void qp3d_fdcoff1(float *c);  /* CPU_function.c:111-128 fdcoff1, c[36] */
+void qp3d_fdcoff2(float *c);  /* CPU_function.c:131-148 fdcoff2, c[36] */
+/* CPU_function.c:150-169 pmlcoff */
+void qp3d_pmlcoff(int pml, float vpmax, float dx, float dy, float dz, float *dlr, float *ddlr, float *dtb,
+                  float *ddtb, float *dfb, float *ddfb);
+/* CPU_function.c:37-64 Source; either output may be NULL to skip it. */
+void qp3d_source(int sx, int sy, int sz, float fm, float amp, float alpha, float dt, float dx, float dy, float dz,
+                 float t0, int nt, int nx, int ny, int nz, float *source, float *wavelet);
+/* CPU_function.c:67-108 extend_model */
+void qp3d_extend_model(const float *in, float *out, int nx, int ny, int nz, int pml);
+/* CPU_function.c:10-22 absMaxval over n cells */
+float qp3d_absmax(const float *a, long n);
+/* CPU_function.c:234-261 readmodel3d; returns a status (the reference checks nothing). */
+int qp3d_readmodel3d(const char *filename, float *a, int ny, int nx, int nz, int fny, int fnx, int fnz, int vny,
+                     int vnx, int vnz);
+/* Synthetic layered VTI model in model-file layout [ny][nx][nz]; no reference counterpart
+ * (the reference's model files are not distributed). */
+int qp3d_genmodel(int nx, int ny, int nz, uint64_t seed, float sigma_cells, float amp_frac, float *vp, float *eps,
+                  float *del);
+
+/* ---- plan API: device-resident runs -------------------------------------------------
+ * No reference counterpart; it splits modeling3d() into allocate / upload / run / fetch so
+ * that a caller can keep the model in HBM across shots, time the loop, or read state. */
+#define QP3D_STORAGE_FP32 0 /* wavefields and PML state in float */
+#define QP3D_STORAGE_BF16 1 /* stored bfloat16, arithmetic float */
+#define QP3D_STORAGE_FP16 2 /* stored half with a power-of-two scale, arithmetic float */
+
+typedef struct qp3d_config {
+  int nx, ny, nz, pml, nt;
+  float dx, dy, dz, dt;
+  int sz;        /* receiver plane, padded z index */
+  int storage;   /* QP3D_STORAGE_* */
+  int use_graph; /* 1: replay a captured hipGraph per step group; 0: direct launches */
+  int snapflag;  /* 1: extract the interior cube when k % ntsnap == 0 */
+  int ntsnap;
+  int zchunk;    /* z planes per workgroup in the stencil kernels, 0 = automatic */
+  int device;    /* HIP device ordinal */
+} qp3d_config;
+
+typedef struct qp3d_plan qp3d_plan;
+
+int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out);
+int qp3d_plan_destroy(qp3d_plan *p);
+/* Uploads model, source and tables (same meaning as the modeling3d() arguments). */
+int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps, const float *del, const float *source,
+                     const float *wavelet, const float *dlr, const float *ddlr, const float *dtb, const float *ddtb,
+                     const float *dfb, const float *ddfb, const float *c, const float *c2);
+/* Zeroes wavefields, PML state, per-step maxima and the device record. */
+int qp3d_plan_reset(qp3d_plan *p);
+/* Enqueues time steps [k0, k1) on the plan's stream (asynchronous). */
+int qp3d_plan_run(qp3d_plan *p, int k0, int k1);
+/* Waits for the plan's stream; returns QP3D_ERR_NONFINITE if the field blew up. */
+int qp3d_plan_sync(qp3d_plan *p);
+/* Copies the device record [nt][hx][hy] to host memory. */
+int qp3d_plan_record(const qp3d_plan *p, float *record);
+/* State after the last step run, padded [nz][nx][ny] floats. which: 0 = u after the last
+ * step (u2), 1 = the step before (u1); 2/3 = w1 newest/older, 4/5 = w3 newest/older,
+ * 6/7 = p newest/older, 8 = w21 (PML state in full-grid layout, zero outside the PML). */
+int qp3d_plan_field(const qp3d_plan *p, int which, float *out);
+/* Number of snapshots held and the copy of snapshot i ([hz][hx][hy]) with its step index. */
+int qp3d_plan_snapshot_count(const qp3d_plan *p);
+int qp3d_plan_snapshot(const qp3d_plan *p, int i, int *step, float *out);
+/* Per-step gradient maxima G[k] (float) of steps [0, nt). */
+int qp3d_plan_step_maxima(const qp3d_plan *p, float *out);
+/* 1: bracket every stencil launch with HIP events on the plan's stream. */
+int qp3d_plan_set_instrument(qp3d_plan *p, int on);
+/* Mean device time (ms) per launch of the gradient-max and the update kernel over the
+ * instrumented steps since the last reset of the counters, and the number of launches. */
+int qp3d_plan_kernel_times(qp3d_plan *p, double *ms_gradmax, double *ms_update, long *launches);
+/* Algorithmic HBM bytes of one time step of the plan (interior 28 B/cell, PML 68 B/cell in FP32),
+ * split into the gradient-max kernel and the update kernel. */
+int qp3d_plan_step_bytes(const qp3d_plan *p, double *bytes_gradmax, double *bytes_update);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QP3D_H */
