@@ -1,0 +1,152 @@
+"""Throughput benchmark of the 3D pure-qP VTI forward-modeling time loop on one MI355X.
+
+Workload (the 512^3 single-GPU configuration): 512x512x512 interior, 32-cell PML on all
+faces (padded 576^3), dx=dy=dz=12.5 m, dt=1 ms, 3000 time steps, 25 Hz Ricker at the surface
+centre, the full 512x512 surface receiver plane, interior snapshots every 500 steps; model =
+four VTI layers with a smooth +-3% random perturbation (seed 1). A bench "step" is one
+complete shot (reset + 3000 time steps) with the model already resident in HBM.
+
+metric: Gcell-updates/s = updated cells (nx-10)(ny-10)(nz-10) x time steps / wall time.
+roofline: the fused update kernel's algorithmic bytes per launch (24 B per interior cell,
+64 B per PML cell) over its mean launch time, from HIP events on the plan's stream.
+cpu_baseline: the CPU oracle (the reference's arithmetic, bit-exact with its CPU kernels)
+on the host's CPU share, on a 20-step window that starts from the GPU's state at step 300.
+
+    python bench.py [--steps K] [--warmup W] [--cpu-steps S] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+for p in (REPO, os.path.join(REPO, "gpu-c_3dqmodeling_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "k_update_576cube_traffic.json")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(setup, plan, start, nsteps):
+    """Time the oracle for nsteps starting from the GPU's wavefield after `start` steps."""
+    import numpy as np
+    from oracle import oracle as O
+
+    plan.reset()
+    plan.run(0, start)
+    plan.sync()
+    fields = [plan.field(w) for w in range(9)]
+    threads = O.default_threads()
+    o = O.OracleState(setup, "f32", nthreads=threads)
+    o.import_state(*fields)
+    del fields
+    t0 = time.perf_counter()
+    for k in range(start, start + nsteps):
+        o.step(setup.wavelet[k])
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": round(setup.n_upd * nsteps / dt / 1e9, 4), "unit": "Gcell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"CPU oracle (FP32, reference arithmetic) on the 576^3 grid, {nsteps} time steps "
+                      f"{start}..{start + nsteps - 1} from the GPU state at step {start}; {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2, help="timed shots")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed shots")
+    ap.add_argument("--nt", type=int, default=3000)
+    ap.add_argument("--n", type=int, default=512, help="interior cells per axis")
+    ap.add_argument("--pml", type=int, default=32)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-start", type=int, default=300)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the shot-parallel and z-slab "
+              f"designs for several GPUs are described, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import qp3d
+
+    t = time.perf_counter()
+    s = qp3d.config_setup(hx=args.n, hy=args.n, nz0=args.n, pml=args.pml, nt=args.nt, seed=1, fm=25.0, src_depth=0)
+    log(f"setup {s.nx}x{s.ny}x{s.nz} in {time.perf_counter() - t:.1f} s")
+    ntsnap = 500
+    plan = qp3d.Plan(s, snapflag=1, ntsnap=ntsnap, use_graph=bool(args.graph))
+    log("model uploaded")
+    for i in range(args.warmup):
+        t = time.perf_counter()
+        plan.forward_async()
+        plan.sync()
+        log(f"warmup shot {i}: {time.perf_counter() - t:.3f} s")
+    plan.set_instrument(True)
+    plan.sync()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        plan.forward_async()
+        plan.sync()  # also harvests this shot's kernel events
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    ms_grad, ms_upd, launches = plan.kernel_times()
+    b_grad, b_upd = plan.step_bytes()
+    plan.set_instrument(False)
+    log(f"timed {args.steps} shots in {wall:.3f} s; gradmax {ms_grad:.4f} ms, update {ms_upd:.4f} ms "
+        f"per launch over {launches} launches")
+
+    cells = s.n_upd * s.nt * args.steps
+    value = cells / wall / 1e9
+    achieved = b_upd / (ms_upd * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        try:
+            with open(TRAFFIC_FILE) as f:
+                tr = json.load(f)
+            if tr.get("nx") == s.nx and tr.get("storage", "fp32") == "fp32":
+                traffic = tr.get("bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+    cpu = None
+    if not args.no_cpu and args.cpu_steps > 0:
+        cpu = cpu_baseline(s, plan, min(args.cpu_start, s.nt - args.cpu_steps), args.cpu_steps)
+        log(f"cpu baseline {cpu}")
+    plan.close()
+
+    out = {
+        "metric": "Gcell-updates/s (512^3 FP32 forward, 3000 steps)",
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (layered VTI + smooth random perturbation, seed 1)",
+        "config": {"workload": "3D pure-qP VTI forward shot, 512^3 interior, pml 32 (576^3 padded), "
+                               "3000 steps, dt 1 ms, dx 12.5 m, 25 Hz Ricker, 512x512 receivers, "
+                               "snapshots every 500 steps",
+                   "padded": [s.nx, s.ny, s.nz], "nt": s.nt, "n_upd": s.n_upd, "n_int": s.n_int,
+                   "gcells_int_per_s": round(s.n_int * s.nt * args.steps / wall / 1e9, 3),
+                   "ms_per_time_step": round(wall / args.steps / s.nt * 1e3, 5),
+                   "ms_gradmax": round(ms_grad, 5), "ms_update": round(ms_upd, 5),
+                   "hbm_frac_step": round((b_grad + b_upd) * s.nt * args.steps / wall / 1e9 / HBM_PEAK_GBS, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -323,7 +323,7 @@ __global__ __launch_bounds__(NTHREADS) void k_update(UpdArgs<S> a)
           y <= a.sb_y1) {
         const int bx = a.sb_x1 - a.sb_x0 + 1, by = a.sb_y1 - a.sb_y0 + 1;
         const float s = a.sbox[((long)(z - a.sb_z0) * bx + (x - a.sb_x0)) * by + (y - a.sb_y0)];
-        un = un + __fmul_rn(s, a.wk);
+        un = fmaf(s, a.wk, un);  // u3 += source*wavelet, contracted as the CUDA build does
       }
       un = S::rnd(un);
       S::st(a.unext, i, un);
@@ -351,7 +351,7 @@ __global__ void k_rim_source(const typename S::T *ucur, typename S::T *unext, Ge
     const bool r5 = x >= H && x < g.nx - H && y >= H && y < g.ny - H && z >= H && z < g.nz - H;
     if (r5) continue;
     const long i = (long)z * g.pln + (long)x * g.ny + y;
-    const float v = S::ld(ucur, i) + __fmul_rn(sbox[t], wk);
+    const float v = fmaf(sbox[t], wk, S::ld(ucur, i));
     S::st(unext, i, S::rnd(v));
   }
 }

@@ -81,6 +81,7 @@ void qp3d_source(int sx, int sy, int sz, float fm, float amp, float alpha, float
                  float t0, int nt, int nx, int ny, int nz, float *source, float *wavelet)
 {
   if (source) {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nz; i++)
       for (int j = 0; j < nx; j++)
         for (int k = 0; k < ny; k++) {
@@ -176,21 +177,24 @@ int qp3d_readmodel3d(const char *filename, float *a, int ny, int nx, int nz, int
  * Four equal-thickness horizontal layers over nz (layer = z*4/nz) with
  *   vp 2000/2500/3000/3500 m/s, epsilon 0/0.10/0.15/0.20, delta 0/0.05/0.08/0.10,
  * each field multiplied by (1 + 0.03 g), g an independent smooth random field per field
- * (drawn in the order vp, epsilon, delta from one SplitMix64 stream; Box-Muller normals;
+ * (drawn in the order vp, epsilon, delta from one SplitMix64 stream, cells in file order,
+ * one Box-Muller pair per two cells;
  * separable Gaussian filter with sigma = 5 cells truncated at 4 sigma, half-sample
  * reflection at the edges; scaled to max|g| = 1), then delta <- min(delta, epsilon).
  * Output arrays are in the model-file layout [ny][nx][nz] (z fastest). */
-static uint64_t splitmix64(uint64_t *s)
+/* Draw d (0-based) of the SplitMix64 stream seeded with s: the generator's state after d+1
+ * increments, mixed. Counter-based, so the stream can be filled in parallel. */
+static uint64_t splitmix64_at(uint64_t s, uint64_t d)
 {
-  uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+  uint64_t z = s + (d + 1) * 0x9E3779B97F4A7C15ULL;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
   return z ^ (z >> 31);
 }
 
-static double unit_open(uint64_t *s) /* uniform in (0, 1] */
+static double unit_open(uint64_t s, uint64_t d) /* uniform in (0, 1] */
 {
-  return ((double)(splitmix64(s) >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+  return ((double)(splitmix64_at(s, d) >> 11) + 1.0) * (1.0 / 9007199254740992.0);
 }
 
 static int reflect_index(int i, int n)
@@ -219,21 +223,23 @@ static void gauss_smooth3(double *g, int ny, int nx, int nz, double sigma)
     long nlines = (long)ny * nx * nz / n;
 #pragma omp parallel
     {
-      double *in = (double *)malloc(sizeof(double) * nmax);
+      double *ext = (double *)malloc(sizeof(double) * (nmax + 2 * rad));
 #pragma omp for schedule(static)
       for (long l = 0; l < nlines; l++) {
         long base;
         if (axis == 0) base = l * nz;
         else if (axis == 1) base = (l / nz) * sy + (l % nz);
         else base = l;
-        for (int i = 0; i < n; i++) in[i] = g[base + i * stride];
+        /* line with rad reflected samples on each side */
+        for (int i = -rad; i < n + rad; i++) ext[i + rad] = g[base + reflect_index(i, n) * stride];
         for (int i = 0; i < n; i++) {
+          const double *e = ext + i;
           double acc = 0;
-          for (int k = -rad; k <= rad; k++) acc += w[k + rad] * in[reflect_index(i + k, n)];
+          for (int k = 0; k <= 2 * rad; k++) acc += w[k] * e[k];
           g[base + i * stride] = acc;
         }
       }
-      free(in);
+      free(ext);
     }
   }
   free(w);
@@ -250,15 +256,18 @@ int qp3d_genmodel(int nx, int ny, int nz, uint64_t seed, float sigma_cells, floa
     qp3d_set_error("genmodel: out of host memory for %ld cells", n);
     return QP3D_ERR_ALLOC;
   }
-  uint64_t st = seed;
   float *outs[3] = {vp, eps, del};
   const double *lay[3] = {lv, le, ld};
+  const long npair = (n + 1) / 2;
   for (int f = 0; f < 3; f++) {
-    for (long i = 0; i < n; i += 2) {
-      double r = sqrt(-2.0 * log(unit_open(&st)));
-      double th = 2.0 * QP3D_PI * unit_open(&st);
-      g[i] = r * cos(th);
-      if (i + 1 < n) g[i + 1] = r * sin(th);
+    /* pair j of field f uses draws 2*(f*npair + j) and 2*(f*npair + j) + 1 */
+#pragma omp parallel for schedule(static)
+    for (long j = 0; j < npair; j++) {
+      const uint64_t d = 2 * ((uint64_t)f * npair + j);
+      double r = sqrt(-2.0 * log(unit_open(seed, d)));
+      double th = 2.0 * QP3D_PI * unit_open(seed, d + 1);
+      g[2 * j] = r * cos(th);
+      if (2 * j + 1 < n) g[2 * j + 1] = r * sin(th);
     }
     if (sigma_cells > 0) gauss_smooth3(g, ny, nx, nz, sigma_cells);
     double gm = 0;

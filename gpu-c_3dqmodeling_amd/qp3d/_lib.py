@@ -43,6 +43,7 @@ class Config(C.Structure):
 # (name, restype, argtypes)
 _SIGS = [
     ("qp3d_last_error", C.c_char_p, []),
+    ("qp3d_set_error", None, [C.c_char_p]),
     ("modeling3d", None, [C.c_int] * 5 + [C.c_float] * 4 + [C.c_int] * 5 + [_F] * 14 + [C.c_char_p]),
     ("qp3d_modeling3d", C.c_int, [C.c_int] * 5 + [C.c_float] * 4 + [C.c_int] * 5 + [_F] * 14 + [C.c_char_p]),
     ("qp3d_fdcoff1", None, [_F]),

@@ -55,9 +55,13 @@ class Plan:
     def sync(self):
         check(self.lib.qp3d_plan_sync(self.h))
 
-    def forward(self):
+    def forward_async(self):
+        """Enqueue one full shot from rest (reset + all time steps) without waiting."""
         self.reset()
         self.run()
+
+    def forward(self):
+        self.forward_async()
         self.sync()
         return self.record()
 

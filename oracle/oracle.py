@@ -141,6 +141,42 @@ class _StateBase:
     def set_aux(self, face, arr, a):
         self.aux(face, arr)[...] = a
 
+    def import_state(self, u_new, u_old, w1_new, w1_old, w3_new, w3_old, p_new, p_old, w21):
+        """Load a mid-run state given in the full-grid layout (PML state held at the cells each
+        face owns: x faces first, then y, then z) into the per-face arrays, as it stands after
+        the reference's exchange: u2 = u3 = newest, u1 = older, w12/w32/p2 = newest,
+        w11/w31/p1 = older, w21 = accumulator."""
+        s = self.s
+        nx, ny, nz, pml = s.nx, s.ny, s.nz, s.pml
+        self.set_field("u2", u_new)
+        self.set_field("u3", u_new)
+        self.set_field("u1", u_old)
+        xs = slice(pml, nx - pml)
+        ys = slice(pml, ny - pml)
+
+        def face_view(a, face):
+            if face == "L":
+                return a[:, :pml, :].copy()
+            if face == "R":
+                return a[:, nx - pml:, :].copy()
+            if face in ("F", "Bk"):
+                v = a[:, :, :pml] if face == "F" else a[:, :, ny - pml:]
+                v = v.copy()
+                v[:, :pml, :] = 0
+                v[:, nx - pml:, :] = 0
+                return v
+            v = (a[:pml] if face == "T" else a[nz - pml:]).copy()
+            m = np.ones(v.shape[1:], bool)
+            m[xs, ys] = False
+            v[:, m] = 0
+            return v
+
+        pairs = [("w12", w1_new), ("w11", w1_old), ("w32", w3_new), ("w31", w3_old), ("p2", p_new),
+                 ("p1", p_old), ("w21", w21)]
+        for face in FACES:
+            for arr, src in pairs:
+                self.set_aux(face, arr, face_view(np.asarray(src), face).reshape(-1))
+
 
 class OracleState(_StateBase):
     """The oracle's state machine: step(w) = grad, max, S, update, source, exchange."""
