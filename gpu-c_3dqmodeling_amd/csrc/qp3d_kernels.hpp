@@ -1,57 +1,95 @@
 // gfx950 kernels of the qP3D time step.
 //
 // One time step of the reference (3dPureqPModeling_single_shot.cu:277-337) launches five
-// per-cell kernels and copies 16 B/cell over PCIe. Here a step is two stencil passes over
-// the wavefield, both hand-tiled for CDNA4:
-//   k_gradmax  max over the grid of |du/dx|,|du/dy|,|du/dz| (GPU_kernel.cu:14-33 grad and the
-//              host absMaxval at 3dPureqPModeling_single_shot.cu:284-292), reduced on device;
-//   k_update   gradient (recomputed in registers), scalar operator S (GPU_kernel.cu:87-115),
-//              10th-order second derivatives, interior leapfrog or split-field PML update
-//              (GPU_kernel.cu:534-639), source injection (GPU_kernel.cu:720-731) and the
-//              receiver-plane write (3dPureqPModeling_single_shot.cu:333-337), in one pass.
-// The copy-rotation kernel (GPU_kernel.cu:733-812) disappears: u_next and the PML state
-// are written in place of the oldest slot (each cell reads that slot only at its own index)
-// and the host flips slot parity each step.
+// per-cell kernels and copies 16 B/cell over PCIe. Here a step is three launches of two
+// stencil passes over the wavefield, hand-tiled for CDNA4:
+//   k_gradmax        max over the grid of |du/dx|,|du/dy|,|du/dz| (GPU_kernel.cu:14-33 grad and
+//                    the host absMaxval at 3dPureqPModeling_single_shot.cu:284-292), reduced on
+//                    device;
+//   k_update<PML=0>  interior cells: gradient recomputed in registers, scalar operator S
+//                    (GPU_kernel.cu:87-115), 10th-order second derivatives and the leapfrog
+//                    (GPU_kernel.cu:552-565), source injection (GPU_kernel.cu:720-731) and the
+//                    receiver-plane write (3dPureqPModeling_single_shot.cu:333-337);
+//   k_update<PML=1>  the same for the tiles that hold PML cells, with the split-field
+//                    recursions (GPU_kernel.cu:566-637).
+// Splitting interior and PML tiles keeps the PML state out of the interior kernel's
+// registers. The copy-rotation kernel (GPU_kernel.cu:733-812) disappears: u_next and the PML
+// state are written in place of the oldest slot (each cell reads that slot only at its own
+// index) and the host flips slot parity each step.
 //
-// Tiling: a workgroup owns a TY x TX column of the xy plane (y unit-stride, one 64-lane
-// wavefront per x row) and marches through a chunk of z planes. The current plane with a
-// 5-cell halo sits in LDS for the x and y derivatives; each lane keeps an 11-deep register
-// queue of its column in z for the z derivatives, so every u value is fetched from HBM
-// once per pass plus halo reuse that L2/MALL serves.
+// Tiling: a workgroup owns an xy tile (at most TX x 64 cells, y unit-stride, one 64-lane
+// wavefront per x row group) and marches through a z range; its work item comes from a list
+// the host builds (tiles aligned to the PML boundary, z ranges split at the PML planes,
+// ordered so that each XCD's L2 sees neighbouring tiles). The current plane with a 5-cell
+// halo sits in LDS (double-buffered, one barrier per plane) for the x and y derivatives; each
+// lane keeps an 11-deep register queue of its column in z for the z derivatives. Every global
+// load of plane z+1 (queue front, halos, model, previous wavefield, PML state) is issued
+// before plane z is computed, so HBM latency overlaps the arithmetic of the plane before.
+// Cell offsets are 32-bit (grids below 2^30 cells) so loads use a scalar base plus a vector
+// offset.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace qp3d {
 
-constexpr int H = 5;          // stencil half-width (10th order)
-constexpr int TY = 64;        // tile extent along y = wavefront width
-constexpr int WX = 4;         // wavefronts per workgroup, stacked along x
-constexpr int RX = 4;         // x rows per lane
-constexpr int TX = WX * RX;   // tile extent along x
-constexpr int LX = TX + 2 * H;
-constexpr int LY = TY + 2 * H;
-constexpr int NTHREADS = TY * WX;
+constexpr int H = 5;    // stencil half-width (10th order)
+constexpr int TY = 64;  // tile extent along y = wavefront width
+
+// Tile shape: WX wavefronts stacked along x, RX x-rows per lane.
+template <int WX_, int RX_>
+struct Tile {
+  static constexpr int WX = WX_, RX = RX_;
+  static constexpr int TX = WX * RX;
+  static constexpr int LX = TX + 2 * H, LY = TY + 2 * H;
+  static constexpr int NT = TY * WX;
+  static constexpr int NXH = (2 * H + WX - 1) / WX;  // x-halo rows per lane
+  static_assert(TX * 2 * H <= NT, "one y-halo value per lane");
+};
+
+template <class P>
+__device__ __forceinline__ P *at(P *p, unsigned byte_off)
+{
+  return reinterpret_cast<P *>(reinterpret_cast<uintptr_t>(p) + (uintptr_t)byte_off);
+}
 
 // Storage policies for wavefields and PML state. Arithmetic is always float.
 struct StoreF32 {
   using T = float;
-  static __device__ __forceinline__ float ld(const T *p, long i) { return p[i]; }
-  static __device__ __forceinline__ void st(T *p, long i, float v) { p[i] = v; }
+  static __device__ __forceinline__ float ld(const T *p, unsigned i) { return *at(p, i << 2); }
+  static __device__ __forceinline__ float ld_stream(const T *p, unsigned i)
+  {
+    return __builtin_nontemporal_load(at(p, i << 2));
+  }
+  static __device__ __forceinline__ void st(T *p, unsigned i, float v) { __builtin_nontemporal_store(v, at(p, i << 2)); }
   static __device__ __forceinline__ float rnd(float v) { return v; }
 };
 
+__device__ __forceinline__ float ld_model(const float *p, unsigned i) { return __builtin_nontemporal_load(at(p, i << 2)); }
+
 struct Geo {
   int nx, ny, nz, pml;
-  long pln;               // nx*ny
-  int tiles_y, tiles_x;   // xy tiles over [5,n-5)
-  int zchunk;             // z planes per workgroup
-  int nlog;               // tiles_y*tiles_x*z chunks; the grid is this rounded up to a multiple of 8
+  unsigned pln;  // nx*ny
 };
 
+__device__ __forceinline__ unsigned cell(const Geo &g, int z, int x, int y)
+{
+  return (unsigned)z * g.pln + (unsigned)x * (unsigned)g.ny + (unsigned)y;
+}
+
+// Work item of one workgroup: x in [x0,x1), y in [y0,y1), z in [z0,z1); x1-x0 <= TX, y1-y0 <= 64.
+// Packed as {x0 | x1<<16, y0 | y1<<16, z0, z1}; z0 == z1 marks padding.
+__device__ __forceinline__ void unpack(int4 w, int &x0, int &x1, int &y0, int &y1, int &z0, int &z1)
+{
+  x0 = w.x & 0xffff; x1 = (unsigned)w.x >> 16;
+  y0 = w.y & 0xffff; y1 = (unsigned)w.y >> 16;
+  z0 = w.z; z1 = w.w;
+}
+
 struct Coef {
-  float cxx[H], cyy[H], czz[H];  // c2[31+m-1] / (d*d)
-  float cx[H], cy[H], cz[H];     // c1[31+m-1] / d
+  float c1[H], c2[H];        // 10th-order rows of the FD tables (entries 31..35)
+  float ix, iy, iz;          // 1/dx, 1/dy, 1/dz
+  float ixx, iyy, izz;       // 1/dx^2, 1/dy^2, 1/dz^2
   float dt;
 };
 
@@ -74,6 +112,7 @@ struct UpdArgs {
   const float *sbox;                        // dense source values over the box
   float *rec;                               // record row of this step, [hx][hy]
   int sz;                                   // receiver plane
+  const int4 *work;                         // one item per workgroup
 };
 
 template <class S>
@@ -83,115 +122,135 @@ struct GradArgs {
   const typename S::T *ucur;
   unsigned *gmax;   // out: atomic max of the gradient components (float bits)
   int *flag;        // out: set to 1 when a non-finite gradient is seen
+  const int4 *work;
 };
 
-// Map the launch's workgroup id to (tile, zchunk). Workgroups are dispatched round-robin
-// over the 8 XCDs; consecutive ids therefore land on different L2s. The remap hands each XCD
-// a contiguous range of z-chunk-major tile ids so that the halo rows a neighbour needs are
-// read through the same L2.
-// Returns false for the padding workgroups past g.nlog.
-__device__ __forceinline__ bool tile_of(const Geo &g, int &ty0, int &tx0, int &z0)
+// Global loads of u for one plane of a tile: the lane's queue front (plane z+5) and its share
+// of the x and y halos of plane z.
+template <class S, class T>
+struct PlaneLoad {
+  float qn[T::RX];
+  float hx[T::NXH];
+  float hy;
+
+  __device__ __forceinline__ void issue(const typename S::T *u, const Geo &g, int z, int x0, int y0, int tx, int ty)
+  {
+    const int y = y0 + ty;
+    const bool yin = y < g.ny;
+#pragma unroll
+    for (int r = 0; r < T::RX; r++) {
+      const int x = x0 + tx + T::WX * r;
+      qn[r] = (yin && x < g.nx) ? S::ld(u, cell(g, z + H, x, y)) : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < T::NXH; i++) {
+      const int h = tx + T::WX * i;
+      const int X = h < H ? h : T::TX + h;
+      const int gx = x0 - H + X;
+      hx[i] = (h < 2 * H && yin && gx < g.nx) ? S::ld(u, cell(g, z, gx, y)) : 0.f;
+    }
+    const int t = tx * TY + ty;
+    hy = 0.f;
+    if (t < T::TX * 2 * H) {
+      const int row = t / (2 * H), col = t - row * (2 * H);
+      const int gy = y0 - H + (col < H ? col : TY + col), gx = x0 + row;
+      if (gy < g.ny && gx < g.nx) hy = S::ld(u, cell(g, z, gx, gy));
+    }
+  }
+
+  // Write plane z (centre values from the queue, halos from this load) into an LDS buffer.
+  __device__ __forceinline__ void stage(float (*sm)[T::LY], const float (&q)[T::RX][2 * H + 1], int tx, int ty) const
+  {
+#pragma unroll
+    for (int r = 0; r < T::RX; r++) sm[H + tx + T::WX * r][H + ty] = q[r][H];
+#pragma unroll
+    for (int i = 0; i < T::NXH; i++) {
+      const int h = tx + T::WX * i;
+      if (h < 2 * H) sm[h < H ? h : T::TX + h][H + ty] = hx[i];
+    }
+    const int t = tx * TY + ty;
+    if (t < T::TX * 2 * H) {
+      const int row = t / (2 * H), col = t - row * (2 * H);
+      sm[H + row][col < H ? col : TY + col] = hy;
+    }
+  }
+};
+
+template <class S, class T>
+__device__ __forceinline__ void prime_queue(float (&q)[T::RX][2 * H + 1], const typename S::T *u, const Geo &g, int z0,
+                                            int x0, int y, int tx)
 {
-  const int nblk = gridDim.x;  // multiple of 8
-  const int b = blockIdx.x;
-  const int l = (b & 7) * (nblk >> 3) + (b >> 3);
-  if (l >= g.nlog) return false;
-  const int ntile = g.tiles_x * g.tiles_y;
-  const int zc = l / ntile;
-  const int t = l - zc * ntile;
-  const int txi = t / g.tiles_y;
-  const int tyi = t - txi * g.tiles_y;
-  ty0 = H + tyi * TY;
-  tx0 = H + txi * TX;
-  z0 = H + zc * g.zchunk;
-  return true;
+#pragma unroll
+  for (int r = 0; r < T::RX; r++) {
+    const int x = x0 + tx + T::WX * r;
+#pragma unroll
+    for (int j = 0; j < 2 * H; j++) q[r][j] = (y < g.ny && x < g.nx) ? S::ld(u, cell(g, z0 - H + j, x, y)) : 0.f;
+  }
 }
 
-// Stage the current plane of u (queue centre + x/y halos) into LDS.
-template <class S>
-__device__ __forceinline__ void stage_plane(float (*sm)[LY], const typename S::T *u, const Geo &g, float (&q)[RX][2 * H + 1],
-                                            int z, int x0, int y0, int tx, int ty)
+template <class T>
+__device__ __forceinline__ void shift_queue(float (&q)[T::RX][2 * H + 1], const float (&qn)[T::RX])
 {
-  const int y = y0 + ty;
-  const bool yin = y < g.ny;
 #pragma unroll
-  for (int r = 0; r < RX; r++) sm[H + tx + WX * r][H + ty] = q[r][H];
-  const long zb = (long)z * g.pln;
-  // x halo: 2H rows of TY lanes
+  for (int r = 0; r < T::RX; r++) {
 #pragma unroll
-  for (int h = tx; h < 2 * H; h += WX) {
-    const int X = h < H ? h : TX + h;
-    const int gx = x0 - H + X;
-    float v = 0.f;
-    if (yin && gx < g.nx) v = S::ld(u, zb + (long)gx * g.ny + y);
-    sm[X][H + ty] = v;
-  }
-  // y halo: TX rows of 2H columns
-  const int t = tx * TY + ty;
-  if (t < TX * 2 * H) {
-    const int row = t / (2 * H), col = t - row * (2 * H);
-    const int Y = col < H ? col : TY + col;
-    const int gy = y0 - H + Y, gx = x0 + row;
-    float v = 0.f;
-    if (gy < g.ny && gx < g.nx) v = S::ld(u, zb + (long)gx * g.ny + gy);
-    sm[H + row][Y] = v;
+    for (int j = 0; j < 2 * H; j++) q[r][j] = q[r][j + 1];
+    q[r][2 * H] = qn[r];
   }
 }
 
-template <class S>
-__global__ __launch_bounds__(NTHREADS) void k_gradmax(GradArgs<S> a)
+template <class S, class T>
+__global__ __launch_bounds__(T::NT) void k_gradmax(GradArgs<S> a)
 {
-  __shared__ float sm[LX][LY];
-  __shared__ float red[WX];
+  __shared__ float sm[2][T::LX][T::LY];
+  __shared__ float red[T::WX];
   const Geo &g = a.g;
   const int ty = threadIdx.x, tx = threadIdx.y;
-  int y0, x0, z0;
-  if (!tile_of(g, y0, x0, z0)) return;
-  const int z1 = min(z0 + g.zchunk, g.nz - H);
+  int x0, x1, y0, y1, z0, z1;
+  unpack(a.work[blockIdx.x], x0, x1, y0, y1, z0, z1);
+  if (z0 >= z1) return;
   const int y = y0 + ty;
-  const bool yin = y < g.ny, ycmp = y < g.ny - H;
-  float q[RX][2 * H + 1];
+  const bool ycmp = y < y1;
+  float q[T::RX][2 * H + 1];
+  prime_queue<S, T>(q, a.ucur, g, z0, x0, y, tx);
+  PlaneLoad<S, T> L;
+  L.issue(a.ucur, g, z0, x0, y0, tx, ty);
 #pragma unroll
-  for (int r = 0; r < RX; r++) {
-    const int x = x0 + tx + WX * r;
-#pragma unroll
-    for (int j = 0; j < 2 * H; j++)
-      q[r][j] = (yin && x < g.nx && z0 < z1) ? S::ld(a.ucur, (long)(z0 - H + j) * g.pln + (long)x * g.ny + y) : 0.f;
-  }
+  for (int r = 0; r < T::RX; r++) q[r][2 * H] = L.qn[r];
+  L.stage(sm[0], q, tx, ty);
+  __syncthreads();
   float m = 0.f;
   int bad = 0;
+  int b = 0;
   for (int z = z0; z < z1; z++) {
+    const bool more = z + 1 < z1;
+    if (more) L.issue(a.ucur, g, z + 1, x0, y0, tx, ty);
+    float (*s)[T::LY] = sm[b];
 #pragma unroll
-    for (int r = 0; r < RX; r++) {
-      const int x = x0 + tx + WX * r;
-      q[r][2 * H] = (yin && x < g.nx) ? S::ld(a.ucur, (long)(z + H) * g.pln + (long)x * g.ny + y) : 0.f;
-    }
-    __syncthreads();
-    stage_plane<S>(sm, a.ucur, g, q, z, x0, y0, tx, ty);
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < RX; r++) {
-      const int X = H + tx + WX * r, Y = H + ty;
-      const int x = x0 + tx + WX * r;
+    for (int r = 0; r < T::RX; r++) {
+      const int X = H + tx + T::WX * r, Y = H + ty;
+      const int x = x0 + tx + T::WX * r;
       float ux = 0.f, uy = 0.f, uz = 0.f;
 #pragma unroll
       for (int k = 1; k <= H; k++) {
-        ux = fmaf(a.c.cx[k - 1], sm[X + k][Y] - sm[X - k][Y], ux);
-        uy = fmaf(a.c.cy[k - 1], sm[X][Y + k] - sm[X][Y - k], uy);
-        uz = fmaf(a.c.cz[k - 1], q[r][H + k] - q[r][H - k], uz);
+        ux = fmaf(a.c.c1[k - 1], s[X + k][Y] - s[X - k][Y], ux);
+        uy = fmaf(a.c.c1[k - 1], s[X][Y + k] - s[X][Y - k], uy);
+        uz = fmaf(a.c.c1[k - 1], q[r][H + k] - q[r][H - k], uz);
       }
-      if (ycmp && x < g.nx - H) {
-        const float ax = fabsf(ux), ay = fabsf(uy), az = fabsf(uz);
+      if (ycmp && x < x1) {
+        const float ax = fabsf(ux * a.c.ix), ay = fabsf(uy * a.c.iy), az = fabsf(uz * a.c.iz);
         m = (ax >= m) ? ax : m;
         m = (ay >= m) ? ay : m;
         m = (az >= m) ? az : m;
         bad |= !(ax <= 3.4e38f) | !(ay <= 3.4e38f) | !(az <= 3.4e38f);
       }
     }
-#pragma unroll
-    for (int r = 0; r < RX; r++)
-#pragma unroll
-      for (int j = 0; j < 2 * H; j++) q[r][j] = q[r][j + 1];
+    if (more) {
+      shift_queue<T>(q, L.qn);
+      L.stage(sm[b ^ 1], q, tx, ty);
+      __syncthreads();
+      b ^= 1;
+    }
   }
   // workgroup max -> one atomic per workgroup
 #pragma unroll
@@ -202,75 +261,121 @@ __global__ __launch_bounds__(NTHREADS) void k_gradmax(GradArgs<S> a)
   if (ty == 0) red[tx] = m;
   __syncthreads();
   if (tx == 0 && ty == 0) {
-    float b = red[0];
+    float v = red[0];
 #pragma unroll
-    for (int w = 1; w < WX; w++) b = (red[w] >= b) ? red[w] : b;
-    atomicMax(a.gmax, __float_as_uint(b));
+    for (int w = 1; w < T::WX; w++) v = (red[w] >= v) ? red[w] : v;
+    atomicMax(a.gmax, __float_as_uint(v));
   }
   if (bad) atomicOr(a.flag, 1);
 }
 
-template <class S>
-__global__ __launch_bounds__(NTHREADS) void k_update(UpdArgs<S> a)
+// Per-cell inputs of k_update other than u_cur: model, and either u_prev (interior cells)
+// or the seven PML state values (PML cells).
+template <class S, class T, bool PML>
+struct CellLoad {
+  static constexpr int NS = PML ? 7 : 1;
+  float v2[T::RX], e[T::RX], ed[T::RX];
+  float st[T::RX][NS];  // interior: st[0] = u_prev; PML: w1p w1c pp pc w3p w3c w21
+
+  __device__ __forceinline__ void issue(const UpdArgs<S> &a, int z, int x0, int x1, int y, bool ycmp, int tx)
+  {
+    const Geo &g = a.g;
+#pragma unroll
+    for (int r = 0; r < T::RX; r++) {
+      const int x = x0 + tx + T::WX * r;
+      const unsigned i = cell(g, z, x, y);
+      const bool cmp = ycmp && x < x1;
+      v2[r] = cmp ? ld_model(a.vdt2, i) : 0.f;
+      e[r] = cmp ? ld_model(a.eps, i) : 0.f;
+      ed[r] = cmp ? ld_model(a.epsd, i) : 0.f;
+      if constexpr (!PML) {
+        st[r][0] = cmp ? S::ld_stream(a.unext, i) : 0.f;
+      } else {
+        const bool inner = x >= g.pml && x < g.nx - g.pml && y >= g.pml && y < g.ny - g.pml && z >= g.pml &&
+                           z < g.nz - g.pml;
+        if (cmp && inner) {
+          st[r][0] = S::ld_stream(a.unext, i);
+#pragma unroll
+          for (int j = 1; j < NS; j++) st[r][j] = 0.f;
+        } else if (cmp) {
+          st[r][0] = S::ld_stream(a.w1p, i);
+          st[r][1] = S::ld_stream(a.w1c, i);
+          st[r][2] = S::ld_stream(a.pp, i);
+          st[r][3] = S::ld_stream(a.pc, i);
+          st[r][4] = S::ld_stream(a.w3p, i);
+          st[r][5] = S::ld_stream(a.w3c, i);
+          st[r][6] = S::ld_stream(a.w21, i);
+        } else {
+#pragma unroll
+          for (int j = 0; j < NS; j++) st[r][j] = 0.f;
+        }
+      }
+    }
+  }
+};
+
+template <class S, class T, bool PML>
+__global__ __launch_bounds__(T::NT) void k_update(UpdArgs<S> a)
 {
-  __shared__ float sm[LX][LY];
+  __shared__ float sm[2][T::LX][T::LY];
   const Geo &g = a.g;
   const int ty = threadIdx.x, tx = threadIdx.y;
-  int y0, x0, z0;
-  if (!tile_of(g, y0, x0, z0)) return;
-  const int z1 = min(z0 + g.zchunk, g.nz - H);
+  int x0, x1, y0, y1, z0, z1;
+  unpack(a.work[blockIdx.x], x0, x1, y0, y1, z0, z1);
+  if (z0 >= z1) return;
   const int y = y0 + ty;
-  const bool yin = y < g.ny, ycmp = y < g.ny - H;
+  const bool ycmp = y < y1;
   const int nx = g.nx, ny = g.ny, nz = g.nz, pml = g.pml;
+  const bool yint = y >= pml && y < ny - pml;
 
   // S threshold: |grad u| > 0.01*G compared in double (GPU_kernel.cu:99); for a float
   // magnitude this equals mode > rd(0.01*G).
   const float G = __uint_as_float(*a.gmax);
   const float thr = __double2float_rd(1.0e-02 * (double)G);
-  const bool yint = y >= pml && y < ny - pml;
 
-  float q[RX][2 * H + 1];
+  float q[T::RX][2 * H + 1];
+  prime_queue<S, T>(q, a.ucur, g, z0, x0, y, tx);
+  PlaneLoad<S, T> L;
+  L.issue(a.ucur, g, z0, x0, y0, tx, ty);
+  CellLoad<S, T, PML> C, Cn;
+  C.issue(a, z0, x0, x1, y, ycmp, tx);
 #pragma unroll
-  for (int r = 0; r < RX; r++) {
-    const int x = x0 + tx + WX * r;
-#pragma unroll
-    for (int j = 0; j < 2 * H; j++)
-      q[r][j] = (yin && x < nx && z0 < z1) ? S::ld(a.ucur, (long)(z0 - H + j) * g.pln + (long)x * ny + y) : 0.f;
-  }
+  for (int r = 0; r < T::RX; r++) q[r][2 * H] = L.qn[r];
+  L.stage(sm[0], q, tx, ty);
+  __syncthreads();
+  int b = 0;
   for (int z = z0; z < z1; z++) {
-#pragma unroll
-    for (int r = 0; r < RX; r++) {
-      const int x = x0 + tx + WX * r;
-      q[r][2 * H] = (yin && x < nx) ? S::ld(a.ucur, (long)(z + H) * g.pln + (long)x * ny + y) : 0.f;
+    const bool more = z + 1 < z1;
+    if (more) {
+      L.issue(a.ucur, g, z + 1, x0, y0, tx, ty);
+      Cn.issue(a, z + 1, x0, x1, y, ycmp, tx);
     }
-    __syncthreads();
-    stage_plane<S>(sm, a.ucur, g, q, z, x0, y0, tx, ty);
-    __syncthreads();
+    float (*s)[T::LY] = sm[b];
     const bool zint = z >= pml && z < nz - pml;
 #pragma unroll
-    for (int r = 0; r < RX; r++) {
-      const int X = H + tx + WX * r, Y = H + ty;
-      const int x = x0 + tx + WX * r;
-      if (!(ycmp && x < nx - H)) continue;
-      const long i = (long)z * g.pln + (long)x * ny + y;
+    for (int r = 0; r < T::RX; r++) {
+      const int X = H + tx + T::WX * r, Y = H + ty;
+      const int x = x0 + tx + T::WX * r;
+      if (!(ycmp && x < x1)) continue;
+      const unsigned i = cell(g, z, x, y);
       const float c0 = q[r][H];
       const float c2 = c0 + c0;
       float Uxx = 0.f, Uyy = 0.f, Uzz = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
 #pragma unroll
       for (int k = 1; k <= H; k++) {
-        const float ax = sm[X + k][Y], bx = sm[X - k][Y];
-        const float ay = sm[X][Y + k], by = sm[X][Y - k];
+        const float ax = s[X + k][Y], bx = s[X - k][Y];
+        const float ay = s[X][Y + k], by = s[X][Y - k];
         const float az = q[r][H + k], bz = q[r][H - k];
-        Uxx = fmaf(a.c.cxx[k - 1], (ax - c2) + bx, Uxx);
-        Uyy = fmaf(a.c.cyy[k - 1], (ay - c2) + by, Uyy);
-        Uzz = fmaf(a.c.czz[k - 1], (az - c2) + bz, Uzz);
-        ux = fmaf(a.c.cx[k - 1], ax - bx, ux);
-        uy = fmaf(a.c.cy[k - 1], ay - by, uy);
-        uz = fmaf(a.c.cz[k - 1], az - bz, uz);
+        Uxx = fmaf(a.c.c2[k - 1], (ax - c2) + bx, Uxx);
+        Uyy = fmaf(a.c.c2[k - 1], (ay - c2) + by, Uyy);
+        Uzz = fmaf(a.c.c2[k - 1], (az - c2) + bz, Uzz);
+        ux = fmaf(a.c.c1[k - 1], ax - bx, ux);
+        uy = fmaf(a.c.c1[k - 1], ay - by, uy);
+        uz = fmaf(a.c.c1[k - 1], az - bz, uz);
       }
-      const float e = a.eps[i];
-      const float ed = a.epsd[i];
-      const float v2 = a.vdt2[i];
+      Uxx *= a.c.ixx; Uyy *= a.c.iyy; Uzz *= a.c.izz;
+      ux *= a.c.ix; uy *= a.c.iy; uz *= a.c.iz;
+      const float e = C.e[r], ed = C.ed[r], v2 = C.v2[r];
       // scalar operator (GPU_kernel.cu:93-111)
       float Sv = 1.f;
       const float mode = sqrtf(ux * ux + uy * uy + uz * uz);
@@ -284,11 +389,10 @@ __global__ __launch_bounds__(NTHREADS) void k_update(UpdArgs<S> a)
       const float e1 = 1.f + 2.f * e;
       float un;
       const bool xint = x >= pml && x < nx - pml;
-      if (xint && yint && zint) {
+      if (!PML || (xint && yint && zint)) {
         // GPU_kernel.cu:558-565
-        const float up = S::ld(a.unext, i);
-        un = (2.f * c0 - up) + ((e1 * (Uxx + Uyy) + Uzz) * Sv) * v2;
-      } else {
+        un = (2.f * c0 - C.st[r][0]) + ((e1 * (Uxx + Uyy) + Uzz) * Sv) * v2;
+      } else if constexpr (PML) {
         // GPU_kernel.cu:566-637; x faces own edges and corners, then y, then z.
         int fam, j;
         float sgn;
@@ -301,16 +405,14 @@ __global__ __launch_bounds__(NTHREADS) void k_update(UpdArgs<S> a)
         }
         const float4 t = a.pmltab[fam * pml + j];
         const float B = Sv * v2;
-        float Cn, Unn, Tg, gn;
-        if (fam == 0) { Cn = e1 * B; Unn = Uxx; Tg = Uzz + e1 * Uyy; gn = ux; }
-        else if (fam == 1) { Cn = e1 * B; Unn = Uyy; Tg = Uzz + e1 * Uxx; gn = uy; }
-        else { Cn = B; Unn = Uzz; Tg = e1 * (Uxx + Uyy); gn = uz; }
-        const float w1p = S::ld(a.w1p, i), w1c = S::ld(a.w1c, i);
-        const float pp = S::ld(a.pp, i), pc = S::ld(a.pc, i);
-        const float w3p = S::ld(a.w3p, i), w3c = S::ld(a.w3c, i);
-        const float w21 = S::ld(a.w21, i);
-        const float w1n = S::rnd(Cn * Unn - t.x * w1c + t.y * w1p);
-        const float pn = S::rnd(-t.x * pc + t.y * pp + sgn * (Cn * t.w * gn));
+        float Cn_, Unn, Tg, gn;
+        if (fam == 0) { Cn_ = e1 * B; Unn = Uxx; Tg = Uzz + e1 * Uyy; gn = ux; }
+        else if (fam == 1) { Cn_ = e1 * B; Unn = Uyy; Tg = Uzz + e1 * Uxx; gn = uy; }
+        else { Cn_ = B; Unn = Uzz; Tg = e1 * (Uxx + Uyy); gn = uz; }
+        const float w1p = C.st[r][0], w1c = C.st[r][1], pp = C.st[r][2], pc = C.st[r][3];
+        const float w3p = C.st[r][4], w3c = C.st[r][5], w21 = C.st[r][6];
+        const float w1n = S::rnd(Cn_ * Unn - t.x * w1c + t.y * w1p);
+        const float pn = S::rnd(-t.x * pc + t.y * pp + sgn * (Cn_ * t.w * gn));
         const float w21n = S::rnd(a.c.dt * pn + w21 * t.z);
         const float w3n = S::rnd(B * Tg + 2.f * w3c - w3p);
         S::st(a.w1p, i, w1n);
@@ -322,17 +424,20 @@ __global__ __launch_bounds__(NTHREADS) void k_update(UpdArgs<S> a)
       if (a.src_on && z >= a.sb_z0 && z <= a.sb_z1 && x >= a.sb_x0 && x <= a.sb_x1 && y >= a.sb_y0 &&
           y <= a.sb_y1) {
         const int bx = a.sb_x1 - a.sb_x0 + 1, by = a.sb_y1 - a.sb_y0 + 1;
-        const float s = a.sbox[((long)(z - a.sb_z0) * bx + (x - a.sb_x0)) * by + (y - a.sb_y0)];
-        un = fmaf(s, a.wk, un);  // u3 += source*wavelet, contracted as the CUDA build does
+        const float sv = a.sbox[((long)(z - a.sb_z0) * bx + (x - a.sb_x0)) * by + (y - a.sb_y0)];
+        un = fmaf(sv, a.wk, un);  // u3 += source*wavelet, contracted as the CUDA build does
       }
       un = S::rnd(un);
       S::st(a.unext, i, un);
-      if (z == a.sz && xint && yint) a.rec[(long)(x - pml) * (ny - 2 * pml) + (y - pml)] = un;
+      if (z == a.sz && xint && yint) a.rec[(x - pml) * (ny - 2 * pml) + (y - pml)] = un;
     }
-#pragma unroll
-    for (int r = 0; r < RX; r++)
-#pragma unroll
-      for (int j = 0; j < 2 * H; j++) q[r][j] = q[r][j + 1];
+    if (more) {
+      shift_queue<T>(q, L.qn);
+      L.stage(sm[b ^ 1], q, tx, ty);
+      C = Cn;
+      __syncthreads();
+      b ^= 1;
+    }
   }
 }
 
@@ -350,7 +455,7 @@ __global__ void k_rim_source(const typename S::T *ucur, typename S::T *unext, Ge
     const int x = x0 + xx, y = y0 + yy, z = z0 + zz;
     const bool r5 = x >= H && x < g.nx - H && y >= H && y < g.ny - H && z >= H && z < g.nz - H;
     if (r5) continue;
-    const long i = (long)z * g.pln + (long)x * g.ny + y;
+    const unsigned i = cell(g, z, x, y);
     const float v = fmaf(sbox[t], wk, S::ld(ucur, i));
     S::st(unext, i, S::rnd(v));
   }
@@ -365,8 +470,11 @@ __global__ void k_snapshot(const typename S::T *u, float *out, Geo g)
   const int y = blockIdx.x * blockDim.x + threadIdx.x;
   const int x = blockIdx.y, z = blockIdx.z;
   if (y >= hy) return;
-  out[((long)z * hx + x) * hy + y] =
-      S::ld(u, (long)(z + g.pml) * g.pln + (long)(x + g.pml) * g.ny + (y + g.pml));
+  out[((long)z * hx + x) * hy + y] = S::ld(u, cell(g, z + g.pml, x + g.pml, y + g.pml));
 }
+
+// Tile shapes the plan can launch (index = qp3d_config.tile_update / tile_grad - 1).
+#define QP3D_TILES(X) X(8, 2) X(16, 1) X(4, 4) X(8, 4) X(16, 2) X(4, 2)
+constexpr int NTILES = 6;
 
 }  // namespace qp3d

@@ -1,13 +1,12 @@
-// Host side of the qP3D HIP library: plan lifecycle, uploads, the time loop, and the
-// C-ABI entry points declared in include/qp3d.h (including the drop-in modeling3d()).
+// Host side of the qP3D HIP library: plan lifecycle, uploads, work decomposition, the time
+// loop, and the qp3d_plan_* entry points declared in include/qp3d.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
-#include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <string>
 #include <vector>
 
 #include "qp3d.h"
@@ -15,38 +14,34 @@
 
 using namespace qp3d;
 
-// ---------------------------------------------------------------------------------------
-// errors
-static thread_local char g_err[1024];
-
-extern "C" const char *qp3d_last_error(void) { return g_err; }
-
-extern "C" void qp3d_set_error(const char *fmt, ...)
-{
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(g_err, sizeof g_err, fmt, ap);
-  va_end(ap);
-}
-
-#define HIP_TRY(expr)                                                                        \
-  do {                                                                                       \
-    hipError_t e_ = (expr);                                                                  \
-    if (e_ != hipSuccess) {                                                                  \
-      qp3d_set_error("%s failed at %s:%d: %s", #expr, __FILE__, __LINE__, hipGetErrorString(e_)); \
-      return QP3D_ERR_HIP;                                                                   \
-    }                                                                                        \
+#define HIP_TRY(expr)                                                                                  \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess) {                                                                            \
+      qp3d_set_error("%s failed at %s:%d: %s", #expr, __FILE__, __LINE__, hipGetErrorString(e_));     \
+      return QP3D_ERR_HIP;                                                                             \
+    }                                                                                                  \
   } while (0)
 
-// ---------------------------------------------------------------------------------------
-// plan
+namespace {
+
+struct WorkList {
+  int4 *d = nullptr;
+  int n = 0;       // launched workgroups (multiple of 8, padding items are empty)
+  long cells = 0;  // cells updated by one launch
+};
+
+}  // namespace
+
 struct qp3d_plan {
   qp3d_config cfg{};
   int hx = 0, hy = 0, hz = 0;
   long N = 0;
   Geo geo{};
   Coef coef{};
-  int nblk = 0;  // launched workgroups (multiple of 8)
+  int tile_u = 0, tile_g = 0;
+  WorkList wl_grad, wl_int, wl_pml;
+  long n_int_cells = 0, n_pml_cells = 0;
   hipStream_t stream = nullptr;
 
   float *state = nullptr;  // u[2], w1[2], w3[2], p[2], w21: 9 fields of N floats
@@ -70,23 +65,216 @@ struct qp3d_plan {
   int steps_done = 0;  // one past the last step enqueued since the last reset
 
   bool instrument = false;
-  std::vector<hipEvent_t> ev;  // 3 per step: before gradmax, between, after update
-  std::vector<int> ev_pending; // steps with recorded events not yet harvested
-  double t_grad = 0, t_upd = 0;
+  std::vector<hipEvent_t> ev;   // 4 per step: before gradmax, after gradmax, after interior, after PML
+  std::vector<int> ev_pending;  // steps with recorded events not yet harvested
+  double t_grad = 0, t_int = 0, t_pml = 0;
   long n_launch = 0;
 };
 
-static int pick_zchunk(const qp3d_config &c, int ntile, int nz5)
+// ---------------------------------------------------------------------------------------
+// tile shapes and launches
+
+static int tile_tx(int t)
 {
-  if (c.zchunk > 0) return c.zchunk;
-  // Aim for >= 4 workgroups per CU on a 256-CU part, but keep chunks long enough that the
-  // 10-plane queue priming stays a small fraction of the work.
-  int want = (1024 + ntile - 1) / ntile;
-  int zc = (nz5 + want - 1) / want;
-  if (zc < 32) zc = 32;
-  if (zc > nz5) zc = nz5;
-  return zc;
+  static const int tx[] = {
+#define QP3D_TX(wx, rx) wx * rx,
+      QP3D_TILES(QP3D_TX)
+#undef QP3D_TX
+  };
+  return tx[t];
 }
+
+template <class S>
+static void launch_gradmax(int tile, const WorkList &w, hipStream_t st, const GradArgs<S> &a)
+{
+  int i = 0;
+#define QP3D_L(wx, rx)                                                                                 \
+  if (tile == i++) {                                                                                   \
+    hipLaunchKernelGGL((k_gradmax<S, Tile<wx, rx>>), dim3(w.n), dim3(TY, wx), 0, st, a);               \
+    return;                                                                                            \
+  }
+  QP3D_TILES(QP3D_L)
+#undef QP3D_L
+}
+
+template <class S, bool PML>
+static void launch_update(int tile, const WorkList &w, hipStream_t st, const UpdArgs<S> &a)
+{
+  int i = 0;
+#define QP3D_L(wx, rx)                                                                                 \
+  if (tile == i++) {                                                                                   \
+    hipLaunchKernelGGL((k_update<S, Tile<wx, rx>, PML>), dim3(w.n), dim3(TY, wx), 0, st, a);           \
+    return;                                                                                            \
+  }
+  QP3D_TILES(QP3D_L)
+#undef QP3D_L
+}
+
+template <class F>
+static int occupancy_of(int tile, F pick)
+{
+  int i = 0, occ = 0;
+#define QP3D_O(wx, rx)                                                                                 \
+  if (tile == i++) {                                                                                   \
+    const void *fn = pick(Tile<wx, rx>{});                                                             \
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, TY * wx, 0) != hipSuccess) occ = 1;     \
+  }
+  QP3D_TILES(QP3D_O)
+#undef QP3D_O
+  return std::max(occ, 1);
+}
+
+// ---------------------------------------------------------------------------------------
+// work decomposition
+
+namespace {
+struct Seg {
+  int a, b;
+};
+
+// [a,b) cut into pieces of at most w.
+void cut(std::vector<Seg> &out, int a, int b, int w)
+{
+  for (int s = a; s < b; s += w) out.push_back({s, std::min(s + w, b)});
+}
+
+int4 pack(int x0, int x1, int y0, int y1, int z0, int z1)
+{
+  return make_int4(x0 | (x1 << 16), y0 | (y1 << 16), z0, z1);
+}
+
+// z planes per item for `ntile` xy tiles over a z extent `len`: the count of items should fill
+// whole rounds of `capacity` resident workgroups while each item stays long enough for the
+// 10-plane queue priming to be small.
+int choose_zchunk(int ntile, int len, int capacity, int forced)
+{
+  if (forced > 0) return std::min(forced, len);
+  double best = -1;
+  int bestc = len;
+  for (int nc = 1; nc <= std::max(1, len / 16); nc++) {
+    const int zc = (len + nc - 1) / nc;
+    const long items = (long)ntile * ((len + zc - 1) / zc);
+    const long rounds = (items + capacity - 1) / capacity;
+    const double fill = (double)items / ((double)rounds * capacity);
+    const double prime = (double)zc / (zc + 2 * H);
+    const double score = fill * prime;
+    if (score > best + 1e-9) {
+      best = score;
+      bestc = zc;
+    }
+  }
+  return bestc;
+}
+
+// Order items so that the workgroups dispatched to one XCD (ids b with equal b % 8) take a
+// contiguous range of the logical order, and pad to a multiple of 8 with empty items.
+std::vector<int4> xcd_order(const std::vector<int4> &logical)
+{
+  const int n = (int)logical.size();
+  const int np = (n + 7) / 8 * 8, per = np / 8;
+  std::vector<int4> out(np, make_int4(0, 0, 0, 0));
+  for (int b = 0; b < np; b++) {
+    const int l = (b & 7) * per + (b >> 3);
+    if (l < n) out[b] = logical[l];
+  }
+  return out;
+}
+}  // namespace
+
+static int upload_list(WorkList &w, const std::vector<int4> &logical)
+{
+  std::vector<int4> v = xcd_order(logical);
+  w.n = (int)v.size();
+  w.cells = 0;
+  for (const int4 &t : logical)
+    w.cells += (long)(((unsigned)t.x >> 16) - (t.x & 0xffff)) * (((unsigned)t.y >> 16) - (t.y & 0xffff)) * (t.w - t.z);
+  if (w.n == 0) return QP3D_OK;
+  HIP_TRY(hipMalloc((void **)&w.d, sizeof(int4) * v.size()));
+  HIP_TRY(hipMemcpy(w.d, v.data(), sizeof(int4) * v.size(), hipMemcpyHostToDevice));
+  return QP3D_OK;
+}
+
+static int build_work(qp3d_plan *p)
+{
+  const qp3d_config &c = p->cfg;
+  int ncu = 256;
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device);
+  const int occ_g = occupancy_of(p->tile_g, [](auto t) {
+    return reinterpret_cast<const void *>(&k_gradmax<StoreF32, decltype(t)>);
+  });
+  const int occ_i = occupancy_of(p->tile_u, [](auto t) {
+    return reinterpret_cast<const void *>(&k_update<StoreF32, decltype(t), false>);
+  });
+  const int occ_p = occupancy_of(p->tile_u, [](auto t) {
+    return reinterpret_cast<const void *>(&k_update<StoreF32, decltype(t), true>);
+  });
+  // gradient-max: uniform tiles over [5,n-5)^3
+  {
+    const int TX = tile_tx(p->tile_g);
+    std::vector<Seg> xs, ys;
+    cut(xs, H, c.nx - H, TX);
+    cut(ys, H, c.ny - H, TY);
+    const int len = c.nz - 2 * H;
+    const int zc = choose_zchunk((int)(xs.size() * ys.size()), len, occ_g * ncu, c.zchunk);
+    std::vector<int4> items;
+    for (int z = H; z < c.nz - H; z += zc)
+      for (const Seg &x : xs)
+        for (const Seg &y : ys) items.push_back(pack(x.a, x.b, y.a, y.b, z, std::min(z + zc, c.nz - H)));
+    int rc = upload_list(p->wl_grad, items);
+    if (rc) return rc;
+  }
+  // update: tiles aligned to the PML boundary in x and y, z split at the PML planes
+  {
+    const int TX = tile_tx(p->tile_u), pml = c.pml;
+    std::vector<Seg> xs, ys;
+    std::vector<bool> xi, yi;
+    auto segs = [](std::vector<Seg> &out, std::vector<bool> &inner, int n, int pml, int w) {
+      const int bounds[4] = {H, pml, n - pml, n - H};
+      for (int s = 0; s < 3; s++) {
+        const size_t before = out.size();
+        cut(out, bounds[s], bounds[s + 1], w);
+        inner.insert(inner.end(), out.size() - before, s == 1);
+      }
+    };
+    segs(xs, xi, c.nx, pml, TX);
+    segs(ys, yi, c.ny, pml, TY);
+    int nint_tiles = 0;
+    for (size_t a = 0; a < xs.size(); a++)
+      for (size_t b = 0; b < ys.size(); b++) nint_tiles += xi[a] && yi[b];
+    const int nall = (int)(xs.size() * ys.size());
+    const int zmid = c.nz - 2 * pml, zpml = pml - H;
+    const int zc_i = choose_zchunk(nint_tiles, zmid, occ_i * ncu, c.zchunk);
+    const int zc_p = choose_zchunk(nall - nint_tiles, c.nz - 2 * H, occ_p * ncu, c.zchunk);
+    std::vector<int4> it, ip;
+    for (int z = pml; z < c.nz - pml; z += zc_i)
+      for (size_t a = 0; a < xs.size(); a++)
+        for (size_t b = 0; b < ys.size(); b++)
+          if (xi[a] && yi[b]) it.push_back(pack(xs[a].a, xs[a].b, ys[b].a, ys[b].b, z, std::min(z + zc_i, c.nz - pml)));
+    // PML tiles over the whole z range, then the interior tiles' top and bottom z faces
+    for (int z = H; z < c.nz - H; z += zc_p)
+      for (size_t a = 0; a < xs.size(); a++)
+        for (size_t b = 0; b < ys.size(); b++)
+          if (!(xi[a] && yi[b])) ip.push_back(pack(xs[a].a, xs[a].b, ys[b].a, ys[b].b, z, std::min(z + zc_p, c.nz - H)));
+    for (int face = 0; face < 2; face++) {
+      const int za = face == 0 ? H : c.nz - pml, zb = face == 0 ? pml : c.nz - H;
+      const int zc = std::max(1, std::min(zpml, zc_p));
+      for (int z = za; z < zb; z += zc)
+        for (size_t a = 0; a < xs.size(); a++)
+          for (size_t b = 0; b < ys.size(); b++)
+            if (xi[a] && yi[b]) ip.push_back(pack(xs[a].a, xs[a].b, ys[b].a, ys[b].b, z, std::min(z + zc, zb)));
+    }
+    int rc = upload_list(p->wl_int, it);
+    if (rc) return rc;
+    rc = upload_list(p->wl_pml, ip);
+    if (rc) return rc;
+    p->n_int_cells = p->wl_int.cells;
+    p->n_pml_cells = p->wl_pml.cells;
+  }
+  return QP3D_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// lifecycle
 
 extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
 {
@@ -113,13 +301,17 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
     qp3d_set_error("plan_create: ntsnap must be positive when snapflag == 1");
     return QP3D_ERR_ARG;
   }
+  if (c.tile_update < 0 || c.tile_update > NTILES || c.tile_grad < 0 || c.tile_grad > NTILES) {
+    qp3d_set_error("plan_create: tile shapes must be in [0,%d]", NTILES);
+    return QP3D_ERR_ARG;
+  }
   if (c.storage != QP3D_STORAGE_FP32) {
     qp3d_set_error("plan_create: storage mode %d is not available in this build", c.storage);
     return QP3D_ERR_ARG;
   }
   const double cells = (double)c.nx * c.ny * c.nz;
-  if (cells > 4.0e9) {
-    qp3d_set_error("plan_create: grid of %.3g cells exceeds the 4e9-cell limit", cells);
+  if (cells >= 1073741824.0 || c.nx > 32767 || c.ny > 32767) {
+    qp3d_set_error("plan_create: grid of %.4g cells exceeds the 2^30-cell limit of one plan", cells);
     return QP3D_ERR_ARG;
   }
   int ndev = 0;
@@ -136,16 +328,10 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
   p->hy = c.ny - 2 * c.pml;
   p->hz = c.nz - 2 * c.pml;
   p->N = (long)c.nx * c.ny * c.nz;
-  Geo &g = p->geo;
-  g.nx = c.nx; g.ny = c.ny; g.nz = c.nz; g.pml = c.pml;
-  g.pln = (long)c.nx * c.ny;
-  const int nx5 = c.nx - 2 * H, ny5 = c.ny - 2 * H, nz5 = c.nz - 2 * H;
-  g.tiles_y = (ny5 + TY - 1) / TY;
-  g.tiles_x = (nx5 + TX - 1) / TX;
-  g.zchunk = pick_zchunk(c, g.tiles_x * g.tiles_y, nz5);
-  const int nzc = (nz5 + g.zchunk - 1) / g.zchunk;
-  g.nlog = g.tiles_x * g.tiles_y * nzc;
-  p->nblk = (g.nlog + 7) / 8 * 8;
+  p->geo.nx = c.nx; p->geo.ny = c.ny; p->geo.nz = c.nz; p->geo.pml = c.pml;
+  p->geo.pln = (unsigned)c.nx * (unsigned)c.ny;
+  p->tile_u = c.tile_update > 0 ? c.tile_update - 1 : 1;  // 16x1 by default
+  p->tile_g = c.tile_grad > 0 ? c.tile_grad - 1 : 0;      // 8x2 by default
 
   auto fail = [&](int rc) {
     qp3d_plan_destroy(p);
@@ -190,6 +376,7 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
       p->snap_steps.push_back(k);
     }
   }
+  if ((rc = build_work(p))) return fail(rc);
   *out = p;
   return QP3D_OK;
 }
@@ -206,6 +393,9 @@ extern "C" int qp3d_plan_destroy(qp3d_plan *p)
   (void)hipFree(p->flag);
   (void)hipFree(p->rec);
   (void)hipFree(p->sbox);
+  (void)hipFree(p->wl_grad.d);
+  (void)hipFree(p->wl_int.d);
+  (void)hipFree(p->wl_pml.d);
   for (float *d : p->snaps) (void)hipFree(d);
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -225,6 +415,7 @@ extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps,
   }
   const qp3d_config &cf = p->cfg;
   HIP_TRY(hipSetDevice(cf.device));
+  HIP_TRY(hipStreamSynchronize(p->stream));
   const long N = p->N;
   // model: (vp*dt)^2, epsilon, epsilon-delta
   std::vector<float> h(3 * (size_t)N);
@@ -250,16 +441,16 @@ extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps,
       tab[(size_t)f * cf.pml + j] = t;
     }
   HIP_TRY(hipMemcpy(p->pmltab, tab.data(), sizeof(float4) * tab.size(), hipMemcpyHostToDevice));
-  // FD coefficients (10th-order row of the 6x6 tables, CPU_function.c:123-127,143-147)
+  // FD coefficients: the 10th-order rows of the 6x6 tables (CPU_function.c:123-127,143-147)
   Coef &k = p->coef;
   for (int m = 1; m <= H; m++) {
-    k.cxx[m - 1] = c2[30 + m] / (cf.dx * cf.dx);
-    k.cyy[m - 1] = c2[30 + m] / (cf.dy * cf.dy);
-    k.czz[m - 1] = c2[30 + m] / (cf.dz * cf.dz);
-    k.cx[m - 1] = c[30 + m] / cf.dx;
-    k.cy[m - 1] = c[30 + m] / cf.dy;
-    k.cz[m - 1] = c[30 + m] / cf.dz;
+    k.c1[m - 1] = c[30 + m];
+    k.c2[m - 1] = c2[30 + m];
   }
+  k.ix = 1.0f / cf.dx; k.iy = 1.0f / cf.dy; k.iz = 1.0f / cf.dz;
+  k.ixx = (float)(1.0 / ((double)cf.dx * cf.dx));
+  k.iyy = (float)(1.0 / ((double)cf.dy * cf.dy));
+  k.izz = (float)(1.0 / ((double)cf.dz * cf.dz));
   k.dt = cf.dt;
   // source: bounding box of the non-zero cells of the full-grid spatial source
   int z0 = cf.nz, z1 = -1, x0 = cf.nx, x1 = -1, y0 = cf.ny, y1 = -1;
@@ -310,15 +501,20 @@ extern "C" int qp3d_plan_reset(qp3d_plan *p)
   return QP3D_OK;
 }
 
+// ---------------------------------------------------------------------------------------
+// time loop
+
 static int harvest_events(qp3d_plan *p)
 {
   for (int k : p->ev_pending) {
-    hipEvent_t *e = &p->ev[3 * (size_t)k];
-    float a = 0, b = 0;
+    hipEvent_t *e = &p->ev[4 * (size_t)k];
+    float a = 0, b = 0, c = 0;
     HIP_TRY(hipEventElapsedTime(&a, e[0], e[1]));
     HIP_TRY(hipEventElapsedTime(&b, e[1], e[2]));
+    HIP_TRY(hipEventElapsedTime(&c, e[2], e[3]));
     p->t_grad += a;
-    p->t_upd += b;
+    p->t_int += b;
+    p->t_pml += c;
     p->n_launch += 1;
   }
   p->ev_pending.clear();
@@ -331,12 +527,12 @@ static int launch_step(qp3d_plan *p, int k)
   const int prev = k & 1, cur = prev ^ 1;  // u[prev] holds u(k-1) and receives u(k+1)
   hipEvent_t *e = nullptr;
   if (p->instrument) {
-    if (p->ev.size() < 3 * (size_t)cf.nt) {
+    if (p->ev.size() < 4 * (size_t)cf.nt) {
       size_t old = p->ev.size();
-      p->ev.resize(3 * (size_t)cf.nt);
+      p->ev.resize(4 * (size_t)cf.nt);
       for (size_t i = old; i < p->ev.size(); i++) HIP_TRY(hipEventCreate(&p->ev[i]));
     }
-    e = &p->ev[3 * (size_t)k];
+    e = &p->ev[4 * (size_t)k];
     HIP_TRY(hipEventRecord(e[0], p->stream));
   }
   GradArgs<StoreF32> ga;
@@ -345,7 +541,8 @@ static int launch_step(qp3d_plan *p, int k)
   ga.ucur = p->u[cur];
   ga.gmax = p->gmax + k;
   ga.flag = p->flag;
-  hipLaunchKernelGGL(k_gradmax<StoreF32>, dim3(p->nblk), dim3(TY, WX), 0, p->stream, ga);
+  ga.work = p->wl_grad.d;
+  launch_gradmax(p->tile_g, p->wl_grad, p->stream, ga);
   HIP_TRY(hipGetLastError());
   if (e) HIP_TRY(hipEventRecord(e[1], p->stream));
 
@@ -368,10 +565,19 @@ static int launch_step(qp3d_plan *p, int k)
   ua.sbox = p->sbox;
   ua.rec = p->rec + (size_t)k * p->hx * p->hy;
   ua.sz = cf.sz;
-  hipLaunchKernelGGL(k_update<StoreF32>, dim3(p->nblk), dim3(TY, WX), 0, p->stream, ua);
-  HIP_TRY(hipGetLastError());
+  if (p->wl_int.n) {
+    ua.work = p->wl_int.d;
+    launch_update<StoreF32, false>(p->tile_u, p->wl_int, p->stream, ua);
+    HIP_TRY(hipGetLastError());
+  }
+  if (e) HIP_TRY(hipEventRecord(e[2], p->stream));
+  if (p->wl_pml.n) {
+    ua.work = p->wl_pml.d;
+    launch_update<StoreF32, true>(p->tile_u, p->wl_pml, p->stream, ua);
+    HIP_TRY(hipGetLastError());
+  }
   if (e) {
-    HIP_TRY(hipEventRecord(e[2], p->stream));
+    HIP_TRY(hipEventRecord(e[3], p->stream));
     p->ev_pending.push_back(k);
   }
   if (p->src_rim) {
@@ -436,6 +642,9 @@ extern "C" int qp3d_plan_sync(qp3d_plan *p)
   return QP3D_OK;
 }
 
+// ---------------------------------------------------------------------------------------
+// results
+
 extern "C" int qp3d_plan_record(const qp3d_plan *p, float *record)
 {
   if (!p || !record) {
@@ -490,6 +699,15 @@ extern "C" int qp3d_plan_snapshot(const qp3d_plan *p, int i, int *step, float *o
   return QP3D_OK;
 }
 
+extern "C" int qp3d_plan_dims(const qp3d_plan *p, int *hx, int *hy, int *hz)
+{
+  if (!p) return QP3D_ERR_ARG;
+  if (hx) *hx = p->hx;
+  if (hy) *hy = p->hy;
+  if (hz) *hz = p->hz;
+  return QP3D_OK;
+}
+
 extern "C" int qp3d_plan_step_maxima(const qp3d_plan *p, float *out)
 {
   if (!p || !out) {
@@ -506,134 +724,40 @@ extern "C" int qp3d_plan_set_instrument(qp3d_plan *p, int on)
 {
   if (!p) return QP3D_ERR_ARG;
   p->instrument = on != 0;
-  p->t_grad = p->t_upd = 0;
+  p->t_grad = p->t_int = p->t_pml = 0;
   p->n_launch = 0;
   p->ev_pending.clear();
   return QP3D_OK;
 }
 
-extern "C" int qp3d_plan_kernel_times(qp3d_plan *p, double *ms_gradmax, double *ms_update, long *launches)
+extern "C" int qp3d_plan_kernel_times(qp3d_plan *p, double *ms)
 {
-  if (!p) return QP3D_ERR_ARG;
+  if (!p || !ms) return QP3D_ERR_ARG;
   int rc = harvest_events(p);
   if (rc) return rc;
   const long n = p->n_launch;
-  if (ms_gradmax) *ms_gradmax = n ? p->t_grad / n : 0;
-  if (ms_update) *ms_update = n ? p->t_upd / n : 0;
-  if (launches) *launches = n;
+  ms[0] = n ? p->t_grad / n : 0;
+  ms[1] = n ? p->t_int / n : 0;
+  ms[2] = n ? p->t_pml / n : 0;
+  ms[3] = (double)n;
   return QP3D_OK;
 }
 
-extern "C" int qp3d_plan_step_bytes(const qp3d_plan *p, double *bytes_gradmax, double *bytes_update)
+extern "C" int qp3d_plan_step_bytes(const qp3d_plan *p, double *bytes)
 {
-  if (!p) return QP3D_ERR_ARG;
+  if (!p || !bytes) return QP3D_ERR_ARG;
   const qp3d_config &c = p->cfg;
   const double nupd = (double)(c.nx - 10) * (c.ny - 10) * (c.nz - 10);
   const double nint = (double)p->hx * p->hy * p->hz;
   const double npml = nupd - nint;
-  // gradient-max: one read of u per updated cell. update: interior u_cur, u_prev, 3 model
-  // fields, u_next (24 B); PML u_cur, 3 model fields, u_next, 7 state reads and 4 state writes (64 B).
-  if (bytes_gradmax) *bytes_gradmax = 4.0 * nupd;
-  if (bytes_update) *bytes_update = 24.0 * nint + 64.0 * npml;
+  // gradient-max: one read of u per updated cell. Interior update: u_cur, u_prev, 3 model
+  // fields, u_next (24 B). PML update: u_cur, 3 model fields, u_next, 7 state reads and 4 state
+  // writes (64 B); the PML launch also carries the interior cells of mixed tiles (24 B).
+  const double pml_launch_int = (double)p->n_pml_cells - npml;
+  bytes[0] = 4.0 * nupd;
+  bytes[1] = 24.0 * (double)p->n_int_cells;
+  bytes[2] = 64.0 * npml + 24.0 * pml_launch_int;
+  bytes[3] = (double)p->n_int_cells;
+  bytes[4] = (double)p->n_pml_cells;
   return QP3D_OK;
-}
-
-// ---------------------------------------------------------------------------------------
-// drop-in entry points
-static int write_floats(const char *name, const float *a, size_t n)
-{
-  FILE *fp = fopen(name, "wb");
-  if (!fp) {
-    qp3d_set_error("cannot open %s for writing", name);
-    return QP3D_ERR_IO;
-  }
-  size_t w = fwrite(a, sizeof(float), n, fp);
-  fclose(fp);
-  if (w != n) {
-    qp3d_set_error("short write to %s", name);
-    return QP3D_ERR_IO;
-  }
-  return QP3D_OK;
-}
-
-// 3dPureqPModeling_single_shot.cu:340-377 / modeling3D.c:89-124: the interior cube and its
-// three centre slices, xz at y'=(hy-1)/2, yz at x'=(hx-1)/2, xy at z'=(hz-1)/2.
-static int write_snapshot_files(const char *prefix, int k, const float *snap, int hx, int hy, int hz)
-{
-  char name[4096];
-  std::vector<float> xz((size_t)hz * hx), yz((size_t)hz * hy), xy((size_t)hx * hy);
-  for (int z = 0; z < hz; z++)
-    for (int x = 0; x < hx; x++) xz[(size_t)z * hx + x] = snap[((size_t)z * hx + x) * hy + (hy - 1) / 2];
-  for (int z = 0; z < hz; z++)
-    for (int y = 0; y < hy; y++) yz[(size_t)z * hy + y] = snap[((size_t)z * hx + (hx - 1) / 2) * hy + y];
-  for (int x = 0; x < hx; x++)
-    for (int y = 0; y < hy; y++) xy[(size_t)x * hy + y] = snap[((size_t)((hz - 1) / 2) * hx + x) * hy + y];
-  int rc;
-  snprintf(name, sizeof name, "%s%d.dat", prefix, k);
-  if ((rc = write_floats(name, snap, (size_t)hx * hy * hz))) return rc;
-  snprintf(name, sizeof name, "%s_xz%d.dat", prefix, k);
-  if ((rc = write_floats(name, xz.data(), xz.size()))) return rc;
-  snprintf(name, sizeof name, "%s_yz%d.dat", prefix, k);
-  if ((rc = write_floats(name, yz.data(), yz.size()))) return rc;
-  snprintf(name, sizeof name, "%s_xy%d.dat", prefix, k);
-  if ((rc = write_floats(name, xy.data(), xy.size()))) return rc;
-  return QP3D_OK;
-}
-
-extern "C" int qp3d_modeling3d(int nx, int ny, int nz, int nt, int ntsnap, float dx, float dy, float dz, float dt,
-                               int pml, int snapflag, int sx, int sy, int sz, const float *vp, const float *epsilon,
-                               const float *delta, const float *source, const float *wavelet, float *record,
-                               const float *dlr, const float *ddlr, const float *dtb, const float *ddtb,
-                               const float *dfb, const float *ddfb, const float *c, const float *c2,
-                               const char *snap_file)
-{
-  (void)sx; (void)sy;
-  if (!record) {
-    qp3d_set_error("modeling3d: record is null");
-    return QP3D_ERR_ARG;
-  }
-  if (snapflag == 1 && !snap_file) {
-    qp3d_set_error("modeling3d: snapflag == 1 needs snap_file");
-    return QP3D_ERR_ARG;
-  }
-  qp3d_config cfg{};
-  cfg.nx = nx; cfg.ny = ny; cfg.nz = nz; cfg.pml = pml; cfg.nt = nt;
-  cfg.dx = dx; cfg.dy = dy; cfg.dz = dz; cfg.dt = dt;
-  cfg.sz = sz;
-  cfg.storage = QP3D_STORAGE_FP32;
-  cfg.snapflag = snapflag == 1 ? 1 : 0;
-  cfg.ntsnap = ntsnap;
-  cfg.device = 0;
-  qp3d_plan *p = nullptr;
-  int rc = qp3d_plan_create(&cfg, &p);
-  if (rc) return rc;
-  if ((rc = qp3d_plan_upload(p, vp, epsilon, delta, source, wavelet, dlr, ddlr, dtb, ddtb, dfb, ddfb, c, c2)) ||
-      (rc = qp3d_plan_reset(p)) || (rc = qp3d_plan_run(p, 0, nt)) || (rc = qp3d_plan_sync(p)) ||
-      (rc = qp3d_plan_record(p, record))) {
-    qp3d_plan_destroy(p);
-    return rc;
-  }
-  if (cfg.snapflag) {
-    std::vector<float> snap((size_t)p->hx * p->hy * p->hz);
-    for (int i = 0; i < qp3d_plan_snapshot_count(p); i++) {
-      int k = 0;
-      if ((rc = qp3d_plan_snapshot(p, i, &k, snap.data())) ||
-          (rc = write_snapshot_files(snap_file, k, snap.data(), p->hx, p->hy, p->hz))) {
-        qp3d_plan_destroy(p);
-        return rc;
-      }
-    }
-  }
-  qp3d_plan_destroy(p);
-  return QP3D_OK;
-}
-
-extern "C" void modeling3d(int nx, int ny, int nz, int nt, int ntsnap, float dx, float dy, float dz, float dt,
-                           int pml, int snapflag, int sx, int sy, int sz, float *vp, float *epsilon, float *delta,
-                           float *source, float *wavelet, float *record, float *dlr, float *ddlr, float *dtb,
-                           float *ddtb, float *dfb, float *ddfb, float *c, float *c2, const char *snap_file)
-{
-  int rc = qp3d_modeling3d(nx, ny, nz, nt, ntsnap, dx, dy, dz, dt, pml, snapflag, sx, sy, sz, vp, epsilon, delta,
-                           source, wavelet, record, dlr, ddlr, dtb, ddtb, dfb, ddfb, c, c2, snap_file);
-  if (rc) fprintf(stderr, "modeling3d: %s\n", qp3d_last_error());
 }

@@ -37,6 +37,7 @@ class Config(C.Structure):
         ("dx", C.c_float), ("dy", C.c_float), ("dz", C.c_float), ("dt", C.c_float),
         ("sz", C.c_int), ("storage", C.c_int), ("use_graph", C.c_int), ("snapflag", C.c_int),
         ("ntsnap", C.c_int), ("zchunk", C.c_int), ("device", C.c_int),
+        ("tile_update", C.c_int), ("tile_grad", C.c_int),
     ]
 
 
@@ -66,9 +67,9 @@ _SIGS = [
     ("qp3d_plan_snapshot", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _F]),
     ("qp3d_plan_step_maxima", C.c_int, [C.c_void_p, _F]),
     ("qp3d_plan_set_instrument", C.c_int, [C.c_void_p, C.c_int]),
-    ("qp3d_plan_kernel_times", C.c_int,
-     [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_long)]),
-    ("qp3d_plan_step_bytes", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ("qp3d_plan_dims", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("qp3d_plan_kernel_times", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    ("qp3d_plan_step_bytes", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]

@@ -30,11 +30,13 @@ def modeling3d(s, record=None, snapflag=0, ntsnap=1, snap_file=None):
 class Plan:
     """Device-resident forward solver for one grid (qp3d_plan_* in include/qp3d.h)."""
 
-    def __init__(self, s, storage="fp32", use_graph=False, snapflag=0, ntsnap=0, zchunk=0, device=0):
+    def __init__(self, s, storage="fp32", use_graph=False, snapflag=0, ntsnap=0, zchunk=0, device=0,
+                 tile_update=0, tile_grad=0):
         self.lib = load()
         self.s = s
         cfg = Config(s.nx, s.ny, s.nz, s.pml, s.nt, s.dx, s.dy, s.dz, s.dt, s.sz, STORAGE[storage],
-                     int(use_graph), int(snapflag), int(ntsnap), int(zchunk), int(device))
+                     int(use_graph), int(snapflag), int(ntsnap), int(zchunk), int(device),
+                     int(tile_update), int(tile_grad))
         h = C.c_void_p()
         check(self.lib.qp3d_plan_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -96,14 +98,18 @@ class Plan:
         check(self.lib.qp3d_plan_set_instrument(self.h, int(on)))
 
     def kernel_times(self):
-        a, b, n = C.c_double(), C.c_double(), C.c_long()
-        check(self.lib.qp3d_plan_kernel_times(self.h, C.byref(a), C.byref(b), C.byref(n)))
-        return a.value, b.value, n.value
+        """Mean ms per step of the gradient-max, interior-update and PML-update launches, and
+        the number of instrumented steps."""
+        ms = (C.c_double * 4)()
+        check(self.lib.qp3d_plan_kernel_times(self.h, ms))
+        return {"gradmax": ms[0], "update_interior": ms[1], "update_pml": ms[2], "steps": int(ms[3])}
 
     def step_bytes(self):
-        a, b = C.c_double(), C.c_double()
-        check(self.lib.qp3d_plan_step_bytes(self.h, C.byref(a), C.byref(b)))
-        return a.value, b.value
+        """Algorithmic bytes per step of each launch and the cells of the two update launches."""
+        b = (C.c_double * 5)()
+        check(self.lib.qp3d_plan_step_bytes(self.h, b))
+        return {"gradmax": b[0], "update_interior": b[1], "update_pml": b[2], "cells_interior": b[3],
+                "cells_pml": b[4]}
 
     def close(self):
         if self.h:

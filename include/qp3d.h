@@ -92,7 +92,13 @@ typedef struct qp3d_config {
   int ntsnap;
   int zchunk;    /* z planes per workgroup in the stencil kernels, 0 = automatic */
   int device;    /* HIP device ordinal */
+  int tile_update; /* tile shape of the update kernel, 1..QP3D_NTILES, 0 = default */
+  int tile_grad;   /* tile shape of the gradient-max kernel, 1..QP3D_NTILES, 0 = default */
 } qp3d_config;
+
+/* Tile shapes (wavefronts along x x rows per lane; a tile is 64 cells along y):
+ * 1: 8x2  2: 16x1  3: 4x4  4: 8x4  5: 16x2  6: 4x2 */
+#define QP3D_NTILES 6
 
 typedef struct qp3d_plan qp3d_plan;
 
@@ -119,14 +125,19 @@ int qp3d_plan_snapshot_count(const qp3d_plan *p);
 int qp3d_plan_snapshot(const qp3d_plan *p, int i, int *step, float *out);
 /* Per-step gradient maxima G[k] (float) of steps [0, nt). */
 int qp3d_plan_step_maxima(const qp3d_plan *p, float *out);
-/* 1: bracket every stencil launch with HIP events on the plan's stream. */
+/* Interior extents hx, hy, hz of the plan's grid (any output may be NULL). */
+int qp3d_plan_dims(const qp3d_plan *p, int *hx, int *hy, int *hz);
+/* 1: bracket every stencil launch with HIP events on the plan's stream (and reset the
+ * accumulated times). */
 int qp3d_plan_set_instrument(qp3d_plan *p, int on);
-/* Mean device time (ms) per launch of the gradient-max and the update kernel over the
- * instrumented steps since the last reset of the counters, and the number of launches. */
-int qp3d_plan_kernel_times(qp3d_plan *p, double *ms_gradmax, double *ms_update, long *launches);
-/* Algorithmic HBM bytes of one time step of the plan (interior 28 B/cell, PML 68 B/cell in FP32),
- * split into the gradient-max kernel and the update kernel. */
-int qp3d_plan_step_bytes(const qp3d_plan *p, double *bytes_gradmax, double *bytes_update);
+/* Mean device time (ms) per time step of the three stencil launches over the instrumented
+ * steps: ms[0] gradient-max, ms[1] interior update, ms[2] PML update; ms[3] = steps counted. */
+int qp3d_plan_kernel_times(qp3d_plan *p, double *ms);
+/* Algorithmic HBM bytes of one time step (interior cells 28 B, PML cells 68 B in FP32):
+ * bytes[0] gradient-max launch (4 B per updated cell), bytes[1] interior update launch (24 B per
+ * cell), bytes[2] PML update launch (64 B per PML cell, 24 B per interior cell it carries);
+ * bytes[3], bytes[4] = cells of the interior and the PML update launches. */
+int qp3d_plan_step_bytes(const qp3d_plan *p, double *bytes);
 
 #ifdef __cplusplus
 }

@@ -152,3 +152,21 @@ def test_blowup_is_reported():
         with pytest.raises(qp3d.Qp3dError) as ei:
             p.sync()
     assert ei.value.code == -5
+
+
+@pytest.mark.parametrize("tile", list(range(1, 7)))
+def test_every_tile_shape(tile):
+    """Each launchable tile shape of both stencil kernels against the oracle, on a grid whose
+    extents are not multiples of any tile and with z-chunks shorter than the grid."""
+    s = qp3d.config_setup(hx=37, hy=71, nz0=29, pml=9, nt=50, seed=7, src_depth=4)
+    say(f"tile {tile}")
+    with qp3d.Plan(s, tile_update=tile, tile_grad=tile, zchunk=11) as p:
+        rec = p.forward()
+        u = p.field(0)
+    o = O.OracleState(s)
+    orec = np.zeros_like(rec)
+    for k in range(s.nt):
+        o.step(s.wavelet[k])
+        orec[k] = o.field("u2")[s.sz, s.pml:s.nx - s.pml, s.pml:s.ny - s.pml]
+    assert rel_l2(rec, orec) <= TOL
+    assert rel_l2(u, o.field("u2")) <= TOL
