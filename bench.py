@@ -34,6 +34,25 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def calib_copy(nbytes):
+    """A plain device-to-device copy of nbytes through the HIP runtime: under rocprofv3 --pmc its
+    FETCH_SIZE / WRITE_SIZE give the counters' scale for a known traffic of nbytes each way."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 0, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    t = time.perf_counter()
+    hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    log(f"calibration copy of {nbytes} bytes: {nbytes / (time.perf_counter() - t) / 1e9:.1f} GB/s (read) ")
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def cpu_baseline(setup, plan, start, nsteps):
     """Time the oracle for nsteps starting from the GPU's wavefield after `start` steps."""
     import numpy as np
@@ -72,6 +91,8 @@ def main():
     ap.add_argument("--graph", type=int, default=0)
     ap.add_argument("--tile-update", type=int, default=0)
     ap.add_argument("--tile-grad", type=int, default=0)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="first run a 1 GiB device-to-device copy (the reference kernel for scaling HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the shot-parallel and z-slab "
@@ -80,6 +101,8 @@ def main():
 
     import qp3d
 
+    if args.calib_copy:
+        calib_copy(1 << 30)
     t = time.perf_counter()
     s = qp3d.config_setup(hx=args.n, hy=args.n, nz0=args.n, pml=args.pml, nt=args.nt, seed=1, fm=25.0, src_depth=0)
     log(f"setup {s.nx}x{s.ny}x{s.nz} in {time.perf_counter() - t:.1f} s")

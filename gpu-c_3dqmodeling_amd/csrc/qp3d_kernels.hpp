@@ -10,26 +10,31 @@
 //                    (GPU_kernel.cu:87-115), 10th-order second derivatives and the leapfrog
 //                    (GPU_kernel.cu:552-565), source injection (GPU_kernel.cu:720-731) and the
 //                    receiver-plane write (3dPureqPModeling_single_shot.cu:333-337);
-//   k_update<PML=1>  the same for the tiles that hold PML cells, with the split-field
-//                    recursions (GPU_kernel.cu:566-637).
-// Splitting interior and PML tiles keeps the PML state out of the interior kernel's
-// registers. The copy-rotation kernel (GPU_kernel.cu:733-812) disappears: u_next and the PML
-// state are written in place of the oldest slot (each cell reads that slot only at its own
-// index) and the host flips slot parity each step.
+//   k_update<PML=1>  the same for the PML tiles, with the split-field recursions
+//                    (GPU_kernel.cu:566-637).
+// Tiles are aligned to the PML boundary, so an interior tile holds only interior cells and a
+// PML tile only cells of one face family (x faces own edges and corners, then y, then z),
+// which makes the family uniform over a workgroup and keeps the PML state out of the interior
+// kernel's registers. The copy-rotation kernel (GPU_kernel.cu:733-812) disappears: u_next and
+// the PML state are written in place of the oldest slot (each cell reads that slot only at its
+// own index) and the host flips slot parity each step.
 //
 // Tiling: a workgroup owns an xy tile (at most TX x 64 cells, y unit-stride, one 64-lane
 // wavefront per x row group) and marches through a z range; its work item comes from a list
-// the host builds (tiles aligned to the PML boundary, z ranges split at the PML planes,
-// ordered so that each XCD's L2 sees neighbouring tiles). The current plane with a 5-cell
-// halo sits in LDS (double-buffered, one barrier per plane) for the x and y derivatives; each
-// lane keeps an 11-deep register queue of its column in z for the z derivatives. Every global
-// load of plane z+1 (queue front, halos, model, previous wavefield, PML state) is issued
-// before plane z is computed, so HBM latency overlaps the arithmetic of the plane before.
-// Cell offsets are 32-bit (grids below 2^30 cells) so loads use a scalar base plus a vector
-// offset.
+// the host builds (ordered so that each XCD's L2 sees neighbouring tiles). The current plane
+// with a 5-cell halo sits in LDS (double-buffered, one barrier per plane) for the x and y
+// derivatives; each lane keeps an 11-deep register queue of its column in z for the z
+// derivatives. Every global load of plane z+1 (queue front, halos, model, previous wavefield,
+// PML state) is issued before plane z is computed, so HBM latency overlaps the arithmetic of
+// the plane before. Loads are unconditional: lanes outside the grid read a clamped cell and
+// never use it. A lane's offsets within a plane are fixed for the whole march and the plane
+// base is a scalar, so each load is one instruction with no vector address arithmetic
+// (grids stay below 2^30 cells).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 namespace qp3d {
 
@@ -47,47 +52,50 @@ struct Tile {
   static_assert(TX * 2 * H <= NT, "one y-halo value per lane");
 };
 
+// p advanced by a byte offset (a 32-bit offset from a scalar base keeps loads in the
+// scalar-base + vector-offset form).
 template <class P>
-__device__ __forceinline__ P *at(P *p, unsigned byte_off)
+__device__ __forceinline__ const P *bat(const P *p, unsigned b)
 {
-  return reinterpret_cast<P *>(reinterpret_cast<uintptr_t>(p) + (uintptr_t)byte_off);
+  return reinterpret_cast<const P *>(reinterpret_cast<const char *>(p) + b);
+}
+template <class P>
+__device__ __forceinline__ P *bat(P *p, unsigned b)
+{
+  return reinterpret_cast<P *>(reinterpret_cast<char *>(p) + b);
 }
 
-// Storage policies for wavefields and PML state. Arithmetic is always float.
+// Storage policies for wavefields and PML state. Arithmetic is always float. Offsets are byte
+// offsets of the cell in a 4-byte-per-cell array.
 struct StoreF32 {
   using T = float;
-  static __device__ __forceinline__ float ld(const T *p, unsigned i) { return *at(p, i << 2); }
-  static __device__ __forceinline__ float ld_stream(const T *p, unsigned i)
-  {
-    return __builtin_nontemporal_load(at(p, i << 2));
-  }
-  static __device__ __forceinline__ void st(T *p, unsigned i, float v) { __builtin_nontemporal_store(v, at(p, i << 2)); }
+  static __device__ __forceinline__ float ld(const T *p, unsigned b) { return *bat(p, b); }
+  static __device__ __forceinline__ float ld_stream(const T *p, unsigned b) { return __builtin_nontemporal_load(bat(p, b)); }
+  static __device__ __forceinline__ void st(T *p, unsigned b, float v) { __builtin_nontemporal_store(v, bat(p, b)); }
   static __device__ __forceinline__ float rnd(float v) { return v; }
 };
 
-__device__ __forceinline__ float ld_model(const float *p, unsigned i) { return __builtin_nontemporal_load(at(p, i << 2)); }
+__device__ __forceinline__ float ld_model(const float *p, unsigned b) { return __builtin_nontemporal_load(bat(p, b)); }
 
 struct Geo {
   int nx, ny, nz, pml;
   unsigned pln;  // nx*ny
 };
 
-__device__ __forceinline__ unsigned cell(const Geo &g, int z, int x, int y)
-{
-  return (unsigned)z * g.pln + (unsigned)x * (unsigned)g.ny + (unsigned)y;
-}
-
 // Work item of one workgroup: x in [x0,x1), y in [y0,y1), z in [z0,z1); x1-x0 <= TX, y1-y0 <= 64.
 // Packed as {x0 | x1<<16, y0 | y1<<16, z0, z1}; z0 == z1 marks padding.
-__device__ __forceinline__ void unpack(int4 w, int &x0, int &x1, int &y0, int &y1, int &z0, int &z1)
+// Bit 31 of the y word marks a folded y-face tile (both y faces of the PML, see Lane).
+__device__ __forceinline__ void unpack(int4 w, int &x0, int &x1, int &y0, int &y1, int &z0, int &z1, bool &fold)
 {
   x0 = w.x & 0xffff; x1 = (unsigned)w.x >> 16;
-  y0 = w.y & 0xffff; y1 = (unsigned)w.y >> 16;
+  y0 = w.y & 0xffff; y1 = ((unsigned)w.y >> 16) & 0x7fff;
+  fold = w.y < 0;
   z0 = w.z; z1 = w.w;
 }
 
 struct Coef {
   float c1[H], c2[H];        // 10th-order rows of the FD tables (entries 31..35)
+  float s2;                  // 2 * sum(c2)
   float ix, iy, iz;          // 1/dx, 1/dy, 1/dz
   float ixx, iyy, izz;       // 1/dx^2, 1/dy^2, 1/dz^2
   float dt;
@@ -125,6 +133,78 @@ struct GradArgs {
   const int4 *work;
 };
 
+// A lane's fixed position in its tile and its in-plane offsets.
+//
+// In a folded tile (the two y faces of the PML, each at most 27 cells wide) lanes 0..31 cover
+// y in [y0, y0+32) and lanes 32..63 cover [yb, yb+32) with yb = ny - pml, and the LDS row holds
+// the two strips side by side, each with its own 5-cell halos (columns [0,37) and [37,74)).
+// Loads of u are exact up to 5 cells past the tile (the halo the stencil reads) and clamped
+// beyond; per-cell loads (model, previous wavefield, PML state) of lanes outside the tile are
+// clamped into the tile, so idle lanes add no memory traffic.
+template <class T>
+struct Lane {
+  int tx, ty;            // wavefront (x group) and lane in the workgroup
+  int y, Y;              // y of the lane and its LDS column
+  int yc;                // y clamped into the tile (per-cell loads)
+  bool yon;              // the lane's y is inside the tile
+  bool fold;             // folded y-face tile (uniform)
+  unsigned qoff[T::RX];  // in-plane byte offsets (4-byte cells) of the lane's rows for u
+  unsigned off[T::RX];   // the same, clamped into the tile, for per-cell fields
+  unsigned hxo[T::NXH];  // x-halo byte offsets
+  unsigned hyo;          // y-halo byte offset
+  bool hx_on[T::NXH];    // wave-uniform
+  bool hy_on;
+
+  __device__ __forceinline__ Lane(const Geo &g, int x0, int x1, int y0, int y1, bool fold_) : fold(fold_)
+  {
+    ty = threadIdx.x;
+    tx = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    int yq;  // y for u loads
+    if (!fold) {
+      y = y0 + ty;
+      Y = H + ty;
+      yon = y < y1;
+      yq = min(y, y1 + H - 1);
+      yc = min(y, y1 - 1);
+    } else {
+      const int s = ty >> 5, j = ty & 31, w = y1 - y0;
+      const int ys = s ? g.ny - g.pml : y0;
+      y = ys + j;
+      Y = H + ty + (s ? H : 0);
+      yon = j < w;
+      yq = ys + min(j, w + H - 1);
+      yc = ys + min(j, w - 1);
+    }
+#pragma unroll
+    for (int r = 0; r < T::RX; r++) {
+      const int x = x0 + tx + T::WX * r;
+      qoff[r] = 4u * ((unsigned)min(x, x1 + H - 1) * g.ny + yq);
+      off[r] = 4u * ((unsigned)min(x, x1 - 1) * g.ny + yc);
+    }
+#pragma unroll
+    for (int i = 0; i < T::NXH; i++) {
+      const int h = tx + T::WX * i;
+      hx_on[i] = h < 2 * H;
+      const int X = h < H ? h : T::TX + h;
+      hxo[i] = 4u * ((unsigned)min(x0 - H + X, x1 + H - 1) * g.ny + yq);
+    }
+    const int t = tx * TY + ty;
+    hy_on = t < T::TX * 2 * H;
+    const int row = t / (2 * H), col = t - row * (2 * H);
+    int gy;
+    if (!fold) gy = min(y0 - H + (col < H ? col : TY + col), y1 + H - 1);
+    else gy = col < H ? y0 - H + col : g.ny - g.pml - H + (col - H);
+    hyo = 4u * ((unsigned)min(x0 + row, x1 + H - 1) * g.ny + gy);
+  }
+
+  // LDS column of y-halo value `col` (0..9) of this lane's load.
+  __device__ __forceinline__ static int halo_col(int col, bool fold)
+  {
+    // folded: strip A's left halo in columns 0..4, strip B's in 37..41
+    return !fold ? (col < H ? col : TY + col) : (col < H ? col : 32 + col);
+  }
+};
+
 // Global loads of u for one plane of a tile: the lane's queue front (plane z+5) and its share
 // of the x and y halos of plane z.
 template <class S, class T>
@@ -133,183 +213,201 @@ struct PlaneLoad {
   float hx[T::NXH];
   float hy;
 
-  __device__ __forceinline__ void issue(const typename S::T *u, const Geo &g, int z, int x0, int y0, int tx, int ty)
+  __device__ __forceinline__ void issue(const typename S::T *u, const Geo &g, const Lane<T> &L, int z)
   {
-    const int y = y0 + ty;
-    const bool yin = y < g.ny;
+    const typename S::T *uq = u + (size_t)(z + H) * g.pln;
+    const typename S::T *uz = u + (size_t)z * g.pln;
 #pragma unroll
-    for (int r = 0; r < T::RX; r++) {
-      const int x = x0 + tx + T::WX * r;
-      qn[r] = (yin && x < g.nx) ? S::ld(u, cell(g, z + H, x, y)) : 0.f;
-    }
+    for (int r = 0; r < T::RX; r++) qn[r] = S::ld(uq, L.qoff[r]);
 #pragma unroll
-    for (int i = 0; i < T::NXH; i++) {
-      const int h = tx + T::WX * i;
-      const int X = h < H ? h : T::TX + h;
-      const int gx = x0 - H + X;
-      hx[i] = (h < 2 * H && yin && gx < g.nx) ? S::ld(u, cell(g, z, gx, y)) : 0.f;
-    }
-    const int t = tx * TY + ty;
-    hy = 0.f;
-    if (t < T::TX * 2 * H) {
-      const int row = t / (2 * H), col = t - row * (2 * H);
-      const int gy = y0 - H + (col < H ? col : TY + col), gx = x0 + row;
-      if (gy < g.ny && gx < g.nx) hy = S::ld(u, cell(g, z, gx, gy));
-    }
+    for (int i = 0; i < T::NXH; i++)
+      if (L.hx_on[i]) hx[i] = S::ld(uz, L.hxo[i]);
+    if (L.hy_on) hy = S::ld(uz, L.hyo);
   }
 
   // Write plane z (centre values from the queue, halos from this load) into an LDS buffer.
-  __device__ __forceinline__ void stage(float (*sm)[T::LY], const float (&q)[T::RX][2 * H + 1], int tx, int ty) const
+  __device__ __forceinline__ void stage(float (*sm)[T::LY], const float (&centre)[T::RX], const Lane<T> &L) const
   {
 #pragma unroll
-    for (int r = 0; r < T::RX; r++) sm[H + tx + T::WX * r][H + ty] = q[r][H];
+    for (int r = 0; r < T::RX; r++) sm[H + L.tx + T::WX * r][L.Y] = centre[r];
 #pragma unroll
     for (int i = 0; i < T::NXH; i++) {
-      const int h = tx + T::WX * i;
-      if (h < 2 * H) sm[h < H ? h : T::TX + h][H + ty] = hx[i];
+      const int h = L.tx + T::WX * i;
+      if (L.hx_on[i]) sm[h < H ? h : T::TX + h][L.Y] = hx[i];
     }
-    const int t = tx * TY + ty;
-    if (t < T::TX * 2 * H) {
+    if (L.hy_on) {
+      const int t = L.tx * TY + L.ty;
       const int row = t / (2 * H), col = t - row * (2 * H);
-      sm[H + row][col < H ? col : TY + col] = hy;
+      sm[H + row][Lane<T>::halo_col(col, L.fold)] = hy;
     }
   }
 };
 
 template <class S, class T>
-__device__ __forceinline__ void prime_queue(float (&q)[T::RX][2 * H + 1], const typename S::T *u, const Geo &g, int z0,
-                                            int x0, int y, int tx)
+__device__ __forceinline__ void prime_queue(float (&q)[T::RX][2 * H + 1], const typename S::T *u, const Geo &g,
+                                            const Lane<T> &L, int z0)
 {
 #pragma unroll
-  for (int r = 0; r < T::RX; r++) {
-    const int x = x0 + tx + T::WX * r;
+  for (int j = 0; j < 2 * H; j++) {
+    const typename S::T *up = u + (size_t)(z0 - H + j) * g.pln;
 #pragma unroll
-    for (int j = 0; j < 2 * H; j++) q[r][j] = (y < g.ny && x < g.nx) ? S::ld(u, cell(g, z0 - H + j, x, y)) : 0.f;
+    for (int r = 0; r < T::RX; r++) q[r][j] = S::ld(up, L.qoff[r]);
   }
 }
 
-template <class T>
-__device__ __forceinline__ void shift_queue(float (&q)[T::RX][2 * H + 1], const float (&qn)[T::RX])
-{
-#pragma unroll
-  for (int r = 0; r < T::RX; r++) {
-#pragma unroll
-    for (int j = 0; j < 2 * H; j++) q[r][j] = q[r][j + 1];
-    q[r][2 * H] = qn[r];
+// The z queue rotates instead of shifting: the march is unrolled by its length (11), and at
+// unrolled position `it` the value of plane z-5+j sits in slot (j + it) % 11, so the only
+// register traffic per plane is the new front landing in the freed slot.
+constexpr int QN = 2 * H + 1;
+
+template <int I>
+using ic = std::integral_constant<int, I>;
+
+// f(ic<I>) for I = 0.. while I < remaining, fully unrolled at compile time.
+template <int I, int N>
+struct Unroll {
+  template <class F>
+  __device__ __forceinline__ static void run(F &f, int remaining)
+  {
+    if (I < remaining) {
+      f(ic<I>{});
+      Unroll<I + 1, N>::run(f, remaining);
+    }
   }
-}
+};
+template <int N>
+struct Unroll<N, N> {
+  template <class F>
+  __device__ __forceinline__ static void run(F &, int)
+  {
+  }
+};
 
 template <class S, class T>
 __global__ __launch_bounds__(T::NT) void k_gradmax(GradArgs<S> a)
 {
   __shared__ float sm[2][T::LX][T::LY];
   __shared__ float red[T::WX];
+  __shared__ int redbad[T::WX];
   const Geo &g = a.g;
-  const int ty = threadIdx.x, tx = threadIdx.y;
   int x0, x1, y0, y1, z0, z1;
-  unpack(a.work[blockIdx.x], x0, x1, y0, y1, z0, z1);
+  bool fold;
+  unpack(a.work[blockIdx.x], x0, x1, y0, y1, z0, z1, fold);
   if (z0 >= z1) return;
-  const int y = y0 + ty;
-  const bool ycmp = y < y1;
-  float q[T::RX][2 * H + 1];
-  prime_queue<S, T>(q, a.ucur, g, z0, x0, y, tx);
-  PlaneLoad<S, T> L;
-  L.issue(a.ucur, g, z0, x0, y0, tx, ty);
+  const Lane<T> L(g, x0, x1, y0, y1, fold);
+  const bool ycmp = L.yon;
+  bool rows_on[T::RX];
 #pragma unroll
-  for (int r = 0; r < T::RX; r++) q[r][2 * H] = L.qn[r];
-  L.stage(sm[0], q, tx, ty);
-  __syncthreads();
-  float m = 0.f;
-  int bad = 0;
-  int b = 0;
-  for (int z = z0; z < z1; z++) {
-    const bool more = z + 1 < z1;
-    if (more) L.issue(a.ucur, g, z + 1, x0, y0, tx, ty);
-    float (*s)[T::LY] = sm[b];
+  for (int r = 0; r < T::RX; r++) rows_on[r] = ycmp && x0 + L.tx + T::WX * r < x1;
+  float q[T::RX][QN];
+  prime_queue<S, T>(q, a.ucur, g, L, z0);
+  PlaneLoad<S, T> P;
+  P.issue(a.ucur, g, L, z0);
+  float centre[T::RX];
 #pragma unroll
-    for (int r = 0; r < T::RX; r++) {
-      const int X = H + tx + T::WX * r, Y = H + ty;
-      const int x = x0 + tx + T::WX * r;
-      float ux = 0.f, uy = 0.f, uz = 0.f;
-#pragma unroll
-      for (int k = 1; k <= H; k++) {
-        ux = fmaf(a.c.c1[k - 1], s[X + k][Y] - s[X - k][Y], ux);
-        uy = fmaf(a.c.c1[k - 1], s[X][Y + k] - s[X][Y - k], uy);
-        uz = fmaf(a.c.c1[k - 1], q[r][H + k] - q[r][H - k], uz);
-      }
-      if (ycmp && x < x1) {
-        const float ax = fabsf(ux * a.c.ix), ay = fabsf(uy * a.c.iy), az = fabsf(uz * a.c.iz);
-        m = (ax >= m) ? ax : m;
-        m = (ay >= m) ? ay : m;
-        m = (az >= m) ? az : m;
-        bad |= !(ax <= 3.4e38f) | !(ay <= 3.4e38f) | !(az <= 3.4e38f);
-      }
-    }
-    if (more) {
-      shift_queue<T>(q, L.qn);
-      L.stage(sm[b ^ 1], q, tx, ty);
-      __syncthreads();
-      b ^= 1;
-    }
+  for (int r = 0; r < T::RX; r++) {
+    q[r][2 * H] = P.qn[r];
+    centre[r] = q[r][H];
   }
+  P.stage(sm[0], centre, L);
+  __syncthreads();
+  // per-axis maxima of |sum_m c1_m (u+m - u-m)|; fmaxf ignores NaN like the reference's
+  // '>=' scan (CPU_function.c:18), and x*0 accumulates NaN for any non-finite gradient.
+  float mx = 0.f, my = 0.f, mz = 0.f, nanacc = 0.f;
+  int b = 0;
+  int zb = z0;
+  auto plane = [&](auto itc) {
+    constexpr int it = decltype(itc)::value;
+    {
+      const int z = zb + it;
+      const bool more = z + 1 < z1;
+      if (more) P.issue(a.ucur, g, L, z + 1);
+      float (*s)[T::LY] = sm[b];
+#pragma unroll
+      for (int r = 0; r < T::RX; r++) {
+        const int X = H + L.tx + T::WX * r, Y = L.Y;
+        float ux = 0.f, uy = 0.f, uz = 0.f;
+#pragma unroll
+        for (int k = 1; k <= H; k++) {
+          ux = fmaf(a.c.c1[k - 1], s[X + k][Y] - s[X - k][Y], ux);
+          uy = fmaf(a.c.c1[k - 1], s[X][Y + k] - s[X][Y - k], uy);
+          uz = fmaf(a.c.c1[k - 1], q[r][(H + k + it) % QN] - q[r][(H - k + it) % QN], uz);
+        }
+        if (rows_on[r]) {
+          mx = fmaxf(mx, fabsf(ux));
+          my = fmaxf(my, fabsf(uy));
+          mz = fmaxf(mz, fabsf(uz));
+          nanacc = fmaf(ux, 0.f, nanacc);
+        }
+      }
+      if (more) {
+#pragma unroll
+        for (int r = 0; r < T::RX; r++) {
+          q[r][it % QN] = P.qn[r];
+          centre[r] = q[r][(H + 1 + it) % QN];
+        }
+        P.stage(sm[b ^ 1], centre, L);
+        __syncthreads();
+        b ^= 1;
+      }
+    }
+  };
+  for (; zb < z1; zb += QN) Unroll<0, QN>::run(plane, z1 - zb);
+  // scaling by 1/d is monotonic, so max(|g|)/d = max(|g|/d)
+  float m = fmaxf(fmaxf(mx * a.c.ix, my * a.c.iy), mz * a.c.iz);
+  int bad = !(nanacc == 0.f) || !(m <= 3.4e38f);
   // workgroup max -> one atomic per workgroup
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
-    const float o = __shfl_xor(m, off, 64);
-    m = (o >= m) ? o : m;
+    m = fmaxf(m, __shfl_xor(m, off, 64));
+    bad |= __shfl_xor(bad, off, 64);
   }
-  if (ty == 0) red[tx] = m;
+  if (L.ty == 0) {
+    red[L.tx] = m;
+    redbad[L.tx] = bad;
+  }
   __syncthreads();
-  if (tx == 0 && ty == 0) {
+  if (L.tx == 0 && L.ty == 0) {
     float v = red[0];
+    int bb = redbad[0];
 #pragma unroll
-    for (int w = 1; w < T::WX; w++) v = (red[w] >= v) ? red[w] : v;
+    for (int w = 1; w < T::WX; w++) {
+      v = fmaxf(v, red[w]);
+      bb |= redbad[w];
+    }
     atomicMax(a.gmax, __float_as_uint(v));
+    if (bb) atomicOr(a.flag, 1);
   }
-  if (bad) atomicOr(a.flag, 1);
 }
 
-// Per-cell inputs of k_update other than u_cur: model, and either u_prev (interior cells)
-// or the seven PML state values (PML cells).
+// Per-cell inputs of k_update other than u_cur: model, and either u_prev (interior tiles) or
+// the seven PML state values (PML tiles).
 template <class S, class T, bool PML>
 struct CellLoad {
   static constexpr int NS = PML ? 7 : 1;
   float v2[T::RX], e[T::RX], ed[T::RX];
   float st[T::RX][NS];  // interior: st[0] = u_prev; PML: w1p w1c pp pc w3p w3c w21
 
-  __device__ __forceinline__ void issue(const UpdArgs<S> &a, int z, int x0, int x1, int y, bool ycmp, int tx)
+  __device__ __forceinline__ void issue(const UpdArgs<S> &a, const Lane<T> &L, int z)
   {
-    const Geo &g = a.g;
+    const size_t zo = (size_t)z * a.g.pln;
+    const float *pv = a.vdt2 + zo, *pe = a.eps + zo, *pd = a.epsd + zo;
 #pragma unroll
     for (int r = 0; r < T::RX; r++) {
-      const int x = x0 + tx + T::WX * r;
-      const unsigned i = cell(g, z, x, y);
-      const bool cmp = ycmp && x < x1;
-      v2[r] = cmp ? ld_model(a.vdt2, i) : 0.f;
-      e[r] = cmp ? ld_model(a.eps, i) : 0.f;
-      ed[r] = cmp ? ld_model(a.epsd, i) : 0.f;
-      if constexpr (!PML) {
-        st[r][0] = cmp ? S::ld_stream(a.unext, i) : 0.f;
-      } else {
-        const bool inner = x >= g.pml && x < g.nx - g.pml && y >= g.pml && y < g.ny - g.pml && z >= g.pml &&
-                           z < g.nz - g.pml;
-        if (cmp && inner) {
-          st[r][0] = S::ld_stream(a.unext, i);
+      v2[r] = ld_model(pv, L.off[r]);
+      e[r] = ld_model(pe, L.off[r]);
+      ed[r] = ld_model(pd, L.off[r]);
+    }
+    if constexpr (!PML) {
+      const typename S::T *pu = a.unext + zo;
 #pragma unroll
-          for (int j = 1; j < NS; j++) st[r][j] = 0.f;
-        } else if (cmp) {
-          st[r][0] = S::ld_stream(a.w1p, i);
-          st[r][1] = S::ld_stream(a.w1c, i);
-          st[r][2] = S::ld_stream(a.pp, i);
-          st[r][3] = S::ld_stream(a.pc, i);
-          st[r][4] = S::ld_stream(a.w3p, i);
-          st[r][5] = S::ld_stream(a.w3c, i);
-          st[r][6] = S::ld_stream(a.w21, i);
-        } else {
+      for (int r = 0; r < T::RX; r++) st[r][0] = S::ld_stream(pu, L.off[r]);
+    } else {
+      const typename S::T *src[7] = {a.w1p + zo, a.w1c + zo, a.pp + zo, a.pc + zo, a.w3p + zo, a.w3c + zo, a.w21 + zo};
 #pragma unroll
-          for (int j = 0; j < NS; j++) st[r][j] = 0.f;
-        }
-      }
+      for (int j = 0; j < 7; j++)
+#pragma unroll
+        for (int r = 0; r < T::RX; r++) st[r][j] = S::ld_stream(src[j], L.off[r]);
     }
   }
 };
@@ -319,124 +417,185 @@ __global__ __launch_bounds__(T::NT) void k_update(UpdArgs<S> a)
 {
   __shared__ float sm[2][T::LX][T::LY];
   const Geo &g = a.g;
-  const int ty = threadIdx.x, tx = threadIdx.y;
   int x0, x1, y0, y1, z0, z1;
-  unpack(a.work[blockIdx.x], x0, x1, y0, y1, z0, z1);
+  bool fold;
+  unpack(a.work[blockIdx.x], x0, x1, y0, y1, z0, z1, fold);
   if (z0 >= z1) return;
-  const int y = y0 + ty;
-  const bool ycmp = y < y1;
+  const Lane<T> L(g, x0, x1, y0, y1, fold);
+  const int y = L.y;
+  const bool ycmp = L.yon;
   const int nx = g.nx, ny = g.ny, nz = g.nz, pml = g.pml;
-  const bool yint = y >= pml && y < ny - pml;
+  bool rows_on[T::RX];
+#pragma unroll
+  for (int r = 0; r < T::RX; r++) rows_on[r] = ycmp && x0 + L.tx + T::WX * r < x1;
 
   // S threshold: |grad u| > 0.01*G compared in double (GPU_kernel.cu:99); for a float
   // magnitude this equals mode > rd(0.01*G).
   const float G = __uint_as_float(*a.gmax);
   const float thr = __double2float_rd(1.0e-02 * (double)G);
+  // source box and receiver plane tests, hoisted to the workgroup and the plane (uniform)
+  const bool src_tile = a.src_on && a.sb_x0 < x1 && a.sb_x1 >= x0 &&
+                        (fold || (a.sb_y0 < y1 && a.sb_y1 >= y0)) && a.sb_z0 < z1 && a.sb_z1 >= z0;
+  const bool rec_tile = a.sz >= z0 && a.sz < z1;
 
-  float q[T::RX][2 * H + 1];
-  prime_queue<S, T>(q, a.ucur, g, z0, x0, y, tx);
-  PlaneLoad<S, T> L;
-  L.issue(a.ucur, g, z0, x0, y0, tx, ty);
-  CellLoad<S, T, PML> C, Cn;
-  C.issue(a, z0, x0, x1, y, ycmp, tx);
+  // PML face family of the tile (uniform) and the per-row / per-lane coefficients of x / y faces
+  int fam = 0;
+  float4 trow[T::RX];
+  float sgn = 1.f;
+  if constexpr (PML) {
+    fam = (x1 <= pml || x0 >= nx - pml) ? 0 : ((y1 <= pml || y0 >= ny - pml) ? 1 : 2);
+    if (fam == 0) {
+      sgn = x1 <= pml ? 1.f : -1.f;
 #pragma unroll
-  for (int r = 0; r < T::RX; r++) q[r][2 * H] = L.qn[r];
-  L.stage(sm[0], q, tx, ty);
+      for (int r = 0; r < T::RX; r++) {
+        const int x = min(x0 + L.tx + T::WX * r, x1 - 1);
+        trow[r] = a.pmltab[x < pml ? pml - x : x - nx + pml];
+      }
+    } else if (fam == 1) {
+      const int yc = L.yc;  // per lane: a folded tile holds both y faces
+      sgn = yc < pml ? 1.f : -1.f;
+      const float4 t = a.pmltab[pml + (yc < pml ? pml - yc : yc - ny + pml)];
+#pragma unroll
+      for (int r = 0; r < T::RX; r++) trow[r] = t;
+    } else {
+      sgn = z1 <= pml ? 1.f : -1.f;
+    }
+  }
+
+  float q[T::RX][QN];
+  prime_queue<S, T>(q, a.ucur, g, L, z0);
+  PlaneLoad<S, T> P;
+  P.issue(a.ucur, g, L, z0);
+  CellLoad<S, T, PML> C, Cn;
+  C.issue(a, L, z0);
+  float centre[T::RX];
+#pragma unroll
+  for (int r = 0; r < T::RX; r++) {
+    q[r][2 * H] = P.qn[r];
+    centre[r] = q[r][H];
+  }
+  P.stage(sm[0], centre, L);
   __syncthreads();
   int b = 0;
-  for (int z = z0; z < z1; z++) {
-    const bool more = z + 1 < z1;
-    if (more) {
-      L.issue(a.ucur, g, z + 1, x0, y0, tx, ty);
-      Cn.issue(a, z + 1, x0, x1, y, ycmp, tx);
-    }
-    float (*s)[T::LY] = sm[b];
-    const bool zint = z >= pml && z < nz - pml;
-#pragma unroll
-    for (int r = 0; r < T::RX; r++) {
-      const int X = H + tx + T::WX * r, Y = H + ty;
-      const int x = x0 + tx + T::WX * r;
-      if (!(ycmp && x < x1)) continue;
-      const unsigned i = cell(g, z, x, y);
-      const float c0 = q[r][H];
-      const float c2 = c0 + c0;
-      float Uxx = 0.f, Uyy = 0.f, Uzz = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
-#pragma unroll
-      for (int k = 1; k <= H; k++) {
-        const float ax = s[X + k][Y], bx = s[X - k][Y];
-        const float ay = s[X][Y + k], by = s[X][Y - k];
-        const float az = q[r][H + k], bz = q[r][H - k];
-        Uxx = fmaf(a.c.c2[k - 1], (ax - c2) + bx, Uxx);
-        Uyy = fmaf(a.c.c2[k - 1], (ay - c2) + by, Uyy);
-        Uzz = fmaf(a.c.c2[k - 1], (az - c2) + bz, Uzz);
-        ux = fmaf(a.c.c1[k - 1], ax - bx, ux);
-        uy = fmaf(a.c.c1[k - 1], ay - by, uy);
-        uz = fmaf(a.c.c1[k - 1], az - bz, uz);
+  int zb = z0;
+  auto plane = [&](auto itc) {
+    constexpr int it = decltype(itc)::value;
+    {
+      const int z = zb + it;
+      const bool more = z + 1 < z1;
+      if (more) {
+        P.issue(a.ucur, g, L, z + 1);
+        Cn.issue(a, L, z + 1);
       }
-      Uxx *= a.c.ixx; Uyy *= a.c.iyy; Uzz *= a.c.izz;
-      ux *= a.c.ix; uy *= a.c.iy; uz *= a.c.iz;
-      const float e = C.e[r], ed = C.ed[r], v2 = C.v2[r];
-      // scalar operator (GPU_kernel.cu:93-111)
-      float Sv = 1.f;
-      const float mode = sqrtf(ux * ux + uy * uy + uz * uz);
-      if (mode > thr) {
-        const float inv = 1.f / mode;
-        const float nxv = ux * inv, nyv = uy * inv, nzv = uz * inv;
-        const float qa = nxv * nxv + nyv * nyv;
-        const float den = 1.f + 2.f * e * qa;
-        Sv = 0.5f * (1.f + sqrtf(1.f - (8.f * ed * qa * nzv * nzv) / (den * den)));
+      float (*s)[T::LY] = sm[b];
+      const bool src_plane = src_tile && z >= a.sb_z0 && z <= a.sb_z1;
+      const bool rec_plane = rec_tile && z == a.sz;
+      float4 tz;
+      if constexpr (PML) {
+        if (fam == 2) tz = a.pmltab[2 * pml + (z < pml ? pml - z : z - nz + pml)];
       }
-      const float e1 = 1.f + 2.f * e;
-      float un;
-      const bool xint = x >= pml && x < nx - pml;
-      if (!PML || (xint && yint && zint)) {
-        // GPU_kernel.cu:558-565
-        un = (2.f * c0 - C.st[r][0]) + ((e1 * (Uxx + Uyy) + Uzz) * Sv) * v2;
-      } else if constexpr (PML) {
-        // GPU_kernel.cu:566-637; x faces own edges and corners, then y, then z.
-        int fam, j;
-        float sgn;
-        if (!xint) {
-          fam = 0; j = x < pml ? pml - x : x - nx + pml; sgn = x < pml ? 1.f : -1.f;
-        } else if (!yint) {
-          fam = 1; j = y < pml ? pml - y : y - ny + pml; sgn = y < pml ? 1.f : -1.f;
-        } else {
-          fam = 2; j = z < pml ? pml - z : z - nz + pml; sgn = z < pml ? 1.f : -1.f;
+      const size_t zo = (size_t)z * g.pln;
+#pragma unroll
+      for (int r = 0; r < T::RX; r++) {
+        if (!rows_on[r]) continue;
+        const int X = H + L.tx + T::WX * r, Y = L.Y;
+        const int x = x0 + L.tx + T::WX * r;
+        const float c0 = q[r][(H + it) % QN];
+        // second derivatives as sum_m c2_m (u+m + u-m) - 2 u0 sum_m c2_m, first derivatives as
+        // sum_m c1_m (u+m - u-m); each pair's sum and difference are shared.
+        float Lxx = 0.f, Lyy = 0.f, Lzz = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
+#pragma unroll
+        for (int k = 1; k <= H; k++) {
+          const float ax = s[X + k][Y], bx = s[X - k][Y];
+          const float ay = s[X][Y + k], by = s[X][Y - k];
+          const float az = q[r][(H + k + it) % QN], bz = q[r][(H - k + it) % QN];
+          Lxx = fmaf(a.c.c2[k - 1], ax + bx, Lxx);
+          Lyy = fmaf(a.c.c2[k - 1], ay + by, Lyy);
+          Lzz = fmaf(a.c.c2[k - 1], az + bz, Lzz);
+          ux = fmaf(a.c.c1[k - 1], ax - bx, ux);
+          uy = fmaf(a.c.c1[k - 1], ay - by, uy);
+          uz = fmaf(a.c.c1[k - 1], az - bz, uz);
         }
-        const float4 t = a.pmltab[fam * pml + j];
-        const float B = Sv * v2;
-        float Cn_, Unn, Tg, gn;
-        if (fam == 0) { Cn_ = e1 * B; Unn = Uxx; Tg = Uzz + e1 * Uyy; gn = ux; }
-        else if (fam == 1) { Cn_ = e1 * B; Unn = Uyy; Tg = Uzz + e1 * Uxx; gn = uy; }
-        else { Cn_ = B; Unn = Uzz; Tg = e1 * (Uxx + Uyy); gn = uz; }
-        const float w1p = C.st[r][0], w1c = C.st[r][1], pp = C.st[r][2], pc = C.st[r][3];
-        const float w3p = C.st[r][4], w3c = C.st[r][5], w21 = C.st[r][6];
-        const float w1n = S::rnd(Cn_ * Unn - t.x * w1c + t.y * w1p);
-        const float pn = S::rnd(-t.x * pc + t.y * pp + sgn * (Cn_ * t.w * gn));
-        const float w21n = S::rnd(a.c.dt * pn + w21 * t.z);
-        const float w3n = S::rnd(B * Tg + 2.f * w3c - w3p);
-        S::st(a.w1p, i, w1n);
-        S::st(a.pp, i, pn);
-        S::st(a.w21, i, w21n);
-        S::st(a.w3p, i, w3n);
-        un = w1n + w21n + w3n;
+        const float Uxx = fmaf(-a.c.s2, c0, Lxx) * a.c.ixx;
+        const float Uyy = fmaf(-a.c.s2, c0, Lyy) * a.c.iyy;
+        const float Uzz = fmaf(-a.c.s2, c0, Lzz) * a.c.izz;
+        ux *= a.c.ix; uy *= a.c.iy; uz *= a.c.iz;
+        const float e = C.e[r], ed = C.ed[r], v2 = C.v2[r];
+        // scalar operator (GPU_kernel.cu:93-111): n = grad u / (|grad u| + 2.22e-17) above the
+        // threshold, S = (1 + sqrt(1 - 8 (eps-del) (nx^2+ny^2) nz^2 / (1 + 2 eps (nx^2+ny^2))^2)) / 2.
+        float Sv = 1.f;
+        const float mode = __builtin_amdgcn_sqrtf(fmaf(ux, ux, fmaf(uy, uy, uz * uz)));
+        if (mode > thr) {
+          const float inv = __builtin_amdgcn_rcpf(mode + 2.22e-17f);
+          const float nxv = ux * inv, nyv = uy * inv, nzv = uz * inv;
+          const float qa = fmaf(nxv, nxv, nyv * nyv);
+          const float den = fmaf(2.f * e, qa, 1.f);
+          const float X8 = 8.f * ed * qa * (nzv * nzv) * __builtin_amdgcn_rcpf(den * den);
+          Sv = 0.5f + 0.5f * __builtin_amdgcn_sqrtf(1.f - X8);
+        }
+        const float e1 = fmaf(2.f, e, 1.f);
+        const unsigned o = L.off[r];
+        float un;
+        if constexpr (!PML) {
+          // GPU_kernel.cu:558-565
+          un = (2.f * c0 - C.st[r][0]) + ((e1 * (Uxx + Uyy) + Uzz) * Sv) * v2;
+        } else {
+          // GPU_kernel.cu:566-637
+          const float4 t = fam == 2 ? tz : trow[r];
+          const float B = Sv * v2;
+          float Cn_, Unn, Tg, gn;
+          if (fam == 0) { Cn_ = e1 * B; Unn = Uxx; Tg = Uzz + e1 * Uyy; gn = ux; }
+          else if (fam == 1) { Cn_ = e1 * B; Unn = Uyy; Tg = Uzz + e1 * Uxx; gn = uy; }
+          else { Cn_ = B; Unn = Uzz; Tg = e1 * (Uxx + Uyy); gn = uz; }
+          const float w1p = C.st[r][0], w1c = C.st[r][1], pp = C.st[r][2], pc = C.st[r][3];
+          const float w3p = C.st[r][4], w3c = C.st[r][5], w21 = C.st[r][6];
+          const float w1n = S::rnd(Cn_ * Unn - t.x * w1c + t.y * w1p);
+          const float pn = S::rnd(-t.x * pc + t.y * pp + sgn * (Cn_ * t.w * gn));
+          const float w21n = S::rnd(a.c.dt * pn + w21 * t.z);
+          const float w3n = S::rnd(B * Tg + 2.f * w3c - w3p);
+          S::st(a.w1p + zo, o, w1n);
+          S::st(a.pp + zo, o, pn);
+          S::st(a.w21 + zo, o, w21n);
+          S::st(a.w3p + zo, o, w3n);
+          un = w1n + w21n + w3n;
+        }
+        if (src_plane && x >= a.sb_x0 && x <= a.sb_x1 && y >= a.sb_y0 && y <= a.sb_y1) {
+          const int bx = a.sb_x1 - a.sb_x0 + 1, by = a.sb_y1 - a.sb_y0 + 1;
+          const float sv = a.sbox[((long)(z - a.sb_z0) * bx + (x - a.sb_x0)) * by + (y - a.sb_y0)];
+          un = fmaf(sv, a.wk, un);  // u3 += source*wavelet, contracted as the CUDA build does
+        }
+        un = S::rnd(un);
+        S::st(a.unext + zo, o, un);
+        if (rec_plane && x >= pml && x < nx - pml && y >= pml && y < ny - pml)
+          a.rec[(x - pml) * (ny - 2 * pml) + (y - pml)] = un;
       }
-      if (a.src_on && z >= a.sb_z0 && z <= a.sb_z1 && x >= a.sb_x0 && x <= a.sb_x1 && y >= a.sb_y0 &&
-          y <= a.sb_y1) {
-        const int bx = a.sb_x1 - a.sb_x0 + 1, by = a.sb_y1 - a.sb_y0 + 1;
-        const float sv = a.sbox[((long)(z - a.sb_z0) * bx + (x - a.sb_x0)) * by + (y - a.sb_y0)];
-        un = fmaf(sv, a.wk, un);  // u3 += source*wavelet, contracted as the CUDA build does
+      if (more) {
+#pragma unroll
+        for (int r = 0; r < T::RX; r++) {
+          q[r][it % QN] = P.qn[r];
+          centre[r] = q[r][(H + 1 + it) % QN];
+        }
+        P.stage(sm[b ^ 1], centre, L);
+        C = Cn;
+        __syncthreads();
+        b ^= 1;
       }
-      un = S::rnd(un);
-      S::st(a.unext, i, un);
-      if (z == a.sz && xint && yint) a.rec[(x - pml) * (ny - 2 * pml) + (y - pml)] = un;
     }
-    if (more) {
-      shift_queue<T>(q, L.qn);
-      L.stage(sm[b ^ 1], q, tx, ty);
-      C = Cn;
-      __syncthreads();
-      b ^= 1;
+  };
+  if constexpr (!PML) {
+    for (; zb < z1; zb += QN) Unroll<0, QN>::run(plane, z1 - zb);
+  } else {
+    // the PML kernel carries seven more state values per row; a rolled loop with a shifted
+    // queue keeps it free of spills
+    for (; zb < z1; zb++) {
+      plane(ic<0>{});
+#pragma unroll
+      for (int r = 0; r < T::RX; r++) {
+        const float f = q[r][0];
+#pragma unroll
+        for (int j = 0; j < 2 * H; j++) q[r][j] = q[r][j + 1];
+        q[r][2 * H] = f;
+      }
     }
   }
 }
@@ -455,9 +614,9 @@ __global__ void k_rim_source(const typename S::T *ucur, typename S::T *unext, Ge
     const int x = x0 + xx, y = y0 + yy, z = z0 + zz;
     const bool r5 = x >= H && x < g.nx - H && y >= H && y < g.ny - H && z >= H && z < g.nz - H;
     if (r5) continue;
-    const unsigned i = cell(g, z, x, y);
-    const float v = fmaf(sbox[t], wk, S::ld(ucur, i));
-    S::st(unext, i, S::rnd(v));
+    const unsigned b = 4u * ((unsigned)z * g.pln + (unsigned)x * g.ny + (unsigned)y);
+    const float v = fmaf(sbox[t], wk, S::ld(ucur, b));
+    S::st(unext, b, S::rnd(v));
   }
 }
 
@@ -470,11 +629,12 @@ __global__ void k_snapshot(const typename S::T *u, float *out, Geo g)
   const int y = blockIdx.x * blockDim.x + threadIdx.x;
   const int x = blockIdx.y, z = blockIdx.z;
   if (y >= hy) return;
-  out[((long)z * hx + x) * hy + y] = S::ld(u, cell(g, z + g.pml, x + g.pml, y + g.pml));
+  out[((long)z * hx + x) * hy + y] =
+      S::ld(u, 4u * ((unsigned)(z + g.pml) * g.pln + (unsigned)(x + g.pml) * g.ny + (unsigned)(y + g.pml)));
 }
 
 // Tile shapes the plan can launch (index = qp3d_config.tile_update / tile_grad - 1).
-#define QP3D_TILES(X) X(8, 2) X(16, 1) X(4, 4) X(8, 4) X(16, 2) X(4, 2)
+#define QP3D_TILES(X) X(8, 2) X(16, 1) X(4, 2) X(8, 1) X(12, 1) X(16, 2)
 constexpr int NTILES = 6;
 
 }  // namespace qp3d

@@ -186,8 +186,10 @@ static int upload_list(WorkList &w, const std::vector<int4> &logical)
   std::vector<int4> v = xcd_order(logical);
   w.n = (int)v.size();
   w.cells = 0;
-  for (const int4 &t : logical)
-    w.cells += (long)(((unsigned)t.x >> 16) - (t.x & 0xffff)) * (((unsigned)t.y >> 16) - (t.y & 0xffff)) * (t.w - t.z);
+  for (const int4 &t : logical) {
+    const long ny_ = (long)((((unsigned)t.y >> 16) & 0x7fff) - (t.y & 0xffff)) * (t.y < 0 ? 2 : 1);
+    w.cells += (long)(((unsigned)t.x >> 16) - (t.x & 0xffff)) * ny_ * (t.w - t.z);
+  }
   if (w.n == 0) return QP3D_OK;
   HIP_TRY(hipMalloc((void **)&w.d, sizeof(int4) * v.size()));
   HIP_TRY(hipMemcpy(w.d, v.data(), sizeof(int4) * v.size(), hipMemcpyHostToDevice));
@@ -241,10 +243,24 @@ static int build_work(qp3d_plan *p)
     int nint_tiles = 0;
     for (size_t a = 0; a < xs.size(); a++)
       for (size_t b = 0; b < ys.size(); b++) nint_tiles += xi[a] && yi[b];
-    const int nall = (int)(xs.size() * ys.size());
+    // The two y faces of an x-interior tile column share one folded tile when each face fits
+    // half a wavefront with its 5-cell halo.
+    const bool fold = pml <= 32;
+    std::vector<std::pair<int4, bool>> ptiles;  // PML xy tiles over the whole z range
+    for (size_t a = 0; a < xs.size(); a++) {
+      if (!xi[a]) {
+        for (size_t b = 0; b < ys.size(); b++)
+          ptiles.push_back({pack(xs[a].a, xs[a].b, ys[b].a, ys[b].b, 0, 0), false});
+      } else if (fold) {
+        ptiles.push_back({pack(xs[a].a, xs[a].b, H, pml, 0, 0), true});
+      } else {
+        for (size_t b = 0; b < ys.size(); b++)
+          if (!yi[b]) ptiles.push_back({pack(xs[a].a, xs[a].b, ys[b].a, ys[b].b, 0, 0), false});
+      }
+    }
     const int zmid = c.nz - 2 * pml, zpml = pml - H;
     const int zc_i = choose_zchunk(nint_tiles, zmid, occ_i * ncu, c.zchunk);
-    const int zc_p = choose_zchunk(nall - nint_tiles, c.nz - 2 * H, occ_p * ncu, c.zchunk);
+    const int zc_p = choose_zchunk((int)ptiles.size(), c.nz - 2 * H, occ_p * ncu, c.zchunk);
     std::vector<int4> it, ip;
     for (int z = pml; z < c.nz - pml; z += zc_i)
       for (size_t a = 0; a < xs.size(); a++)
@@ -252,9 +268,13 @@ static int build_work(qp3d_plan *p)
           if (xi[a] && yi[b]) it.push_back(pack(xs[a].a, xs[a].b, ys[b].a, ys[b].b, z, std::min(z + zc_i, c.nz - pml)));
     // PML tiles over the whole z range, then the interior tiles' top and bottom z faces
     for (int z = H; z < c.nz - H; z += zc_p)
-      for (size_t a = 0; a < xs.size(); a++)
-        for (size_t b = 0; b < ys.size(); b++)
-          if (!(xi[a] && yi[b])) ip.push_back(pack(xs[a].a, xs[a].b, ys[b].a, ys[b].b, z, std::min(z + zc_p, c.nz - H)));
+      for (const auto &t : ptiles) {
+        int4 w = t.first;
+        w.z = z;
+        w.w = std::min(z + zc_p, c.nz - H);
+        if (t.second) w.y = (int)((unsigned)w.y | 0x80000000u);
+        ip.push_back(w);
+      }
     for (int face = 0; face < 2; face++) {
       const int za = face == 0 ? H : c.nz - pml, zb = face == 0 ? pml : c.nz - H;
       const int zc = std::max(1, std::min(zpml, zc_p));
@@ -451,6 +471,7 @@ extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps,
   k.ixx = (float)(1.0 / ((double)cf.dx * cf.dx));
   k.iyy = (float)(1.0 / ((double)cf.dy * cf.dy));
   k.izz = (float)(1.0 / ((double)cf.dz * cf.dz));
+  k.s2 = 2.0f * (k.c2[0] + k.c2[1] + k.c2[2] + k.c2[3] + k.c2[4]);
   k.dt = cf.dt;
   // source: bounding box of the non-zero cells of the full-grid spatial source
   int z0 = cf.nz, z1 = -1, x0 = cf.nx, x1 = -1, y0 = cf.ny, y1 = -1;

@@ -170,3 +170,20 @@ def test_every_tile_shape(tile):
         orec[k] = o.field("u2")[s.sz, s.pml:s.nx - s.pml, s.pml:s.ny - s.pml]
     assert rel_l2(rec, orec) <= TOL
     assert rel_l2(u, o.field("u2")) <= TOL
+
+
+@pytest.mark.parametrize("pml", [6, 13, 32, 36])
+def test_pml_widths_and_source_in_pml(pml):
+    """PML widths around the folded y-face tiling limit (faces up to 27 cells are folded into
+    one wavefront), with source cells inside the y-face PML strips."""
+    s = qp3d.config_setup(hx=19, hy=23, nz0=17, pml=pml, nt=40, seed=8, src_depth=2)
+    src = s.source.copy()
+    src[pml + 3, s.nx // 2, 7] = 0.5          # low y face
+    src[pml + 4, s.nx // 2 + 1, s.ny - 8] = -0.4  # high y face
+    src[pml + 2, 7, s.ny // 2] = 0.3          # low x face
+    s.source = src
+    rec, st, gm, orec, o, og = run_both(s)
+    assert rel_l2(rec, orec) <= TOL
+    assert rel_l2(st[0], o.field("u2")) <= TOL
+    for which, arr in [(2, "w12"), (6, "p2"), (8, "w21")]:
+        assert rel_l2(st[which], _unified(o, s, arr)) <= TOL, (which, arr)
