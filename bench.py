@@ -91,6 +91,8 @@ def main():
     ap.add_argument("--graph", type=int, default=0)
     ap.add_argument("--tile-update", type=int, default=0)
     ap.add_argument("--tile-grad", type=int, default=0)
+    ap.add_argument("--tile-pml", type=int, default=0)
+    ap.add_argument("--streams", type=int, default=0, help="2: interior and PML updates concurrent; 1: serial")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run a 1 GiB device-to-device copy (the reference kernel for scaling HBM counters)")
     args = ap.parse_args()
@@ -108,7 +110,7 @@ def main():
     log(f"setup {s.nx}x{s.ny}x{s.nz} in {time.perf_counter() - t:.1f} s")
     ntsnap = 500
     plan = qp3d.Plan(s, snapflag=1, ntsnap=ntsnap, use_graph=bool(args.graph), tile_update=args.tile_update,
-                     tile_grad=args.tile_grad)
+                     tile_grad=args.tile_grad, tile_pml=args.tile_pml, streams=args.streams)
     log("model uploaded")
     for i in range(args.warmup):
         t = time.perf_counter()
@@ -169,7 +171,7 @@ def main():
                    "gcells_int_per_s": round(s.n_int * s.nt * args.steps / wall / 1e9, 3),
                    "ms_per_time_step": round(wall / args.steps / s.nt * 1e3, 5),
                    "ms_gradmax": round(kt["gradmax"], 5), "ms_update_interior": round(kt["update_interior"], 5),
-                   "ms_update_pml": round(kt["update_pml"], 5),
+                   "ms_update_pml": round(kt["update_pml"], 5), "ms_update_stage": round(kt["update_stage"], 5),
                    "gbs_gradmax": round(gbs.get("gradmax", 0), 1), "gbs_update_pml": round(gbs.get("update_pml", 0), 1),
                    "roofline_kernel": "k_update<interior> (24 B per interior cell)",
                    "hbm_frac_step": round(step_bytes * s.nt * args.steps / wall / 1e9 / HBM_PEAK_GBS, 4)},

@@ -39,7 +39,10 @@ struct qp3d_plan {
   long N = 0;
   Geo geo{};
   Coef coef{};
-  int tile_u = 0, tile_g = 0;
+  int tile_u = 0, tile_g = 0, tile_p = 0;
+  bool two_streams = false;
+  hipStream_t stream2 = nullptr;                          // PML updates when two_streams
+  hipEvent_t ev_grad = nullptr, ev_pml = nullptr;         // cross-stream ordering, no timing
   WorkList wl_grad, wl_int, wl_pml;
   long n_int_cells = 0, n_pml_cells = 0;
   hipStream_t stream = nullptr;
@@ -67,7 +70,7 @@ struct qp3d_plan {
   bool instrument = false;
   std::vector<hipEvent_t> ev;   // 4 per step: before gradmax, after gradmax, after interior, after PML
   std::vector<int> ev_pending;  // steps with recorded events not yet harvested
-  double t_grad = 0, t_int = 0, t_pml = 0;
+  double t_grad = 0, t_int = 0, t_pml = 0, t_upd = 0;
   long n_launch = 0;
 };
 
@@ -207,7 +210,7 @@ static int build_work(qp3d_plan *p)
   const int occ_i = occupancy_of(p->tile_u, [](auto t) {
     return reinterpret_cast<const void *>(&k_update<StoreF32, decltype(t), false>);
   });
-  const int occ_p = occupancy_of(p->tile_u, [](auto t) {
+  const int occ_p = occupancy_of(p->tile_p, [](auto t) {
     return reinterpret_cast<const void *>(&k_update<StoreF32, decltype(t), true>);
   });
   // gradient-max: uniform tiles over [5,n-5)^3
@@ -227,9 +230,7 @@ static int build_work(qp3d_plan *p)
   }
   // update: tiles aligned to the PML boundary in x and y, z split at the PML planes
   {
-    const int TX = tile_tx(p->tile_u), pml = c.pml;
-    std::vector<Seg> xs, ys;
-    std::vector<bool> xi, yi;
+    const int pml = c.pml;
     auto segs = [](std::vector<Seg> &out, std::vector<bool> &inner, int n, int pml, int w) {
       const int bounds[4] = {H, pml, n - pml, n - H};
       for (int s = 0; s < 3; s++) {
@@ -238,11 +239,15 @@ static int build_work(qp3d_plan *p)
         inner.insert(inner.end(), out.size() - before, s == 1);
       }
     };
-    segs(xs, xi, c.nx, pml, TX);
+    // interior kernel: its own x tiling of the interior; PML kernel: x tiling with its tile width
+    std::vector<Seg> xu, xs, ys;
+    std::vector<bool> xui, xi, yi;
+    segs(xu, xui, c.nx, pml, tile_tx(p->tile_u));
+    segs(xs, xi, c.nx, pml, tile_tx(p->tile_p));
     segs(ys, yi, c.ny, pml, TY);
     int nint_tiles = 0;
-    for (size_t a = 0; a < xs.size(); a++)
-      for (size_t b = 0; b < ys.size(); b++) nint_tiles += xi[a] && yi[b];
+    for (size_t a = 0; a < xu.size(); a++)
+      for (size_t b = 0; b < ys.size(); b++) nint_tiles += xui[a] && yi[b];
     // The two y faces of an x-interior tile column share one folded tile when each face fits
     // half a wavefront with its 5-cell halo.
     const bool fold = pml <= 32;
@@ -263,9 +268,9 @@ static int build_work(qp3d_plan *p)
     const int zc_p = choose_zchunk((int)ptiles.size(), c.nz - 2 * H, occ_p * ncu, c.zchunk);
     std::vector<int4> it, ip;
     for (int z = pml; z < c.nz - pml; z += zc_i)
-      for (size_t a = 0; a < xs.size(); a++)
+      for (size_t a = 0; a < xu.size(); a++)
         for (size_t b = 0; b < ys.size(); b++)
-          if (xi[a] && yi[b]) it.push_back(pack(xs[a].a, xs[a].b, ys[b].a, ys[b].b, z, std::min(z + zc_i, c.nz - pml)));
+          if (xui[a] && yi[b]) it.push_back(pack(xu[a].a, xu[a].b, ys[b].a, ys[b].b, z, std::min(z + zc_i, c.nz - pml)));
     // PML tiles over the whole z range, then the interior tiles' top and bottom z faces
     for (int z = H; z < c.nz - H; z += zc_p)
       for (const auto &t : ptiles) {
@@ -321,7 +326,8 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
     qp3d_set_error("plan_create: ntsnap must be positive when snapflag == 1");
     return QP3D_ERR_ARG;
   }
-  if (c.tile_update < 0 || c.tile_update > NTILES || c.tile_grad < 0 || c.tile_grad > NTILES) {
+  if (c.tile_update < 0 || c.tile_update > NTILES || c.tile_grad < 0 || c.tile_grad > NTILES || c.tile_pml < 0 ||
+      c.tile_pml > NTILES || c.streams < 0 || c.streams > 2) {
     qp3d_set_error("plan_create: tile shapes must be in [0,%d]", NTILES);
     return QP3D_ERR_ARG;
   }
@@ -350,8 +356,10 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
   p->N = (long)c.nx * c.ny * c.nz;
   p->geo.nx = c.nx; p->geo.ny = c.ny; p->geo.nz = c.nz; p->geo.pml = c.pml;
   p->geo.pln = (unsigned)c.nx * (unsigned)c.ny;
-  p->tile_u = c.tile_update > 0 ? c.tile_update - 1 : 1;  // 16x1 by default
+  p->tile_u = c.tile_update > 0 ? c.tile_update - 1 : 5;  // 16x2 by default
   p->tile_g = c.tile_grad > 0 ? c.tile_grad - 1 : 0;      // 8x2 by default
+  p->tile_p = c.tile_pml > 0 ? c.tile_pml - 1 : 1;        // 16x1 by default
+  p->two_streams = c.streams == 2;
 
   auto fail = [&](int rc) {
     qp3d_plan_destroy(p);
@@ -370,6 +378,17 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
   if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
     qp3d_set_error("plan_create: hipStreamCreate failed");
     return fail(QP3D_ERR_HIP);
+  }
+  if (p->two_streams) {
+    // second stream (default priority) for the PML update; ev_grad orders it after the
+    // gradient maximum, ev_pml orders the next step (and the end of every run) after it
+    if (hipStreamCreateWithFlags(&p->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&p->ev_grad, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&p->ev_pml, hipEventDisableTiming) != hipSuccess ||
+        hipEventRecord(p->ev_pml, p->stream2) != hipSuccess) {
+      qp3d_set_error("plan_create: creating the second stream or its events failed");
+      return fail(QP3D_ERR_HIP);
+    }
   }
   int rc;
   if ((rc = dmalloc((void **)&p->state, 9 * fb, "wavefields and PML state"))) return fail(rc);
@@ -406,6 +425,7 @@ extern "C" int qp3d_plan_destroy(qp3d_plan *p)
   if (!p) return QP3D_OK;
   (void)hipSetDevice(p->cfg.device);
   if (p->stream) (void)hipStreamSynchronize(p->stream);
+  if (p->stream2) (void)hipStreamSynchronize(p->stream2);
   (void)hipFree(p->state);
   (void)hipFree(p->model);
   (void)hipFree(p->pmltab);
@@ -418,6 +438,9 @@ extern "C" int qp3d_plan_destroy(qp3d_plan *p)
   (void)hipFree(p->wl_pml.d);
   for (float *d : p->snaps) (void)hipFree(d);
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
+  if (p->ev_grad) (void)hipEventDestroy(p->ev_grad);
+  if (p->ev_pml) (void)hipEventDestroy(p->ev_pml);
+  if (p->stream2) (void)hipStreamDestroy(p->stream2);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
   return QP3D_OK;
@@ -529,13 +552,16 @@ static int harvest_events(qp3d_plan *p)
 {
   for (int k : p->ev_pending) {
     hipEvent_t *e = &p->ev[4 * (size_t)k];
+    // e0..e1 gradient max; e1..e2 interior update; PML update ends at e3 and starts at e1
+    // (concurrent, two streams) or e2 (one stream)
     float a = 0, b = 0, c = 0;
     HIP_TRY(hipEventElapsedTime(&a, e[0], e[1]));
     HIP_TRY(hipEventElapsedTime(&b, e[1], e[2]));
-    HIP_TRY(hipEventElapsedTime(&c, e[2], e[3]));
+    HIP_TRY(hipEventElapsedTime(&c, p->two_streams ? e[1] : e[2], e[3]));
     p->t_grad += a;
     p->t_int += b;
     p->t_pml += c;
+    p->t_upd += p->two_streams ? std::max(b, c) : b + c;
     p->n_launch += 1;
   }
   p->ev_pending.clear();
@@ -554,8 +580,10 @@ static int launch_step(qp3d_plan *p, int k)
       for (size_t i = old; i < p->ev.size(); i++) HIP_TRY(hipEventCreate(&p->ev[i]));
     }
     e = &p->ev[4 * (size_t)k];
-    HIP_TRY(hipEventRecord(e[0], p->stream));
   }
+  // u(k) is complete only when the previous step's PML update has finished too
+  if (p->two_streams) HIP_TRY(hipStreamWaitEvent(p->stream, p->ev_pml, 0));
+  if (e) HIP_TRY(hipEventRecord(e[0], p->stream));
   GradArgs<StoreF32> ga;
   ga.g = p->geo;
   ga.c = p->coef;
@@ -566,6 +594,10 @@ static int launch_step(qp3d_plan *p, int k)
   launch_gradmax(p->tile_g, p->wl_grad, p->stream, ga);
   HIP_TRY(hipGetLastError());
   if (e) HIP_TRY(hipEventRecord(e[1], p->stream));
+  if (p->two_streams) {
+    HIP_TRY(hipEventRecord(p->ev_grad, p->stream));
+    HIP_TRY(hipStreamWaitEvent(p->stream2, p->ev_grad, 0));
+  }
 
   UpdArgs<StoreF32> ua;
   ua.g = p->geo;
@@ -592,15 +624,19 @@ static int launch_step(qp3d_plan *p, int k)
     HIP_TRY(hipGetLastError());
   }
   if (e) HIP_TRY(hipEventRecord(e[2], p->stream));
+  hipStream_t ps = p->two_streams ? p->stream2 : p->stream;
   if (p->wl_pml.n) {
     ua.work = p->wl_pml.d;
-    launch_update<StoreF32, true>(p->tile_u, p->wl_pml, p->stream, ua);
+    launch_update<StoreF32, true>(p->tile_p, p->wl_pml, ps, ua);
     HIP_TRY(hipGetLastError());
   }
   if (e) {
-    HIP_TRY(hipEventRecord(e[3], p->stream));
+    HIP_TRY(hipEventRecord(e[3], ps));
     p->ev_pending.push_back(k);
   }
+  // The rim source and the snapshot touch cells no PML tile writes (the outer 5 cells; the
+  // interior cube), so they follow the interior update on the main stream.
+  if (p->two_streams) HIP_TRY(hipEventRecord(p->ev_pml, p->stream2));
   if (p->src_rim) {
     const long nb = (long)(p->sb[1] - p->sb[0] + 1) * (p->sb[3] - p->sb[2] + 1) * (p->sb[5] - p->sb[4] + 1);
     const int blocks = (int)std::min<long>((nb + 255) / 256, 4096);
@@ -641,6 +677,8 @@ extern "C" int qp3d_plan_run(qp3d_plan *p, int k0, int k1)
     if (rc) return rc;
     p->steps_done = k + 1;
   }
+  // everything enqueued so far is ordered before later work on the plan's stream
+  if (p->two_streams) HIP_TRY(hipStreamWaitEvent(p->stream, p->ev_pml, 0));
   return QP3D_OK;
 }
 
@@ -745,7 +783,7 @@ extern "C" int qp3d_plan_set_instrument(qp3d_plan *p, int on)
 {
   if (!p) return QP3D_ERR_ARG;
   p->instrument = on != 0;
-  p->t_grad = p->t_int = p->t_pml = 0;
+  p->t_grad = p->t_int = p->t_pml = p->t_upd = 0;
   p->n_launch = 0;
   p->ev_pending.clear();
   return QP3D_OK;
@@ -760,7 +798,8 @@ extern "C" int qp3d_plan_kernel_times(qp3d_plan *p, double *ms)
   ms[0] = n ? p->t_grad / n : 0;
   ms[1] = n ? p->t_int / n : 0;
   ms[2] = n ? p->t_pml / n : 0;
-  ms[3] = (double)n;
+  ms[3] = n ? p->t_upd / n : 0;
+  ms[4] = (double)n;
   return QP3D_OK;
 }
 

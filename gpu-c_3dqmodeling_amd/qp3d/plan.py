@@ -31,12 +31,12 @@ class Plan:
     """Device-resident forward solver for one grid (qp3d_plan_* in include/qp3d.h)."""
 
     def __init__(self, s, storage="fp32", use_graph=False, snapflag=0, ntsnap=0, zchunk=0, device=0,
-                 tile_update=0, tile_grad=0):
+                 tile_update=0, tile_grad=0, tile_pml=0, streams=0):
         self.lib = load()
         self.s = s
         cfg = Config(s.nx, s.ny, s.nz, s.pml, s.nt, s.dx, s.dy, s.dz, s.dt, s.sz, STORAGE[storage],
                      int(use_graph), int(snapflag), int(ntsnap), int(zchunk), int(device),
-                     int(tile_update), int(tile_grad))
+                     int(tile_update), int(tile_grad), int(tile_pml), int(streams))
         h = C.c_void_p()
         check(self.lib.qp3d_plan_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -98,11 +98,13 @@ class Plan:
         check(self.lib.qp3d_plan_set_instrument(self.h, int(on)))
 
     def kernel_times(self):
-        """Mean ms per step of the gradient-max, interior-update and PML-update launches, and
-        the number of instrumented steps."""
-        ms = (C.c_double * 4)()
+        """Mean ms per step of the gradient-max, interior-update and PML-update launches, of the
+        whole update stage (the two updates overlap on two streams), and the number of
+        instrumented steps."""
+        ms = (C.c_double * 5)()
         check(self.lib.qp3d_plan_kernel_times(self.h, ms))
-        return {"gradmax": ms[0], "update_interior": ms[1], "update_pml": ms[2], "steps": int(ms[3])}
+        return {"gradmax": ms[0], "update_interior": ms[1], "update_pml": ms[2], "update_stage": ms[3],
+                "steps": int(ms[4])}
 
     def step_bytes(self):
         """Algorithmic bytes per step of each launch and the cells of the two update launches."""

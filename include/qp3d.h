@@ -92,8 +92,11 @@ typedef struct qp3d_config {
   int ntsnap;
   int zchunk;    /* z planes per workgroup in the stencil kernels, 0 = automatic */
   int device;    /* HIP device ordinal */
-  int tile_update; /* tile shape of the update kernel, 1..QP3D_NTILES, 0 = default */
+  int tile_update; /* tile shape of the interior update kernel, 1..QP3D_NTILES, 0 = default */
   int tile_grad;   /* tile shape of the gradient-max kernel, 1..QP3D_NTILES, 0 = default */
+  int tile_pml;    /* tile shape of the PML update kernel, 1..QP3D_NTILES, 0 = default */
+  int streams;     /* 2: run the interior and PML updates concurrently on two streams; 1: one stream;
+                      0 = default (1: on MI355X the two compete for HBM and gain nothing) */
 } qp3d_config;
 
 /* Tile shapes (wavefronts along x x rows per lane; a tile is 64 cells along y):
@@ -130,8 +133,9 @@ int qp3d_plan_dims(const qp3d_plan *p, int *hx, int *hy, int *hz);
 /* 1: bracket every stencil launch with HIP events on the plan's stream (and reset the
  * accumulated times). */
 int qp3d_plan_set_instrument(qp3d_plan *p, int on);
-/* Mean device time (ms) per time step of the three stencil launches over the instrumented
- * steps: ms[0] gradient-max, ms[1] interior update, ms[2] PML update; ms[3] = steps counted. */
+/* Mean device time (ms) per time step over the instrumented steps: ms[0] gradient-max launch,
+ * ms[1] interior update, ms[2] PML update, ms[3] the whole update stage (end of the gradient
+ * maximum to the end of both updates; they overlap when streams == 2); ms[4] = steps counted. */
 int qp3d_plan_kernel_times(qp3d_plan *p, double *ms);
 /* Algorithmic HBM bytes of one time step (interior cells 28 B, PML cells 68 B in FP32):
  * bytes[0] gradient-max launch (4 B per updated cell), bytes[1] interior update launch (24 B per

@@ -156,11 +156,13 @@ def test_blowup_is_reported():
 
 @pytest.mark.parametrize("tile", list(range(1, 7)))
 def test_every_tile_shape(tile):
-    """Each launchable tile shape of both stencil kernels against the oracle, on a grid whose
-    extents are not multiples of any tile and with z-chunks shorter than the grid."""
+    """Each launchable tile shape of the three stencil kernels against the oracle, on a grid
+    whose extents are not multiples of any tile and with z-chunks shorter than the grid; odd
+    shapes run the two updates on one stream, even shapes concurrently on two."""
     s = qp3d.config_setup(hx=37, hy=71, nz0=29, pml=9, nt=50, seed=7, src_depth=4)
-    say(f"tile {tile}")
-    with qp3d.Plan(s, tile_update=tile, tile_grad=tile, zchunk=11) as p:
+    streams = 1 if tile % 2 else 2
+    say(f"tile {tile} streams {streams}")
+    with qp3d.Plan(s, tile_update=tile, tile_grad=tile, tile_pml=tile, zchunk=11, streams=streams) as p:
         rec = p.forward()
         u = p.field(0)
     o = O.OracleState(s)
