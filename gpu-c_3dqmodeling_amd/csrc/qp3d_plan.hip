@@ -328,7 +328,7 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
   }
   if (c.tile_update < 0 || c.tile_update > NTILES || c.tile_grad < 0 || c.tile_grad > NTILES || c.tile_pml < 0 ||
       c.tile_pml > NTILES || c.streams < 0 || c.streams > 2) {
-    qp3d_set_error("plan_create: tile shapes must be in [0,%d]", NTILES);
+    qp3d_set_error("plan_create: tile shapes must be in [0,%d] and streams in [0,2]", NTILES);
     return QP3D_ERR_ARG;
   }
   if (c.storage != QP3D_STORAGE_FP32) {
