@@ -101,6 +101,14 @@ struct Coef {
   float dt;
 };
 
+// Time-step index of a launch: *kdev + koff when kdev is set (launches replayed from a
+// captured graph read the step from device memory), koff otherwise.
+struct Step {
+  const int *kdev;
+  int koff;
+  __device__ __forceinline__ int get() const { return (kdev ? *kdev : 0) + koff; }
+};
+
 // Parameters of one k_update launch.
 template <class S>
 struct UpdArgs {
@@ -113,12 +121,13 @@ struct UpdArgs {
   const typename S::T *w1c, *w3c, *pc;      // PML slots k
   typename S::T *w21;                       // PML accumulator (updated in place)
   const float4 *pmltab;                     // [3][pml]: K1, K2, K3, E per face family and layer
-  const unsigned *gmax;                     // G of this step (float bits)
-  float wk;                                 // wavelet[k]
+  const unsigned *gmax;                     // [nt] G per step (float bits)
+  const float *wav;                         // [nt] wavelet
+  Step step;
   int src_on;
   int sb_z0, sb_z1, sb_x0, sb_x1, sb_y0, sb_y1;  // source box (inclusive bounds)
   const float *sbox;                        // dense source values over the box
-  float *rec;                               // record row of this step, [hx][hy]
+  float *rec;                               // record [nt][hx][hy]
   int sz;                                   // receiver plane
   const int4 *work;                         // one item per workgroup
 };
@@ -128,9 +137,10 @@ struct GradArgs {
   Geo g;
   Coef c;
   const typename S::T *ucur;
-  unsigned *gmax;   // out: atomic max of the gradient components (float bits)
+  unsigned *gmax;   // [nt] out: atomic max of the gradient components (float bits)
   int *flag;        // out: set to 1 when a non-finite gradient is seen
   const int4 *work;
+  Step step;
 };
 
 // A lane's fixed position in its tile and its in-plane offsets.
@@ -375,7 +385,7 @@ __global__ __launch_bounds__(T::NT) void k_gradmax(GradArgs<S> a)
       v = fmaxf(v, red[w]);
       bb |= redbad[w];
     }
-    atomicMax(a.gmax, __float_as_uint(v));
+    atomicMax(a.gmax + a.step.get(), __float_as_uint(v));
     if (bb) atomicOr(a.flag, 1);
   }
 }
@@ -431,7 +441,10 @@ __global__ __launch_bounds__(T::NT) void k_update(UpdArgs<S> a)
 
   // S threshold: |grad u| > 0.01*G compared in double (GPU_kernel.cu:99); for a float
   // magnitude this equals mode > rd(0.01*G).
-  const float G = __uint_as_float(*a.gmax);
+  const int kstep = a.step.get();
+  const float G = __uint_as_float(a.gmax[kstep]);
+  const float wk = a.wav[kstep];
+  float *const rec = a.rec + (size_t)kstep * (g.nx - 2 * g.pml) * (g.ny - 2 * g.pml);
   const float thr = __double2float_rd(1.0e-02 * (double)G);
   // source box and receiver plane tests, hoisted to the workgroup and the plane (uniform)
   const bool src_tile = a.src_on && a.sb_x0 < x1 && a.sb_x1 >= x0 &&
@@ -562,12 +575,12 @@ __global__ __launch_bounds__(T::NT) void k_update(UpdArgs<S> a)
         if (src_plane && x >= a.sb_x0 && x <= a.sb_x1 && y >= a.sb_y0 && y <= a.sb_y1) {
           const int bx = a.sb_x1 - a.sb_x0 + 1, by = a.sb_y1 - a.sb_y0 + 1;
           const float sv = a.sbox[((long)(z - a.sb_z0) * bx + (x - a.sb_x0)) * by + (y - a.sb_y0)];
-          un = fmaf(sv, a.wk, un);  // u3 += source*wavelet, contracted as the CUDA build does
+          un = fmaf(sv, wk, un);  // u3 += source*wavelet, contracted as the CUDA build does
         }
         un = S::rnd(un);
         S::st(a.unext + zo, o, un);
         if (rec_plane && x >= pml && x < nx - pml && y >= pml && y < ny - pml)
-          a.rec[(x - pml) * (ny - 2 * pml) + (y - pml)] = un;
+          rec[(x - pml) * (ny - 2 * pml) + (y - pml)] = un;
       }
       if (more) {
 #pragma unroll
@@ -605,8 +618,9 @@ __global__ __launch_bounds__(T::NT) void k_update(UpdArgs<S> a)
 // GPU_kernel.cu:749-750), i.e. u_next = u_cur + s*w.
 template <class S>
 __global__ void k_rim_source(const typename S::T *ucur, typename S::T *unext, Geo g, int z0, int z1, int x0, int x1,
-                             int y0, int y1, const float *sbox, float wk)
+                             int y0, int y1, const float *sbox, const float *wav, Step step)
 {
+  const float wk = wav[step.get()];
   const int bx = x1 - x0 + 1, by = y1 - y0 + 1, bz = z1 - z0 + 1;
   const long n = (long)bx * by * bz;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
@@ -631,6 +645,12 @@ __global__ void k_snapshot(const typename S::T *u, float *out, Geo g)
   if (y >= hy) return;
   out[((long)z * hx + x) * hy + y] =
       S::ld(u, 4u * ((unsigned)(z + g.pml) * g.pln + (unsigned)(x + g.pml) * g.ny + (unsigned)(y + g.pml)));
+}
+
+// Sets the device step index read by graph-replayed launches.
+__global__ void k_set_step(int *kdev, int k)
+{
+  if (threadIdx.x == 0) *kdev = k;
 }
 
 // Tile shapes the plan can launch (index = qp3d_config.tile_update / tile_grad - 1).

@@ -25,6 +25,8 @@ using namespace qp3d;
 
 namespace {
 
+constexpr int GRAPH_LEN = 10;  // time steps per captured graph
+
 struct WorkList {
   int4 *d = nullptr;
   int n = 0;       // launched workgroups (multiple of 8, padding items are empty)
@@ -58,7 +60,13 @@ struct qp3d_plan {
   float *sbox = nullptr;
   int sb[6]{};               // z0 z1 x0 x1 y0 y1
   bool src_on = false, src_rim = false;
-  std::vector<float> wav;
+  float *wav = nullptr;      // [nt] wavelet on the device
+  int *kdev = nullptr;       // step index read by graph-replayed launches
+
+  // use_graph: GRAPH_LEN consecutive steps (starting at an even step) captured once and
+  // replayed with the step index set on the device before each replay
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
 
   std::vector<float *> snaps;
   std::vector<int> snap_steps;
@@ -359,7 +367,7 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
   p->tile_u = c.tile_update > 0 ? c.tile_update - 1 : 5;  // 16x2 by default
   p->tile_g = c.tile_grad > 0 ? c.tile_grad - 1 : 0;      // 8x2 by default
   p->tile_p = c.tile_pml > 0 ? c.tile_pml - 1 : 1;        // 16x1 by default
-  p->two_streams = c.streams == 2;
+  p->two_streams = c.streams == 2 && !c.use_graph;  // a captured loop runs on one stream
 
   auto fail = [&](int rc) {
     qp3d_plan_destroy(p);
@@ -404,6 +412,8 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
   if ((rc = dmalloc((void **)&p->gmax, sizeof(unsigned) * (size_t)(c.nt > 0 ? c.nt : 1), "step maxima")))
     return fail(rc);
   if ((rc = dmalloc((void **)&p->flag, sizeof(int), "flag"))) return fail(rc);
+  if ((rc = dmalloc((void **)&p->kdev, sizeof(int), "step index"))) return fail(rc);
+  if ((rc = dmalloc((void **)&p->wav, sizeof(float) * (size_t)(c.nt > 0 ? c.nt : 1), "wavelet"))) return fail(rc);
   const size_t recb = sizeof(float) * (size_t)c.nt * p->hx * p->hy;
   if (recb && (rc = dmalloc((void **)&p->rec, recb, "record"))) return fail(rc);
   if (c.snapflag == 1) {
@@ -431,6 +441,10 @@ extern "C" int qp3d_plan_destroy(qp3d_plan *p)
   (void)hipFree(p->pmltab);
   (void)hipFree(p->gmax);
   (void)hipFree(p->flag);
+  (void)hipFree(p->kdev);
+  (void)hipFree(p->wav);
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
   (void)hipFree(p->rec);
   (void)hipFree(p->sbox);
   (void)hipFree(p->wl_grad.d);
@@ -524,7 +538,12 @@ extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps,
     p->sb[0] = z0; p->sb[1] = z1; p->sb[2] = x0; p->sb[3] = x1; p->sb[4] = y0; p->sb[5] = y1;
     p->src_rim = z0 < H || x0 < H || y0 < H || z1 >= cf.nz - H || x1 >= cf.nx - H || y1 >= cf.ny - H;
   }
-  p->wav.assign(wavelet, wavelet + cf.nt);
+  if (cf.nt > 0) HIP_TRY(hipMemcpy(p->wav, wavelet, sizeof(float) * cf.nt, hipMemcpyHostToDevice));
+  // a captured loop holds the source box pointer and bounds: capture again on the next run
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->gexec = nullptr;
+  p->graph = nullptr;
   p->uploaded = true;
   return QP3D_OK;
 }
@@ -568,12 +587,15 @@ static int harvest_events(qp3d_plan *p)
   return QP3D_OK;
 }
 
-static int launch_step(qp3d_plan *p, int k)
+// Enqueues time step k. `st` is the step index the kernels use (k itself, or a device index
+// plus an offset when the launches are being captured; k then only sets the slot parity).
+// While capturing, the snapshot and the instrumentation events are left out.
+static int launch_step(qp3d_plan *p, int k, Step st, bool capture)
 {
   const qp3d_config &cf = p->cfg;
   const int prev = k & 1, cur = prev ^ 1;  // u[prev] holds u(k-1) and receives u(k+1)
   hipEvent_t *e = nullptr;
-  if (p->instrument) {
+  if (p->instrument && !capture) {
     if (p->ev.size() < 4 * (size_t)cf.nt) {
       size_t old = p->ev.size();
       p->ev.resize(4 * (size_t)cf.nt);
@@ -588,9 +610,10 @@ static int launch_step(qp3d_plan *p, int k)
   ga.g = p->geo;
   ga.c = p->coef;
   ga.ucur = p->u[cur];
-  ga.gmax = p->gmax + k;
+  ga.gmax = p->gmax;
   ga.flag = p->flag;
   ga.work = p->wl_grad.d;
+  ga.step = st;
   launch_gradmax(p->tile_g, p->wl_grad, p->stream, ga);
   HIP_TRY(hipGetLastError());
   if (e) HIP_TRY(hipEventRecord(e[1], p->stream));
@@ -610,13 +633,14 @@ static int launch_step(qp3d_plan *p, int k)
   ua.pp = p->pp[prev]; ua.pc = p->pp[cur];
   ua.w21 = p->w21;
   ua.pmltab = p->pmltab;
-  ua.gmax = p->gmax + k;
-  ua.wk = p->wav[k];
+  ua.gmax = p->gmax;
+  ua.wav = p->wav;
+  ua.step = st;
   ua.src_on = p->src_on ? 1 : 0;
   ua.sb_z0 = p->sb[0]; ua.sb_z1 = p->sb[1]; ua.sb_x0 = p->sb[2]; ua.sb_x1 = p->sb[3];
   ua.sb_y0 = p->sb[4]; ua.sb_y1 = p->sb[5];
   ua.sbox = p->sbox;
-  ua.rec = p->rec + (size_t)k * p->hx * p->hy;
+  ua.rec = p->rec;
   ua.sz = cf.sz;
   if (p->wl_int.n) {
     ua.work = p->wl_int.d;
@@ -642,10 +666,10 @@ static int launch_step(qp3d_plan *p, int k)
     const int blocks = (int)std::min<long>((nb + 255) / 256, 4096);
     hipLaunchKernelGGL(k_rim_source<StoreF32>, dim3(blocks), dim3(256), 0, p->stream, (const float *)p->u[cur],
                        p->u[prev], p->geo, p->sb[0], p->sb[1], p->sb[2], p->sb[3], p->sb[4], p->sb[5],
-                       (const float *)p->sbox, p->wav[k]);
+                       (const float *)p->sbox, (const float *)p->wav, st);
     HIP_TRY(hipGetLastError());
   }
-  if (cf.snapflag == 1 && k % cf.ntsnap == 0) {
+  if (!capture && cf.snapflag == 1 && k % cf.ntsnap == 0) {
     const int si = k / cf.ntsnap;
     if (si < (int)p->snaps.size()) {
       hipLaunchKernelGGL(k_snapshot<StoreF32>, dim3((p->hy + 255) / 256, p->hx, p->hz), dim3(256), 0, p->stream,
@@ -655,6 +679,40 @@ static int launch_step(qp3d_plan *p, int k)
     }
   }
   return QP3D_OK;
+}
+
+static bool snapshot_in(const qp3d_plan *p, int k0, int k1)
+{
+  if (p->cfg.snapflag != 1) return false;
+  const int first = (k0 + p->cfg.ntsnap - 1) / p->cfg.ntsnap * p->cfg.ntsnap;  // first multiple >= k0
+  return first < k1 && first / p->cfg.ntsnap < (int)p->snaps.size();
+}
+
+// Captures GRAPH_LEN steps (even start) into p->gexec; kernels read the step from p->kdev.
+static int capture_steps(qp3d_plan *p)
+{
+  HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+  int rc = QP3D_OK;
+  for (int i = 0; i < GRAPH_LEN && rc == QP3D_OK; i++) rc = launch_step(p, i, Step{p->kdev, i}, true);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(p->stream, &g);
+  if (rc == QP3D_OK && e != hipSuccess) {
+    qp3d_set_error("plan_run: graph capture failed: %s", hipGetErrorString(e));
+    rc = QP3D_ERR_HIP;
+  }
+  if (rc == QP3D_OK) {
+    e = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      qp3d_set_error("plan_run: graph instantiation failed: %s", hipGetErrorString(e));
+      p->gexec = nullptr;
+      rc = QP3D_ERR_HIP;
+    }
+  }
+  if (rc == QP3D_OK)
+    p->graph = g;
+  else if (g)
+    (void)hipGraphDestroy(g);
+  return rc;
 }
 
 extern "C" int qp3d_plan_run(qp3d_plan *p, int k0, int k1)
@@ -672,10 +730,23 @@ extern "C" int qp3d_plan_run(qp3d_plan *p, int k0, int k1)
     return QP3D_ERR_ARG;
   }
   HIP_TRY(hipSetDevice(p->cfg.device));
-  for (int k = k0; k < k1; k++) {
-    int rc = launch_step(p, k);
-    if (rc) return rc;
-    p->steps_done = k + 1;
+  const bool graph = p->cfg.use_graph && !p->instrument;
+  for (int k = k0; k < k1;) {
+    if (graph && (k & 1) == 0 && k + GRAPH_LEN <= k1 && !snapshot_in(p, k, k + GRAPH_LEN)) {
+      if (!p->gexec) {
+        int rc = capture_steps(p);
+        if (rc) return rc;
+      }
+      hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, p->stream, p->kdev, k);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipGraphLaunch(p->gexec, p->stream));
+      k += GRAPH_LEN;
+    } else {
+      int rc = launch_step(p, k, Step{nullptr, k}, false);
+      if (rc) return rc;
+      k++;
+    }
+    p->steps_done = k;
   }
   // everything enqueued so far is ordered before later work on the plan's stream
   if (p->two_streams) HIP_TRY(hipStreamWaitEvent(p->stream, p->ev_pml, 0));

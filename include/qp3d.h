@@ -87,7 +87,10 @@ typedef struct qp3d_config {
   float dx, dy, dz, dt;
   int sz;        /* receiver plane, padded z index */
   int storage;   /* QP3D_STORAGE_* */
-  int use_graph; /* 1: replay a captured hipGraph per step group; 0: direct launches */
+  int use_graph; /* 1: steps run as replays of one captured hipGraph of 10 steps (a group that
+                    starts on an odd step or holds a snapshot step is launched directly, and so
+                    is every step while instrumentation is on); implies streams = 1.
+                    0: direct launches */
   int snapflag;  /* 1: extract the interior cube when k % ntsnap == 0 */
   int ntsnap;
   int zchunk;    /* z planes per workgroup in the stencil kernels, 0 = automatic */
