@@ -93,6 +93,8 @@ def main():
     ap.add_argument("--tile-grad", type=int, default=0)
     ap.add_argument("--tile-pml", type=int, default=0)
     ap.add_argument("--streams", type=int, default=0, help="2: interior and PML updates concurrent; 1: serial")
+    ap.add_argument("--storage", default="fp32", choices=["fp32", "bf16", "fp16"],
+                    help="storage of wavefields and PML state (arithmetic is always FP32)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run a 1 GiB device-to-device copy (the reference kernel for scaling HBM counters)")
     args = ap.parse_args()
@@ -110,7 +112,7 @@ def main():
     log(f"setup {s.nx}x{s.ny}x{s.nz} in {time.perf_counter() - t:.1f} s")
     ntsnap = 500
     plan = qp3d.Plan(s, snapflag=1, ntsnap=ntsnap, use_graph=bool(args.graph), tile_update=args.tile_update,
-                     tile_grad=args.tile_grad, tile_pml=args.tile_pml, streams=args.streams)
+                     tile_grad=args.tile_grad, tile_pml=args.tile_pml, streams=args.streams, storage=args.storage)
     log("model uploaded")
     for i in range(args.warmup):
         t = time.perf_counter()
@@ -148,7 +150,7 @@ def main():
         try:
             with open(TRAFFIC_FILE) as f:
                 tr = json.load(f)
-            if tr.get("nx") == s.nx and tr.get("storage", "fp32") == "fp32":
+            if tr.get("nx") == s.nx and tr.get("storage", "fp32") == args.storage:
                 traffic = tr.get("bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
@@ -159,7 +161,8 @@ def main():
     plan.close()
 
     out = {
-        "metric": "Gcell-updates/s (512^3 FP32 forward, 3000 steps)",
+        "metric": "Gcell-updates/s (512^3 FP32 forward, 3000 steps)" if args.storage == "fp32" else
+                  f"Gcell-updates/s (512^3 forward, {args.storage.upper()} storage / FP32 arithmetic, 3000 steps)",
         "value": round(value, 3),
         "unit": "Gcell-updates/s",
         "n_gpus": 1,
@@ -180,7 +183,8 @@ def main():
                    "ms_gradmax": round(kt["gradmax"], 5), "ms_update_interior": round(kt["update_interior"], 5),
                    "ms_update_pml": round(kt["update_pml"], 5), "ms_update_stage": round(kt["update_stage"], 5),
                    "gbs_gradmax": round(gbs.get("gradmax", 0), 1), "gbs_update_pml": round(gbs.get("update_pml", 0), 1),
-                   "roofline_kernel": "k_update<interior> (24 B per interior cell)",
+                   "storage": args.storage,
+                   "roofline_kernel": f"k_update<interior> ({kb['update_interior'] / kb['cells_interior']:.0f} B per interior cell)",
                    "hbm_frac_step": round(step_bytes * s.nt * args.steps / wall / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},

@@ -75,6 +75,31 @@ struct StoreF32 {
   static __device__ __forceinline__ float rnd(float v) { return v; }
 };
 
+// bfloat16 storage (round to nearest even), float arithmetic. Offsets stay 4-byte-cell
+// offsets; the element offset is half of it.
+struct StoreBF16 {
+  using T = unsigned short;
+  static __device__ __forceinline__ unsigned short enc(float v) { return __builtin_bit_cast(unsigned short, (__bf16)v); }
+  static __device__ __forceinline__ float dec(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+  static __device__ __forceinline__ float ld(const T *p, unsigned b) { return dec(*bat(p, b >> 1)); }
+  static __device__ __forceinline__ float ld_stream(const T *p, unsigned b) { return dec(__builtin_nontemporal_load(bat(p, b >> 1))); }
+  static __device__ __forceinline__ void st(T *p, unsigned b, float v) { __builtin_nontemporal_store(enc(v), bat(p, b >> 1)); }
+  static __device__ __forceinline__ float rnd(float v) { return dec(enc(v)); }
+};
+
+// IEEE half storage, float arithmetic. The fields are stored times a power-of-two scale that
+// the plan folds into the source (the equations are linear in the wavefield and PML state) and
+// takes out of the record, snapshots and maxima it returns.
+struct StoreF16 {
+  using T = unsigned short;
+  static __device__ __forceinline__ unsigned short enc(float v) { return __builtin_bit_cast(unsigned short, (_Float16)v); }
+  static __device__ __forceinline__ float dec(unsigned short h) { return (float)__builtin_bit_cast(_Float16, h); }
+  static __device__ __forceinline__ float ld(const T *p, unsigned b) { return dec(*bat(p, b >> 1)); }
+  static __device__ __forceinline__ float ld_stream(const T *p, unsigned b) { return dec(__builtin_nontemporal_load(bat(p, b >> 1))); }
+  static __device__ __forceinline__ void st(T *p, unsigned b, float v) { __builtin_nontemporal_store(enc(v), bat(p, b >> 1)); }
+  static __device__ __forceinline__ float rnd(float v) { return dec(enc(v)); }
+};
+
 __device__ __forceinline__ float ld_model(const float *p, unsigned b) { return __builtin_nontemporal_load(bat(p, b)); }
 
 struct Geo {

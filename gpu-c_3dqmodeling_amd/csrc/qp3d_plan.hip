@@ -49,8 +49,11 @@ struct qp3d_plan {
   long n_int_cells = 0, n_pml_cells = 0;
   hipStream_t stream = nullptr;
 
-  float *state = nullptr;  // u[2], w1[2], w3[2], p[2], w21: 9 fields of N floats
-  float *u[2]{}, *w1[2]{}, *w3[2]{}, *pp[2]{}, *w21 = nullptr;
+  // u[2], w1[2], w3[2], p[2], w21: 9 fields of N cells of esz bytes (float, bfloat16 or half)
+  char *state = nullptr;
+  size_t esz = 4;
+  void *u[2]{}, *w1[2]{}, *w3[2]{}, *pp[2]{}, *w21 = nullptr;
+  int scale_log2 = 0;  // half storage: fields are stored times 2^scale_log2
   float *model = nullptr;  // vdt2, eps, epsd
   float *vdt2 = nullptr, *eps = nullptr, *epsd = nullptr;
   float4 *pmltab = nullptr;
@@ -135,6 +138,18 @@ static int occupancy_of(int tile, F pick)
   return std::max(occ, 1);
 }
 
+// Calls f with a value of the storage policy of a QP3D_STORAGE_* mode.
+template <class F>
+static void with_storage(int storage, F f)
+{
+  if (storage == QP3D_STORAGE_BF16)
+    f(StoreBF16{});
+  else if (storage == QP3D_STORAGE_FP16)
+    f(StoreF16{});
+  else
+    f(StoreF32{});
+}
+
 // ---------------------------------------------------------------------------------------
 // work decomposition
 
@@ -212,14 +227,14 @@ static int build_work(qp3d_plan *p)
   const qp3d_config &c = p->cfg;
   int ncu = 256;
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device);
-  const int occ_g = occupancy_of(p->tile_g, [](auto t) {
-    return reinterpret_cast<const void *>(&k_gradmax<StoreF32, decltype(t)>);
-  });
-  const int occ_i = occupancy_of(p->tile_u, [](auto t) {
-    return reinterpret_cast<const void *>(&k_update<StoreF32, decltype(t), false>);
-  });
-  const int occ_p = occupancy_of(p->tile_p, [](auto t) {
-    return reinterpret_cast<const void *>(&k_update<StoreF32, decltype(t), true>);
+  int occ_g = 1, occ_i = 1, occ_p = 1;
+  with_storage(c.storage, [&](auto sp) {
+    using S = decltype(sp);
+    occ_g = occupancy_of(p->tile_g, [](auto t) { return reinterpret_cast<const void *>(&k_gradmax<S, decltype(t)>); });
+    occ_i = occupancy_of(p->tile_u,
+                         [](auto t) { return reinterpret_cast<const void *>(&k_update<S, decltype(t), false>); });
+    occ_p = occupancy_of(p->tile_p,
+                         [](auto t) { return reinterpret_cast<const void *>(&k_update<S, decltype(t), true>); });
   });
   // gradient-max: uniform tiles over [5,n-5)^3
   {
@@ -339,8 +354,8 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
     qp3d_set_error("plan_create: tile shapes must be in [0,%d] and streams in [0,2]", NTILES);
     return QP3D_ERR_ARG;
   }
-  if (c.storage != QP3D_STORAGE_FP32) {
-    qp3d_set_error("plan_create: storage mode %d is not available in this build", c.storage);
+  if (c.storage != QP3D_STORAGE_FP32 && c.storage != QP3D_STORAGE_BF16 && c.storage != QP3D_STORAGE_FP16) {
+    qp3d_set_error("plan_create: unknown storage mode %d", c.storage);
     return QP3D_ERR_ARG;
   }
   const double cells = (double)c.nx * c.ny * c.nz;
@@ -399,13 +414,15 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
     }
   }
   int rc;
-  if ((rc = dmalloc((void **)&p->state, 9 * fb, "wavefields and PML state"))) return fail(rc);
-  float *s = p->state;
-  p->u[0] = s; p->u[1] = s + p->N;
-  p->w1[0] = s + 2 * p->N; p->w1[1] = s + 3 * p->N;
-  p->w3[0] = s + 4 * p->N; p->w3[1] = s + 5 * p->N;
-  p->pp[0] = s + 6 * p->N; p->pp[1] = s + 7 * p->N;
-  p->w21 = s + 8 * p->N;
+  p->esz = c.storage == QP3D_STORAGE_FP32 ? 4 : 2;
+  const size_t fs = p->esz * (size_t)p->N;
+  if ((rc = dmalloc((void **)&p->state, 9 * fs, "wavefields and PML state"))) return fail(rc);
+  char *s = p->state;
+  p->u[0] = s; p->u[1] = s + fs;
+  p->w1[0] = s + 2 * fs; p->w1[1] = s + 3 * fs;
+  p->w3[0] = s + 4 * fs; p->w3[1] = s + 5 * fs;
+  p->pp[0] = s + 6 * fs; p->pp[1] = s + 7 * fs;
+  p->w21 = s + 8 * fs;
   if ((rc = dmalloc((void **)&p->model, 3 * fb, "model"))) return fail(rc);
   p->vdt2 = p->model; p->eps = p->model + p->N; p->epsd = p->model + 2 * p->N;
   if ((rc = dmalloc((void **)&p->pmltab, sizeof(float4) * 3 * c.pml, "PML table"))) return fail(rc);
@@ -526,6 +543,7 @@ extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps,
   p->sbox = nullptr;
   p->src_on = z1 >= 0;
   p->src_rim = false;
+  p->scale_log2 = 0;
   if (p->src_on) {
     const int bz = z1 - z0 + 1, bx = x1 - x0 + 1, by = y1 - y0 + 1;
     std::vector<float> b((size_t)bz * bx * by);
@@ -533,6 +551,17 @@ extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps,
       for (int x = 0; x < bx; x++)
         memcpy(&b[((size_t)z * bx + x) * by], source + (long)(z + z0) * cf.nx * cf.ny + (long)(x + x0) * cf.ny + y0,
                sizeof(float) * by);
+    if (cf.storage == QP3D_STORAGE_FP16) {
+      // Half storage: scale the fields (through the source) so that the peak injected value
+      // sits at 2^7. The wavefield peaks near 8x the peak of source*wavelet and the PML state
+      // below that, which leaves 2^5 of headroom under the half range and keeps far-field
+      // amplitudes out of the subnormals.
+      double ms = 0, mw = 0;
+      for (float v : b) ms = std::max(ms, (double)std::fabs(v));
+      for (int k = 0; k < cf.nt; k++) mw = std::max(mw, (double)std::fabs(wavelet[k]));
+      if (ms * mw > 0) p->scale_log2 = std::max(-100, std::min(100, (int)std::floor(std::log2(128.0 / (ms * mw)))));
+      for (float &v : b) v = std::ldexp(v, p->scale_log2);
+    }
     HIP_TRY(hipMalloc((void **)&p->sbox, sizeof(float) * b.size()));
     HIP_TRY(hipMemcpy(p->sbox, b.data(), sizeof(float) * b.size(), hipMemcpyHostToDevice));
     p->sb[0] = z0; p->sb[1] = z1; p->sb[2] = x0; p->sb[3] = x1; p->sb[4] = y0; p->sb[5] = y1;
@@ -555,7 +584,7 @@ extern "C" int qp3d_plan_reset(qp3d_plan *p)
     return QP3D_ERR_ARG;
   }
   HIP_TRY(hipSetDevice(p->cfg.device));
-  HIP_TRY(hipMemsetAsync(p->state, 0, sizeof(float) * 9 * (size_t)p->N, p->stream));
+  HIP_TRY(hipMemsetAsync(p->state, 0, p->esz * 9 * (size_t)p->N, p->stream));
   HIP_TRY(hipMemsetAsync(p->gmax, 0, sizeof(unsigned) * (size_t)(p->cfg.nt > 0 ? p->cfg.nt : 1), p->stream));
   HIP_TRY(hipMemsetAsync(p->flag, 0, sizeof(int), p->stream));
   if (p->rec) HIP_TRY(hipMemsetAsync(p->rec, 0, sizeof(float) * (size_t)p->cfg.nt * p->hx * p->hy, p->stream));
@@ -590,8 +619,10 @@ static int harvest_events(qp3d_plan *p)
 // Enqueues time step k. `st` is the step index the kernels use (k itself, or a device index
 // plus an offset when the launches are being captured; k then only sets the slot parity).
 // While capturing, the snapshot and the instrumentation events are left out.
-static int launch_step(qp3d_plan *p, int k, Step st, bool capture)
+template <class S>
+static int launch_step_s(qp3d_plan *p, int k, Step st, bool capture)
 {
+  using T = typename S::T;
   const qp3d_config &cf = p->cfg;
   const int prev = k & 1, cur = prev ^ 1;  // u[prev] holds u(k-1) and receives u(k+1)
   hipEvent_t *e = nullptr;
@@ -606,10 +637,10 @@ static int launch_step(qp3d_plan *p, int k, Step st, bool capture)
   // u(k) is complete only when the previous step's PML update has finished too
   if (p->two_streams) HIP_TRY(hipStreamWaitEvent(p->stream, p->ev_pml, 0));
   if (e) HIP_TRY(hipEventRecord(e[0], p->stream));
-  GradArgs<StoreF32> ga;
+  GradArgs<S> ga;
   ga.g = p->geo;
   ga.c = p->coef;
-  ga.ucur = p->u[cur];
+  ga.ucur = (const T *)p->u[cur];
   ga.gmax = p->gmax;
   ga.flag = p->flag;
   ga.work = p->wl_grad.d;
@@ -622,16 +653,16 @@ static int launch_step(qp3d_plan *p, int k, Step st, bool capture)
     HIP_TRY(hipStreamWaitEvent(p->stream2, p->ev_grad, 0));
   }
 
-  UpdArgs<StoreF32> ua;
+  UpdArgs<S> ua;
   ua.g = p->geo;
   ua.c = p->coef;
-  ua.ucur = p->u[cur];
-  ua.unext = p->u[prev];
+  ua.ucur = (const T *)p->u[cur];
+  ua.unext = (T *)p->u[prev];
   ua.vdt2 = p->vdt2; ua.eps = p->eps; ua.epsd = p->epsd;
-  ua.w1p = p->w1[prev]; ua.w1c = p->w1[cur];
-  ua.w3p = p->w3[prev]; ua.w3c = p->w3[cur];
-  ua.pp = p->pp[prev]; ua.pc = p->pp[cur];
-  ua.w21 = p->w21;
+  ua.w1p = (T *)p->w1[prev]; ua.w1c = (const T *)p->w1[cur];
+  ua.w3p = (T *)p->w3[prev]; ua.w3c = (const T *)p->w3[cur];
+  ua.pp = (T *)p->pp[prev]; ua.pc = (const T *)p->pp[cur];
+  ua.w21 = (T *)p->w21;
   ua.pmltab = p->pmltab;
   ua.gmax = p->gmax;
   ua.wav = p->wav;
@@ -644,14 +675,14 @@ static int launch_step(qp3d_plan *p, int k, Step st, bool capture)
   ua.sz = cf.sz;
   if (p->wl_int.n) {
     ua.work = p->wl_int.d;
-    launch_update<StoreF32, false>(p->tile_u, p->wl_int, p->stream, ua);
+    launch_update<S, false>(p->tile_u, p->wl_int, p->stream, ua);
     HIP_TRY(hipGetLastError());
   }
   if (e) HIP_TRY(hipEventRecord(e[2], p->stream));
   hipStream_t ps = p->two_streams ? p->stream2 : p->stream;
   if (p->wl_pml.n) {
     ua.work = p->wl_pml.d;
-    launch_update<StoreF32, true>(p->tile_p, p->wl_pml, ps, ua);
+    launch_update<S, true>(p->tile_p, p->wl_pml, ps, ua);
     HIP_TRY(hipGetLastError());
   }
   if (e) {
@@ -664,21 +695,28 @@ static int launch_step(qp3d_plan *p, int k, Step st, bool capture)
   if (p->src_rim) {
     const long nb = (long)(p->sb[1] - p->sb[0] + 1) * (p->sb[3] - p->sb[2] + 1) * (p->sb[5] - p->sb[4] + 1);
     const int blocks = (int)std::min<long>((nb + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_rim_source<StoreF32>, dim3(blocks), dim3(256), 0, p->stream, (const float *)p->u[cur],
-                       p->u[prev], p->geo, p->sb[0], p->sb[1], p->sb[2], p->sb[3], p->sb[4], p->sb[5],
+    hipLaunchKernelGGL(k_rim_source<S>, dim3(blocks), dim3(256), 0, p->stream, (const T *)p->u[cur],
+                       (T *)p->u[prev], p->geo, p->sb[0], p->sb[1], p->sb[2], p->sb[3], p->sb[4], p->sb[5],
                        (const float *)p->sbox, (const float *)p->wav, st);
     HIP_TRY(hipGetLastError());
   }
   if (!capture && cf.snapflag == 1 && k % cf.ntsnap == 0) {
     const int si = k / cf.ntsnap;
     if (si < (int)p->snaps.size()) {
-      hipLaunchKernelGGL(k_snapshot<StoreF32>, dim3((p->hy + 255) / 256, p->hx, p->hz), dim3(256), 0, p->stream,
-                         (const float *)p->u[prev], p->snaps[si], p->geo);
+      hipLaunchKernelGGL(k_snapshot<S>, dim3((p->hy + 255) / 256, p->hx, p->hz), dim3(256), 0, p->stream,
+                         (const T *)p->u[prev], p->snaps[si], p->geo);
       HIP_TRY(hipGetLastError());
       p->nsnap_taken = std::max(p->nsnap_taken, si + 1);
     }
   }
   return QP3D_OK;
+}
+
+static int launch_step(qp3d_plan *p, int k, Step st, bool capture)
+{
+  int rc = QP3D_OK;
+  with_storage(p->cfg.storage, [&](auto sp) { rc = launch_step_s<decltype(sp)>(p, k, st, capture); });
+  return rc;
 }
 
 static bool snapshot_in(const qp3d_plan *p, int k0, int k1)
@@ -775,6 +813,14 @@ extern "C" int qp3d_plan_sync(qp3d_plan *p)
 // ---------------------------------------------------------------------------------------
 // results
 
+// Takes the half-storage scale out of values read back from the device.
+static void unscale(const qp3d_plan *p, float *a, size_t n)
+{
+  if (!p->scale_log2) return;
+  const float f = std::ldexp(1.0f, -p->scale_log2);
+  for (size_t i = 0; i < n; i++) a[i] *= f;
+}
+
 extern "C" int qp3d_plan_record(const qp3d_plan *p, float *record)
 {
   if (!p || !record) {
@@ -783,8 +829,9 @@ extern "C" int qp3d_plan_record(const qp3d_plan *p, float *record)
   }
   HIP_TRY(hipSetDevice(p->cfg.device));
   HIP_TRY(hipStreamSynchronize(p->stream));
-  if (p->rec)
-    HIP_TRY(hipMemcpy(record, p->rec, sizeof(float) * (size_t)p->cfg.nt * p->hx * p->hy, hipMemcpyDeviceToHost));
+  const size_t n = (size_t)p->cfg.nt * p->hx * p->hy;
+  if (p->rec) HIP_TRY(hipMemcpy(record, p->rec, sizeof(float) * n, hipMemcpyDeviceToHost));
+  unscale(p, record, n);
   return QP3D_OK;
 }
 
@@ -798,7 +845,7 @@ extern "C" int qp3d_plan_field(const qp3d_plan *p, int which, float *out)
   HIP_TRY(hipSetDevice(p->cfg.device));
   HIP_TRY(hipStreamSynchronize(p->stream));
   const int newest = (p->steps_done - 1) & 1, older = newest ^ 1;
-  const float *src = nullptr;
+  const void *src = nullptr;
   switch (which) {
   case 0: src = p->u[newest]; break;
   case 1: src = p->u[older]; break;
@@ -810,7 +857,24 @@ extern "C" int qp3d_plan_field(const qp3d_plan *p, int which, float *out)
   case 7: src = p->pp[older]; break;
   case 8: src = p->w21; break;
   }
-  HIP_TRY(hipMemcpy(out, src, sizeof(float) * (size_t)p->N, hipMemcpyDeviceToHost));
+  if (p->esz == 4) {
+    HIP_TRY(hipMemcpy(out, src, sizeof(float) * (size_t)p->N, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<unsigned short> h((size_t)p->N);
+    HIP_TRY(hipMemcpy(h.data(), src, sizeof(unsigned short) * (size_t)p->N, hipMemcpyDeviceToHost));
+    if (p->cfg.storage == QP3D_STORAGE_BF16)
+      for (long i = 0; i < p->N; i++) {
+        const unsigned bits = (unsigned)h[i] << 16;
+        memcpy(&out[i], &bits, 4);
+      }
+    else
+      for (long i = 0; i < p->N; i++) {
+        _Float16 v;
+        memcpy(&v, &h[i], 2);
+        out[i] = (float)v;
+      }
+  }
+  unscale(p, out, (size_t)p->N);
   return QP3D_OK;
 }
 
@@ -825,7 +889,9 @@ extern "C" int qp3d_plan_snapshot(const qp3d_plan *p, int i, int *step, float *o
   HIP_TRY(hipSetDevice(p->cfg.device));
   HIP_TRY(hipStreamSynchronize(p->stream));
   if (step) *step = p->snap_steps[i];
-  HIP_TRY(hipMemcpy(out, p->snaps[i], sizeof(float) * (size_t)p->hx * p->hy * p->hz, hipMemcpyDeviceToHost));
+  const size_t n = (size_t)p->hx * p->hy * p->hz;
+  HIP_TRY(hipMemcpy(out, p->snaps[i], sizeof(float) * n, hipMemcpyDeviceToHost));
+  unscale(p, out, n);
   return QP3D_OK;
 }
 
@@ -847,6 +913,7 @@ extern "C" int qp3d_plan_step_maxima(const qp3d_plan *p, float *out)
   HIP_TRY(hipSetDevice(p->cfg.device));
   HIP_TRY(hipStreamSynchronize(p->stream));
   HIP_TRY(hipMemcpy(out, p->gmax, sizeof(float) * (size_t)p->cfg.nt, hipMemcpyDeviceToHost));
+  unscale(p, out, (size_t)p->cfg.nt);
   return QP3D_OK;
 }
 
@@ -884,10 +951,12 @@ extern "C" int qp3d_plan_step_bytes(const qp3d_plan *p, double *bytes)
   // gradient-max: one read of u per updated cell. Interior update: u_cur, u_prev, 3 model
   // fields, u_next (24 B). PML update: u_cur, 3 model fields, u_next, 7 state reads and 4 state
   // writes (64 B); the PML launch also carries the interior cells of mixed tiles (24 B).
+  // Stored fields take esz bytes (4 float, 2 bfloat16/half); the model stays float.
   const double pml_launch_int = (double)p->n_pml_cells - npml;
-  bytes[0] = 4.0 * nupd;
-  bytes[1] = 24.0 * (double)p->n_int_cells;
-  bytes[2] = 64.0 * npml + 24.0 * pml_launch_int;
+  const double e = (double)p->esz, bi = 12.0 + 3.0 * e, bp = 12.0 + 13.0 * e;
+  bytes[0] = e * nupd;
+  bytes[1] = bi * (double)p->n_int_cells;
+  bytes[2] = bp * npml + bi * pml_launch_int;
   bytes[3] = (double)p->n_int_cells;
   bytes[4] = (double)p->n_pml_cells;
   return QP3D_OK;
