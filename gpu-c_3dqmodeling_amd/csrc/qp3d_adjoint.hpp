@@ -1,0 +1,417 @@
+// Adjoint-state kernels: the exact transpose of the linearised forward time step of
+// qp3d_kernels.hpp, for the gradient of J = 1/2 sum_k |record_k - observed_k|^2 with respect
+// to the model ((vp dt)^2, epsilon, epsilon - delta).
+//
+// Forward step k (per updated cell, g = grad u_k, S = S(eps, eps-del, g/|g|) above the
+// 0.01*G threshold and 1 below it):
+//   interior  u_{k+1} = 2 u_k - u_{k-1} + v2 S ((1+2eps)(Uxx+Uyy) + Uzz)
+//   PML       w1' = Cn Unn - K1 w1 + K2 w1o,  p' = -K1 p + K2 po + sgn Cn E gn,
+//             w21' = dt p' + K3 w21,  w3' = B Tg + 2 w3 - w3o,  u_{k+1} = w1' + w21' + w3'
+//             (B = S v2; Cn = (1+2eps) B on x/y faces, B on z faces).
+// The threshold G and the frozen rim are constants of the linearisation (G is a maximum, whose
+// derivative exists almost nowhere; the rim never depends on the model).
+//
+// One adjoint step is two launches:
+//   k_adj_prod   per cell: forward stencils of u_k (as in k_update), the adjoint of the
+//                cell's own equations (PML adjoint states in place, the leapfrog terms of the
+//                adjoint of u), the model-gradient terms, and six product fields whose
+//                transposed stencils carry the adjoint back to u_k: Pxx, Pyy, Pzz (second
+//                derivatives) and Qx, Qy, Qz (first derivatives, including the path through
+//                S's dependence on the gradient direction);
+//   k_adj_apply  per cell: adds the transposed stencils of the six product fields to the
+//                adjoint of u_k.
+// Adjoint state uses the forward's slot parity: at step k slot k&1 holds the adjoint of the
+// newest level (k+1) and receives the partial adjoint of level k-1; slot (k&1)^1 holds the
+// partial adjoint of level k and is completed by this step.
+#pragma once
+#include "qp3d_kernels.hpp"
+
+namespace qp3d {
+
+struct AdjArgs {
+  Geo g;
+  Coef c;
+  const float *u;                         // forward u_k
+  const float *vdt2, *eps, *epsd;         // model
+  const float4 *pmltab;
+  const unsigned *gmax;                   // G_k (float bits)
+  float *an, *ac;                         // adjoint of u: newest level / current level
+  float *o1n, *o1c, *pin, *pic, *o3n, *o3c, *o21;  // PML adjoint states (newest / current)
+  float *pxx, *pyy, *pzz, *qx, *qy, *qz;  // product fields
+  float *gv2, *ge, *ged;                  // gradient accumulators w.r.t. v2, eps, eps-del
+  const float *resid;                     // adjoint source of u_{k+1} on the receiver plane [hx][hy]
+  int sz;
+  const int4 *work;
+};
+
+__device__ __forceinline__ float &at(float *p, size_t zo, unsigned b) { return *bat(p + zo, b); }
+
+// Loads of one plane z of a field that has no z queue: the lane's own rows plus its share of
+// the x and y halos.
+template <class T>
+struct PlaneLoadC {
+  float c[T::RX];
+  float hx[T::NXH];
+  float hy;
+
+  __device__ __forceinline__ void issue(const float *f, const Geo &g, const Lane<T> &L, int z)
+  {
+    const float *fz = f + (size_t)z * g.pln;
+#pragma unroll
+    for (int r = 0; r < T::RX; r++) c[r] = *bat(fz, L.qoff[r]);
+#pragma unroll
+    for (int i = 0; i < T::NXH; i++)
+      if (L.hx_on[i]) hx[i] = *bat(fz, L.hxo[i]);
+    if (L.hy_on) hy = *bat(fz, L.hyo);
+  }
+
+  __device__ __forceinline__ void stage(float (*sm)[T::LY], const Lane<T> &L) const
+  {
+#pragma unroll
+    for (int r = 0; r < T::RX; r++) sm[H + L.tx + T::WX * r][L.Y] = c[r];
+#pragma unroll
+    for (int i = 0; i < T::NXH; i++) {
+      const int h = L.tx + T::WX * i;
+      if (L.hx_on[i]) sm[h < H ? h : T::TX + h][L.Y] = hx[i];
+    }
+    if (L.hy_on) {
+      const int t = L.tx * TY + L.ty;
+      const int row = t / (2 * H), col = t - row * (2 * H);
+      sm[H + row][Lane<T>::halo_col(col, L.fold)] = hy;
+    }
+  }
+};
+
+template <class T>
+__device__ __forceinline__ void shift_queue(float (&q)[T::RX][QN])
+{
+#pragma unroll
+  for (int r = 0; r < T::RX; r++)
+#pragma unroll
+    for (int j = 0; j < 2 * H; j++) q[r][j] = q[r][j + 1];
+}
+
+template <class T, bool PML>
+__global__ __launch_bounds__(T::NT) void k_adj_prod(AdjArgs a)
+{
+  __shared__ float sm[2][T::LX][T::LY];
+  const Geo &g = a.g;
+  int x0, x1, y0, y1, z0, z1;
+  bool fold;
+  unpack(a.work[blockIdx.x], x0, x1, y0, y1, z0, z1, fold);
+  if (z0 >= z1) return;
+  const Lane<T> L(g, x0, x1, y0, y1, fold);
+  const int y = L.y;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, pml = g.pml;
+  const int hy = ny - 2 * pml;
+  bool rows_on[T::RX];
+#pragma unroll
+  for (int r = 0; r < T::RX; r++) rows_on[r] = L.yon && x0 + L.tx + T::WX * r < x1;
+  const float G = __uint_as_float(*a.gmax);
+  const float thr = __double2float_rd(1.0e-02 * (double)G);
+
+  int fam = 0;
+  float4 trow[T::RX];
+  float sgn = 1.f;
+  if constexpr (PML) {
+    fam = (x1 <= pml || x0 >= nx - pml) ? 0 : ((y1 <= pml || y0 >= ny - pml) ? 1 : 2);
+    if (fam == 0) {
+      sgn = x1 <= pml ? 1.f : -1.f;
+#pragma unroll
+      for (int r = 0; r < T::RX; r++) {
+        const int x = min(x0 + L.tx + T::WX * r, x1 - 1);
+        trow[r] = a.pmltab[x < pml ? pml - x : x - nx + pml];
+      }
+    } else if (fam == 1) {
+      const int yc = L.yc;
+      sgn = yc < pml ? 1.f : -1.f;
+      const float4 t = a.pmltab[pml + (yc < pml ? pml - yc : yc - ny + pml)];
+#pragma unroll
+      for (int r = 0; r < T::RX; r++) trow[r] = t;
+    } else {
+      sgn = z1 <= pml ? 1.f : -1.f;
+    }
+  }
+
+  float q[T::RX][QN];
+  prime_queue<StoreF32, T>(q, a.u, g, L, z0);
+  PlaneLoad<StoreF32, T> P;
+  P.issue(a.u, g, L, z0);
+  float centre[T::RX];
+#pragma unroll
+  for (int r = 0; r < T::RX; r++) {
+    q[r][2 * H] = P.qn[r];
+    centre[r] = q[r][H];
+  }
+  P.stage(sm[0], centre, L);
+  __syncthreads();
+  int b = 0;
+  for (int z = z0; z < z1; z++) {
+    const bool more = z + 1 < z1;
+    if (more) P.issue(a.u, g, L, z + 1);
+    float (*s)[T::LY] = sm[b];
+    float4 tz = make_float4(0.f, 0.f, 0.f, 0.f);
+    if constexpr (PML) {
+      if (fam == 2) tz = a.pmltab[2 * pml + (z < pml ? pml - z : z - nz + pml)];
+    }
+    const size_t zo = (size_t)z * g.pln;
+    const bool rec_plane = a.resid && z == a.sz;
+#pragma unroll
+    for (int r = 0; r < T::RX; r++) {
+      if (!rows_on[r]) continue;
+      const int X = H + L.tx + T::WX * r, Y = L.Y;
+      const int x = x0 + L.tx + T::WX * r;
+      const float c0 = q[r][H];
+      float Lxx = 0.f, Lyy = 0.f, Lzz = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
+#pragma unroll
+      for (int k = 1; k <= H; k++) {
+        const float ax = s[X + k][Y], bx = s[X - k][Y];
+        const float ay = s[X][Y + k], by = s[X][Y - k];
+        const float az = q[r][H + k], bz = q[r][H - k];
+        Lxx = fmaf(a.c.c2[k - 1], ax + bx, Lxx);
+        Lyy = fmaf(a.c.c2[k - 1], ay + by, Lyy);
+        Lzz = fmaf(a.c.c2[k - 1], az + bz, Lzz);
+        ux = fmaf(a.c.c1[k - 1], ax - bx, ux);
+        uy = fmaf(a.c.c1[k - 1], ay - by, uy);
+        uz = fmaf(a.c.c1[k - 1], az - bz, uz);
+      }
+      const float Uxx = fmaf(-a.c.s2, c0, Lxx) * a.c.ixx;
+      const float Uyy = fmaf(-a.c.s2, c0, Lyy) * a.c.iyy;
+      const float Uzz = fmaf(-a.c.s2, c0, Lzz) * a.c.izz;
+      ux *= a.c.ix; uy *= a.c.iy; uz *= a.c.iz;
+      const unsigned o = L.off[r];
+      const float e = *bat(a.eps + zo, o), ed = *bat(a.epsd + zo, o), v2 = *bat(a.vdt2 + zo, o);
+
+      // S and its derivatives (the forward's arithmetic for S itself)
+      float Sv = 1.f, dSde = 0.f, dSded = 0.f, dSdqa = 0.f, dSdb = 0.f;
+      float nxv = 0.f, nyv = 0.f, nzv = 0.f, inv = 0.f, mode = 0.f;
+      bool above = false;
+      {
+        mode = __builtin_amdgcn_sqrtf(fmaf(ux, ux, fmaf(uy, uy, uz * uz)));
+        if (mode > thr) {
+          above = true;
+          inv = __builtin_amdgcn_rcpf(mode + 2.22e-17f);
+          nxv = ux * inv; nyv = uy * inv; nzv = uz * inv;
+          const float qa = fmaf(nxv, nxv, nyv * nyv), bq = nzv * nzv;
+          const float den = fmaf(2.f * e, qa, 1.f);
+          const float rd2 = __builtin_amdgcn_rcpf(den * den), rd3 = rd2 / den;
+          const float X8 = 8.f * ed * qa * bq * rd2;
+          const float R = __builtin_amdgcn_sqrtf(1.f - X8);
+          Sv = 0.5f + 0.5f * R;
+          const float dSdX = -0.25f / fmaxf(R, 1e-6f);
+          dSde = dSdX * (-32.f * ed * qa * qa * bq * rd3);
+          dSded = dSdX * (8.f * qa * bq * rd2);
+          dSdqa = dSdX * (8.f * ed * bq * (den - 4.f * e * qa) * rd3);
+          dSdb = dSdX * (8.f * ed * qa * rd2);
+        }
+      }
+      const float e1 = fmaf(2.f, e, 1.f);
+      float mu = at(a.an, zo, o);
+      if (rec_plane && x >= pml && x < nx - pml && y >= pml && y < ny - pml)
+        mu += a.resid[(x - pml) * hy + (y - pml)];
+
+      float Pxx, Pyy, Pzz, Qx = 0.f, Qy = 0.f, Qz = 0.f, dS, gv2, ge;
+      if constexpr (!PML) {
+        const float Pt = fmaf(e1, Uxx + Uyy, Uzz);
+        const float w = mu * v2 * Sv;
+        Pxx = w * e1; Pyy = Pxx; Pzz = w;
+        dS = mu * v2 * Pt;
+        gv2 = mu * Sv * Pt;
+        ge = 2.f * w * (Uxx + Uyy);
+        at(a.ac, zo, o) += 2.f * mu;
+        at(a.an, zo, o) = -mu;
+      } else {
+        const float4 t = fam == 2 ? tz : trow[r];
+        const float B = Sv * v2;
+        const float w1b = at(a.o1n, zo, o) + mu;
+        const float w21b = at(a.o21, zo, o) + mu;
+        const float w3b = at(a.o3n, zo, o) + mu;
+        const float pb = fmaf(a.c.dt, w21b, at(a.pin, zo, o));
+        at(a.o1c, zo, o) += -t.x * w1b;
+        at(a.o1n, zo, o) = t.y * w1b;
+        at(a.pic, zo, o) += -t.x * pb;
+        at(a.pin, zo, o) = t.y * pb;
+        at(a.o21, zo, o) = t.z * w21b;
+        at(a.o3c, zo, o) += 2.f * w3b;
+        at(a.o3n, zo, o) = -w3b;
+        at(a.an, zo, o) = 0.f;  // u_{k-1} is not read by PML cells
+        float dCn, dB, de1;
+        if (fam == 0) {
+          const float Cn = e1 * B;
+          Pxx = Cn * w1b; Pyy = e1 * B * w3b; Pzz = B * w3b; Qx = sgn * Cn * t.w * pb;
+          dCn = fmaf(w1b, Uxx, pb * sgn * t.w * ux);
+          dB = w3b * fmaf(e1, Uyy, Uzz) + e1 * dCn;
+          de1 = w3b * B * Uyy + B * dCn;
+        } else if (fam == 1) {
+          const float Cn = e1 * B;
+          Pyy = Cn * w1b; Pxx = e1 * B * w3b; Pzz = B * w3b; Qy = sgn * Cn * t.w * pb;
+          dCn = fmaf(w1b, Uyy, pb * sgn * t.w * uy);
+          dB = w3b * fmaf(e1, Uxx, Uzz) + e1 * dCn;
+          de1 = w3b * B * Uxx + B * dCn;
+        } else {
+          Pzz = B * w1b; Pxx = e1 * B * w3b; Pyy = Pxx; Qz = sgn * B * t.w * pb;
+          dCn = fmaf(w1b, Uzz, pb * sgn * t.w * uz);
+          dB = w3b * e1 * (Uxx + Uyy) + dCn;
+          de1 = w3b * B * (Uxx + Uyy);
+        }
+        dS = v2 * dB;
+        gv2 = Sv * dB;
+        ge = 2.f * de1;
+      }
+      float ged = 0.f;
+      if (above) {
+        ge = fmaf(dS, dSde, ge);
+        ged = dS * dSded;
+        // adjoint of n = g/(|g|+r), then of g
+        const float vx = dS * dSdqa * 2.f * nxv, vy = dS * dSdqa * 2.f * nyv, vz = dS * dSdb * 2.f * nzv;
+        const float cdot = (vx * nxv + vy * nyv + vz * nzv) * inv / mode;
+        Qx += vx * inv - cdot * ux;
+        Qy += vy * inv - cdot * uy;
+        Qz += vz * inv - cdot * uz;
+      }
+      at(a.pxx, zo, o) = Pxx; at(a.pyy, zo, o) = Pyy; at(a.pzz, zo, o) = Pzz;
+      at(a.qx, zo, o) = Qx; at(a.qy, zo, o) = Qy; at(a.qz, zo, o) = Qz;
+      at(a.gv2, zo, o) += gv2;
+      at(a.ge, zo, o) += ge;
+      at(a.ged, zo, o) += ged;
+    }
+    if (more) {
+      shift_queue<T>(q);
+#pragma unroll
+      for (int r = 0; r < T::RX; r++) {
+        q[r][2 * H] = P.qn[r];
+        centre[r] = q[r][H];
+      }
+      P.stage(sm[b ^ 1], centre, L);
+      __syncthreads();
+      b ^= 1;
+    }
+  }
+}
+
+struct ApplyArgs {
+  Geo g;
+  Coef c;
+  const float *pxx, *pyy, *pzz, *qx, *qy, *qz;
+  float *ac;
+  const int4 *work;
+};
+
+// ac += Uxx^T Pxx + Uyy^T Pyy + Uzz^T Pzz + Dx^T Qx + Dy^T Qy + Dz^T Qz over [5,n-5)^3. The
+// second-derivative stencils are symmetric; the first-derivative ones change sign. Product
+// fields are zero outside [5,n-5)^3.
+template <class T>
+__global__ __launch_bounds__(T::NT) void k_adj_apply(ApplyArgs a)
+{
+  __shared__ float sm[2][4][T::LX][T::LY];  // Pxx, Qx, Pyy, Qy
+  const Geo &g = a.g;
+  int x0, x1, y0, y1, z0, z1;
+  bool fold;
+  unpack(a.work[blockIdx.x], x0, x1, y0, y1, z0, z1, fold);
+  if (z0 >= z1) return;
+  const Lane<T> L(g, x0, x1, y0, y1, fold);
+  bool rows_on[T::RX];
+#pragma unroll
+  for (int r = 0; r < T::RX; r++) rows_on[r] = L.yon && x0 + L.tx + T::WX * r < x1;
+  const float *pl[4] = {a.pxx, a.qx, a.pyy, a.qy};
+  float qzz[T::RX][QN], qqz[T::RX][QN];
+  prime_queue<StoreF32, T>(qzz, a.pzz, g, L, z0);
+  prime_queue<StoreF32, T>(qqz, a.qz, g, L, z0);
+  PlaneLoadC<T> P[4];
+  float fzz[T::RX], fqz[T::RX];
+  auto issue = [&](int z) {
+#pragma unroll
+    for (int f = 0; f < 4; f++) P[f].issue(pl[f], g, L, z);
+    const float *uzz = a.pzz + (size_t)(z + H) * g.pln, *uqz = a.qz + (size_t)(z + H) * g.pln;
+#pragma unroll
+    for (int r = 0; r < T::RX; r++) {
+      fzz[r] = *bat(uzz, L.qoff[r]);
+      fqz[r] = *bat(uqz, L.qoff[r]);
+    }
+  };
+  issue(z0);
+#pragma unroll
+  for (int r = 0; r < T::RX; r++) {
+    qzz[r][2 * H] = fzz[r];
+    qqz[r][2 * H] = fqz[r];
+  }
+#pragma unroll
+  for (int f = 0; f < 4; f++) P[f].stage(sm[0][f], L);
+  __syncthreads();
+  int b = 0;
+  for (int z = z0; z < z1; z++) {
+    const bool more = z + 1 < z1;
+    if (more) issue(z + 1);
+    const size_t zo = (size_t)z * g.pln;
+#pragma unroll
+    for (int r = 0; r < T::RX; r++) {
+      if (!rows_on[r]) continue;
+      const int X = H + L.tx + T::WX * r, Y = L.Y;
+      float (*sxx)[T::LY] = sm[b][0], (*sqx)[T::LY] = sm[b][1], (*syy)[T::LY] = sm[b][2], (*sqy)[T::LY] = sm[b][3];
+      float lxx = 0.f, lyy = 0.f, lzz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
+#pragma unroll
+      for (int k = 1; k <= H; k++) {
+        lxx = fmaf(a.c.c2[k - 1], sxx[X + k][Y] + sxx[X - k][Y], lxx);
+        lyy = fmaf(a.c.c2[k - 1], syy[X][Y + k] + syy[X][Y - k], lyy);
+        lzz = fmaf(a.c.c2[k - 1], qzz[r][H + k] + qzz[r][H - k], lzz);
+        dx = fmaf(a.c.c1[k - 1], sqx[X - k][Y] - sqx[X + k][Y], dx);
+        dy = fmaf(a.c.c1[k - 1], sqy[X][Y - k] - sqy[X][Y + k], dy);
+        dz = fmaf(a.c.c1[k - 1], qqz[r][H - k] - qqz[r][H + k], dz);
+      }
+      float acc = fmaf(-a.c.s2, sxx[X][Y], lxx) * a.c.ixx;
+      acc = fmaf(fmaf(-a.c.s2, syy[X][Y], lyy), a.c.iyy, acc);
+      acc = fmaf(fmaf(-a.c.s2, qzz[r][H], lzz), a.c.izz, acc);
+      acc = fmaf(dx, a.c.ix, acc);
+      acc = fmaf(dy, a.c.iy, acc);
+      acc = fmaf(dz, a.c.iz, acc);
+      at(a.ac, zo, L.off[r]) += acc;
+    }
+    if (more) {
+      shift_queue<T>(qzz);
+      shift_queue<T>(qqz);
+#pragma unroll
+      for (int r = 0; r < T::RX; r++) {
+        qzz[r][2 * H] = fzz[r];
+        qqz[r][2 * H] = fqz[r];
+      }
+#pragma unroll
+      for (int f = 0; f < 4; f++) P[f].stage(sm[b ^ 1][f], L);
+      __syncthreads();
+      b ^= 1;
+    }
+  }
+}
+
+// gsum += gacc over n cells (FP64 accumulation over time of the per-segment FP32 sums).
+__global__ void k_fold(double *gsum, const float *gacc, long n)
+{
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    gsum[i] += (double)gacc[i];
+}
+
+// Gradients w.r.t. (vp, eps, del) from those w.r.t. (v2 = (vp dt)^2, eps, eps-del), in place:
+// g[0..n) v2 -> vp, g[n..2n) eps -> eps (+ eps-del part), g[2n..3n) eps-del -> del.
+__global__ void k_grad_finish(double *g, const float *vdt2, float dt, long n)
+{
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double vdt = sqrt((double)vdt2[i]);
+    g[i] *= 2.0 * vdt * dt;
+    const double ged = g[2 * n + i];
+    g[n + i] += ged;
+    g[2 * n + i] = -ged;
+  }
+}
+
+// FP32 <-> BF16 copies of a state block (checkpoints stored in BF16).
+__global__ void k_pack_bf16(unsigned short *dst, const float *src, long n)
+{
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    dst[i] = StoreBF16::enc(src[i]);
+}
+__global__ void k_unpack_bf16(float *dst, const unsigned short *src, long n)
+{
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    dst[i] = StoreBF16::dec(src[i]);
+}
+
+}  // namespace qp3d

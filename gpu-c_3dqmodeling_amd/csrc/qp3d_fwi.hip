@@ -1,0 +1,311 @@
+// Adjoint-state FWI gradient on top of a forward plan: a checkpointed forward run, then
+// segments in reverse order, each recomputed from its checkpoint (keeping u_k of every step
+// of the segment) and back-propagated with the adjoint kernels of qp3d_adjoint.hpp. Gradient
+// terms are summed in FP32 within a segment and accumulated over time in FP64.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "qp3d_adjoint.hpp"
+#include "qp3d_plan_impl.hpp"
+
+namespace {
+
+constexpr int TILE_PROD = 1;   // Tile<16, 1> for k_adj_prod (interior and PML lists)
+constexpr int TILE_APPLY = 3;  // Tile<8, 1> for k_adj_apply
+using TP = Tile<16, 1>;
+using TA = Tile<8, 1>;
+
+int occupancy(const void *fn, int threads)
+{
+  int occ = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, threads, 0) != hipSuccess) occ = 1;
+  return occ > 0 ? occ : 1;
+}
+
+}  // namespace
+
+struct qp3d_fwi {
+  qp3d_plan *p = nullptr;
+  int interval = 0, nseg = 0;
+  int ckpt_storage = QP3D_STORAGE_FP32;
+  std::vector<void *> ckpt;      // state block at the start of each segment
+  float *ubuf = nullptr;         // [interval][N] u_k of the segment being back-propagated
+  float *adj = nullptr;          // adjoint state: u[2], w1[2], p[2], w3[2], w21 (9 N)
+  float *prod = nullptr;         // Pxx Pyy Pzz Qx Qy Qz (6 N)
+  float *gacc = nullptr;         // per-segment FP32 sums w.r.t. v2, eps, eps-del (3 N)
+  double *gsum = nullptr;        // FP64 sums (3 N)
+  float *resid = nullptr;        // [nt][hx][hy]
+  WorkList wl_i, wl_p, wl_a;
+  bool forward_done = false;
+  hipEvent_t ev[4] = {};
+  double ms_forward = 0, ms_recompute = 0, ms_adjoint = 0;
+};
+
+static void fwi_free(qp3d_fwi *f)
+{
+  if (!f) return;
+  for (void *c : f->ckpt) (void)hipFree(c);
+  (void)hipFree(f->ubuf);
+  (void)hipFree(f->adj);
+  (void)hipFree(f->prod);
+  (void)hipFree(f->gacc);
+  (void)hipFree(f->gsum);
+  (void)hipFree(f->resid);
+  (void)hipFree(f->wl_i.d);
+  (void)hipFree(f->wl_p.d);
+  (void)hipFree(f->wl_a.d);
+  for (hipEvent_t e : f->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete f;
+}
+
+extern "C" int qp3d_fwi_create(qp3d_plan *p, int interval, int ckpt_storage, qp3d_fwi **out)
+{
+  if (!p || !out) {
+    qp3d_set_error("fwi_create: null argument");
+    return QP3D_ERR_ARG;
+  }
+  *out = nullptr;
+  const qp3d_config &c = p->cfg;
+  if (c.storage != QP3D_STORAGE_FP32 || p->two_streams) {
+    qp3d_set_error("fwi_create: the gradient needs a plan with FP32 storage and one stream");
+    return QP3D_ERR_ARG;
+  }
+  if (c.nt < 1 || interval < 1 || (ckpt_storage != QP3D_STORAGE_FP32 && ckpt_storage != QP3D_STORAGE_BF16)) {
+    qp3d_set_error("fwi_create: need nt >= 1, interval >= 1, checkpoint storage FP32 or BF16");
+    return QP3D_ERR_ARG;
+  }
+  HIP_TRY(hipSetDevice(c.device));
+  qp3d_fwi *f = new qp3d_fwi();
+  f->p = p;
+  f->interval = std::min(interval, c.nt);
+  f->nseg = (c.nt + f->interval - 1) / f->interval;
+  f->ckpt_storage = ckpt_storage;
+  auto fail = [&](int rc) {
+    fwi_free(f);
+    return rc;
+  };
+  auto dmalloc = [&](void **ptr, size_t bytes, const char *what) -> int {
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+      qp3d_set_error("fwi_create: hipMalloc of %zu bytes for %s failed: %s", bytes, what, hipGetErrorString(e));
+      *ptr = nullptr;
+      return QP3D_ERR_ALLOC;
+    }
+    return 0;
+  };
+  const size_t N = (size_t)p->N;
+  const size_t cb = 9 * N * (ckpt_storage == QP3D_STORAGE_FP32 ? 4 : 2);
+  int rc;
+  for (int s = 1; s < f->nseg; s++) {  // segment 0 starts from rest
+    void *d = nullptr;
+    if ((rc = dmalloc(&d, cb, "checkpoint"))) return fail(rc);
+    f->ckpt.push_back(d);
+  }
+  if ((rc = dmalloc((void **)&f->ubuf, sizeof(float) * N * f->interval, "segment wavefields"))) return fail(rc);
+  if ((rc = dmalloc((void **)&f->adj, sizeof(float) * 9 * N, "adjoint state"))) return fail(rc);
+  if ((rc = dmalloc((void **)&f->prod, sizeof(float) * 6 * N, "product fields"))) return fail(rc);
+  if ((rc = dmalloc((void **)&f->gacc, sizeof(float) * 3 * N, "segment gradient"))) return fail(rc);
+  if ((rc = dmalloc((void **)&f->gsum, sizeof(double) * 3 * N, "gradient"))) return fail(rc);
+  if ((rc = dmalloc((void **)&f->resid, sizeof(float) * (size_t)c.nt * p->hx * p->hy, "residuals"))) return fail(rc);
+  // product fields stay zero outside [5,n-5)^3, where k_adj_prod never writes
+  HIP_TRY(hipMemset(f->prod, 0, sizeof(float) * 6 * N));
+  const int oi = occupancy(reinterpret_cast<const void *>(&k_adj_prod<TP, false>), TP::NT);
+  const int op = occupancy(reinterpret_cast<const void *>(&k_adj_prod<TP, true>), TP::NT);
+  const int oa = occupancy(reinterpret_cast<const void *>(&k_adj_apply<TA>), TA::NT);
+  if ((rc = qp3d_build_update_lists(p, TILE_PROD, TILE_PROD, oi, op, f->wl_i, f->wl_p))) return fail(rc);
+  if ((rc = qp3d_build_uniform_list(p, TILE_APPLY, oa, f->wl_a))) return fail(rc);
+  for (hipEvent_t &e : f->ev)
+    if (hipEventCreate(&e) != hipSuccess) return fail(QP3D_ERR_HIP);
+  *out = f;
+  return QP3D_OK;
+}
+
+extern "C" int qp3d_fwi_destroy(qp3d_fwi *f)
+{
+  if (!f) return QP3D_OK;
+  (void)hipSetDevice(f->p->cfg.device);
+  (void)hipStreamSynchronize(f->p->stream);
+  fwi_free(f);
+  return QP3D_OK;
+}
+
+static int grid_of(long n) { return (int)std::min<long>((n + 255) / 256, 8192); }
+
+static int save_ckpt(qp3d_fwi *f, int s)
+{
+  qp3d_plan *p = f->p;
+  const long n = 9 * p->N;
+  if (f->ckpt_storage == QP3D_STORAGE_FP32) {
+    HIP_TRY(hipMemcpyAsync(f->ckpt[s - 1], p->state, sizeof(float) * n, hipMemcpyDeviceToDevice, p->stream));
+  } else {
+    hipLaunchKernelGGL(k_pack_bf16, dim3(grid_of(n)), dim3(256), 0, p->stream, (unsigned short *)f->ckpt[s - 1],
+                       (const float *)p->state, n);
+    HIP_TRY(hipGetLastError());
+  }
+  return QP3D_OK;
+}
+
+static int load_ckpt(qp3d_fwi *f, int s)
+{
+  qp3d_plan *p = f->p;
+  const long n = 9 * p->N;
+  if (s == 0) {
+    HIP_TRY(hipMemsetAsync(p->state, 0, sizeof(float) * n, p->stream));
+  } else if (f->ckpt_storage == QP3D_STORAGE_FP32) {
+    HIP_TRY(hipMemcpyAsync(p->state, f->ckpt[s - 1], sizeof(float) * n, hipMemcpyDeviceToDevice, p->stream));
+  } else {
+    hipLaunchKernelGGL(k_unpack_bf16, dim3(grid_of(n)), dim3(256), 0, p->stream, (float *)p->state,
+                       (const unsigned short *)f->ckpt[s - 1], n);
+    HIP_TRY(hipGetLastError());
+  }
+  return QP3D_OK;
+}
+
+// Forward run from rest with a checkpoint at the start of every segment; the plan then holds
+// the record, the per-step maxima and the final state as after qp3d_plan_run.
+extern "C" int qp3d_fwi_forward(qp3d_fwi *f)
+{
+  if (!f) return QP3D_ERR_ARG;
+  qp3d_plan *p = f->p;
+  if (!p->uploaded) {
+    qp3d_set_error("fwi_forward: upload the model first");
+    return QP3D_ERR_STATE;
+  }
+  HIP_TRY(hipSetDevice(p->cfg.device));
+  int rc = qp3d_plan_reset(p);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(f->ev[0], p->stream));
+  for (int k = 0; k < p->cfg.nt; k++) {
+    if (k % f->interval == 0 && k > 0 && (rc = save_ckpt(f, k / f->interval))) return rc;
+    if ((rc = qp3d_forward_step(p, k))) return rc;
+  }
+  p->steps_done = p->cfg.nt;
+  HIP_TRY(hipEventRecord(f->ev[1], p->stream));
+  rc = qp3d_plan_sync(p);
+  if (rc) return rc;
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, f->ev[0], f->ev[1]));
+  f->ms_forward = ms;
+  f->forward_done = true;
+  return QP3D_OK;
+}
+
+static int adjoint_step(qp3d_fwi *f, int k, const float *uk)
+{
+  qp3d_plan *p = f->p;
+  const size_t N = (size_t)p->N;
+  const int nw = k & 1, cw = nw ^ 1;  // slot of the newest level / of level k
+  float *A = f->adj;
+  AdjArgs a;
+  a.g = p->geo;
+  a.c = p->coef;
+  a.u = uk;
+  a.vdt2 = p->vdt2; a.eps = p->eps; a.epsd = p->epsd;
+  a.pmltab = p->pmltab;
+  a.gmax = p->gmax + k;
+  a.an = A + nw * N; a.ac = A + cw * N;
+  a.o1n = A + (2 + nw) * N; a.o1c = A + (2 + cw) * N;
+  a.pin = A + (4 + nw) * N; a.pic = A + (4 + cw) * N;
+  a.o3n = A + (6 + nw) * N; a.o3c = A + (6 + cw) * N;
+  a.o21 = A + 8 * N;
+  a.pxx = f->prod; a.pyy = f->prod + N; a.pzz = f->prod + 2 * N;
+  a.qx = f->prod + 3 * N; a.qy = f->prod + 4 * N; a.qz = f->prod + 5 * N;
+  a.gv2 = f->gacc; a.ge = f->gacc + N; a.ged = f->gacc + 2 * N;
+  a.resid = f->resid + (size_t)k * p->hx * p->hy;
+  a.sz = p->cfg.sz;
+  if (f->wl_i.n) {
+    a.work = f->wl_i.d;
+    hipLaunchKernelGGL((k_adj_prod<TP, false>), dim3(f->wl_i.n), dim3(TY, TP::WX), 0, p->stream, a);
+    HIP_TRY(hipGetLastError());
+  }
+  if (f->wl_p.n) {
+    a.work = f->wl_p.d;
+    hipLaunchKernelGGL((k_adj_prod<TP, true>), dim3(f->wl_p.n), dim3(TY, TP::WX), 0, p->stream, a);
+    HIP_TRY(hipGetLastError());
+  }
+  ApplyArgs b;
+  b.g = p->geo;
+  b.c = p->coef;
+  b.pxx = a.pxx; b.pyy = a.pyy; b.pzz = a.pzz; b.qx = a.qx; b.qy = a.qy; b.qz = a.qz;
+  b.ac = a.ac;
+  b.work = f->wl_a.d;
+  hipLaunchKernelGGL((k_adj_apply<TA>), dim3(f->wl_a.n), dim3(TY, TA::WX), 0, p->stream, b);
+  HIP_TRY(hipGetLastError());
+  return QP3D_OK;
+}
+
+// Gradient of J = 1/2 sum |record - observed|^2 given resid = record - observed ([nt][hx][hy],
+// host), with respect to the padded-grid vp, epsilon and delta (host, [nz][nx][ny] each).
+extern "C" int qp3d_fwi_gradient(qp3d_fwi *f, const float *resid, double *g_vp, double *g_eps, double *g_del)
+{
+  if (!f || !resid || !g_vp || !g_eps || !g_del) {
+    qp3d_set_error("fwi_gradient: null argument");
+    return QP3D_ERR_ARG;
+  }
+  if (!f->forward_done) {
+    qp3d_set_error("fwi_gradient: run qp3d_fwi_forward first");
+    return QP3D_ERR_STATE;
+  }
+  qp3d_plan *p = f->p;
+  const qp3d_config &c = p->cfg;
+  const size_t N = (size_t)p->N;
+  HIP_TRY(hipSetDevice(c.device));
+  HIP_TRY(hipMemcpyAsync(f->resid, resid, sizeof(float) * (size_t)c.nt * p->hx * p->hy, hipMemcpyHostToDevice,
+                         p->stream));
+  HIP_TRY(hipMemsetAsync(f->adj, 0, sizeof(float) * 9 * N, p->stream));
+  HIP_TRY(hipMemsetAsync(f->gsum, 0, sizeof(double) * 3 * N, p->stream));
+  double ms_re = 0, ms_adj = 0;
+  int rc;
+  for (int s = f->nseg - 1; s >= 0; s--) {
+    const int k0 = s * f->interval, k1 = std::min(k0 + f->interval, c.nt);
+    HIP_TRY(hipEventRecord(f->ev[0], p->stream));
+    if ((rc = load_ckpt(f, s))) return rc;
+    for (int k = k0; k < k1; k++) {
+      const int cur = (k & 1) ^ 1;  // slot of u_k in the forward state
+      HIP_TRY(hipMemcpyAsync(f->ubuf + (size_t)(k - k0) * N, p->u[cur], sizeof(float) * N, hipMemcpyDeviceToDevice,
+                             p->stream));
+      if ((rc = qp3d_forward_step(p, k))) return rc;
+    }
+    HIP_TRY(hipEventRecord(f->ev[1], p->stream));
+    HIP_TRY(hipMemsetAsync(f->gacc, 0, sizeof(float) * 3 * N, p->stream));
+    for (int k = k1 - 1; k >= k0; k--)
+      if ((rc = adjoint_step(f, k, f->ubuf + (size_t)(k - k0) * N))) return rc;
+    hipLaunchKernelGGL(k_fold, dim3(grid_of(3 * (long)N)), dim3(256), 0, p->stream, f->gsum, (const float *)f->gacc,
+                       3 * (long)N);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(f->ev[2], p->stream));
+    HIP_TRY(hipEventSynchronize(f->ev[2]));
+    float a = 0, b = 0;
+    HIP_TRY(hipEventElapsedTime(&a, f->ev[0], f->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&b, f->ev[1], f->ev[2]));
+    ms_re += a;
+    ms_adj += b;
+  }
+  hipLaunchKernelGGL(k_grad_finish, dim3(grid_of((long)N)), dim3(256), 0, p->stream, f->gsum,
+                     (const float *)p->vdt2, c.dt, (long)N);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(g_vp, f->gsum, sizeof(double) * N, hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipMemcpyAsync(g_eps, f->gsum + N, sizeof(double) * N, hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipMemcpyAsync(g_del, f->gsum + 2 * N, sizeof(double) * N, hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  f->ms_recompute = ms_re;
+  f->ms_adjoint = ms_adj;
+  // the plan's state now holds the end of segment 0's recomputation
+  p->steps_done = std::min(f->interval, c.nt);
+  return QP3D_OK;
+}
+
+// Device time (ms) of the last checkpointed forward run, of the recomputation and of the
+// adjoint sweep of the last gradient.
+extern "C" int qp3d_fwi_times(const qp3d_fwi *f, double *ms)
+{
+  if (!f || !ms) return QP3D_ERR_ARG;
+  ms[0] = f->ms_forward;
+  ms[1] = f->ms_recompute;
+  ms[2] = f->ms_adjoint;
+  return QP3D_OK;
+}

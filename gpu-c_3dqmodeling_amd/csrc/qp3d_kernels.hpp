@@ -673,7 +673,7 @@ __global__ void k_snapshot(const typename S::T *u, float *out, Geo g)
 }
 
 // Sets the device step index read by graph-replayed launches.
-__global__ void k_set_step(int *kdev, int k)
+static __global__ void k_set_step(int *kdev, int k)
 {
   if (threadIdx.x == 0) *kdev = k;
 }

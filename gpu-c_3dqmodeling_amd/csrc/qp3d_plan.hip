@@ -9,94 +9,10 @@
 #include <cstring>
 #include <vector>
 
-#include "qp3d.h"
-#include "qp3d_kernels.hpp"
-
-using namespace qp3d;
-
-#define HIP_TRY(expr)                                                                                  \
-  do {                                                                                                 \
-    hipError_t e_ = (expr);                                                                            \
-    if (e_ != hipSuccess) {                                                                            \
-      qp3d_set_error("%s failed at %s:%d: %s", #expr, __FILE__, __LINE__, hipGetErrorString(e_));     \
-      return QP3D_ERR_HIP;                                                                             \
-    }                                                                                                  \
-  } while (0)
-
-namespace {
-
-constexpr int GRAPH_LEN = 10;  // time steps per captured graph
-
-struct WorkList {
-  int4 *d = nullptr;
-  int n = 0;       // launched workgroups (multiple of 8, padding items are empty)
-  long cells = 0;  // cells updated by one launch
-};
-
-}  // namespace
-
-struct qp3d_plan {
-  qp3d_config cfg{};
-  int hx = 0, hy = 0, hz = 0;
-  long N = 0;
-  Geo geo{};
-  Coef coef{};
-  int tile_u = 0, tile_g = 0, tile_p = 0;
-  bool two_streams = false;
-  hipStream_t stream2 = nullptr;                          // PML updates when two_streams
-  hipEvent_t ev_grad = nullptr, ev_pml = nullptr;         // cross-stream ordering, no timing
-  WorkList wl_grad, wl_int, wl_pml;
-  long n_int_cells = 0, n_pml_cells = 0;
-  hipStream_t stream = nullptr;
-
-  // u[2], w1[2], w3[2], p[2], w21: 9 fields of N cells of esz bytes (float, bfloat16 or half)
-  char *state = nullptr;
-  size_t esz = 4;
-  void *u[2]{}, *w1[2]{}, *w3[2]{}, *pp[2]{}, *w21 = nullptr;
-  int scale_log2 = 0;  // half storage: fields are stored times 2^scale_log2
-  float *model = nullptr;  // vdt2, eps, epsd
-  float *vdt2 = nullptr, *eps = nullptr, *epsd = nullptr;
-  float4 *pmltab = nullptr;
-  unsigned *gmax = nullptr;  // [nt]
-  int *flag = nullptr;
-  float *rec = nullptr;      // [nt][hx][hy]
-  float *sbox = nullptr;
-  int sb[6]{};               // z0 z1 x0 x1 y0 y1
-  bool src_on = false, src_rim = false;
-  float *wav = nullptr;      // [nt] wavelet on the device
-  int *kdev = nullptr;       // step index read by graph-replayed launches
-
-  // use_graph: GRAPH_LEN consecutive steps (starting at an even step) captured once and
-  // replayed with the step index set on the device before each replay
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-
-  std::vector<float *> snaps;
-  std::vector<int> snap_steps;
-  int nsnap_taken = 0;
-
-  bool uploaded = false;
-  int steps_done = 0;  // one past the last step enqueued since the last reset
-
-  bool instrument = false;
-  std::vector<hipEvent_t> ev;   // 4 per step: before gradmax, after gradmax, after interior, after PML
-  std::vector<int> ev_pending;  // steps with recorded events not yet harvested
-  double t_grad = 0, t_int = 0, t_pml = 0, t_upd = 0;
-  long n_launch = 0;
-};
+#include "qp3d_plan_impl.hpp"
 
 // ---------------------------------------------------------------------------------------
 // tile shapes and launches
-
-static int tile_tx(int t)
-{
-  static const int tx[] = {
-#define QP3D_TX(wx, rx) wx * rx,
-      QP3D_TILES(QP3D_TX)
-#undef QP3D_TX
-  };
-  return tx[t];
-}
 
 template <class S>
 static void launch_gradmax(int tile, const WorkList &w, hipStream_t st, const GradArgs<S> &a)
@@ -136,18 +52,6 @@ static int occupancy_of(int tile, F pick)
   QP3D_TILES(QP3D_O)
 #undef QP3D_O
   return std::max(occ, 1);
-}
-
-// Calls f with a value of the storage policy of a QP3D_STORAGE_* mode.
-template <class F>
-static void with_storage(int storage, F f)
-{
-  if (storage == QP3D_STORAGE_BF16)
-    f(StoreBF16{});
-  else if (storage == QP3D_STORAGE_FP16)
-    f(StoreF16{});
-  else
-    f(StoreF32{});
 }
 
 // ---------------------------------------------------------------------------------------
@@ -222,13 +126,104 @@ static int upload_list(WorkList &w, const std::vector<int4> &logical)
   return QP3D_OK;
 }
 
-static int build_work(qp3d_plan *p)
+static int cu_count(const qp3d_plan *p)
+{
+  int ncu = 256;
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->cfg.device);
+  return ncu;
+}
+
+// Uniform tiles over [5,n-5)^3 (the gradient-max pass).
+int qp3d_build_uniform_list(const qp3d_plan *p, int tile, int occ, WorkList &w)
 {
   const qp3d_config &c = p->cfg;
-  int ncu = 256;
-  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device);
+  const int TX = tile_tx(tile);
+  std::vector<Seg> xs, ys;
+  cut(xs, H, c.nx - H, TX);
+  cut(ys, H, c.ny - H, TY);
+  const int len = c.nz - 2 * H;
+  const int zc = choose_zchunk((int)(xs.size() * ys.size()), len, occ * cu_count(p), c.zchunk);
+  std::vector<int4> items;
+  for (int z = H; z < c.nz - H; z += zc)
+    for (const Seg &x : xs)
+      for (const Seg &y : ys) items.push_back(pack(x.a, x.b, y.a, y.b, z, std::min(z + zc, c.nz - H)));
+  return upload_list(w, items);
+}
+
+// Update tiles aligned to the PML boundary in x and y, z split at the PML planes: interior
+// tiles (tile shape tile_i) hold only interior cells, PML tiles (tile_p) one face family each.
+int qp3d_build_update_lists(const qp3d_plan *p, int tile_i, int tile_p, int occ_i, int occ_p, WorkList &wi,
+                            WorkList &wp)
+{
+  const qp3d_config &c = p->cfg;
+  const int ncu = cu_count(p);
+  const int pml = c.pml;
+  auto segs = [](std::vector<Seg> &out, std::vector<bool> &inner, int n, int pml, int w) {
+    const int bounds[4] = {H, pml, n - pml, n - H};
+    for (int s = 0; s < 3; s++) {
+      const size_t before = out.size();
+      cut(out, bounds[s], bounds[s + 1], w);
+      inner.insert(inner.end(), out.size() - before, s == 1);
+    }
+  };
+  // interior kernel: its own x tiling of the interior; PML kernel: x tiling with its tile width
+  std::vector<Seg> xu, xs, ys;
+  std::vector<bool> xui, xi, yi;
+  segs(xu, xui, c.nx, pml, tile_tx(tile_i));
+  segs(xs, xi, c.nx, pml, tile_tx(tile_p));
+  segs(ys, yi, c.ny, pml, TY);
+  int nint_tiles = 0;
+  for (size_t a = 0; a < xu.size(); a++)
+    for (size_t b = 0; b < ys.size(); b++) nint_tiles += xui[a] && yi[b];
+  // The two y faces of an x-interior tile column share one folded tile when each face fits
+  // half a wavefront with its 5-cell halo.
+  const bool fold = pml <= 32;
+  std::vector<std::pair<int4, bool>> ptiles;  // PML xy tiles over the whole z range
+  for (size_t a = 0; a < xs.size(); a++) {
+    if (!xi[a]) {
+      for (size_t b = 0; b < ys.size(); b++)
+        ptiles.push_back({pack(xs[a].a, xs[a].b, ys[b].a, ys[b].b, 0, 0), false});
+    } else if (fold) {
+      ptiles.push_back({pack(xs[a].a, xs[a].b, H, pml, 0, 0), true});
+    } else {
+      for (size_t b = 0; b < ys.size(); b++)
+        if (!yi[b]) ptiles.push_back({pack(xs[a].a, xs[a].b, ys[b].a, ys[b].b, 0, 0), false});
+    }
+  }
+  const int zmid = c.nz - 2 * pml, zpml = pml - H;
+  const int zc_i = choose_zchunk(nint_tiles, zmid, occ_i * ncu, c.zchunk);
+  const int zc_p = choose_zchunk((int)ptiles.size(), c.nz - 2 * H, occ_p * ncu, c.zchunk);
+  std::vector<int4> it, ip;
+  for (int z = pml; z < c.nz - pml; z += zc_i)
+    for (size_t a = 0; a < xu.size(); a++)
+      for (size_t b = 0; b < ys.size(); b++)
+        if (xui[a] && yi[b]) it.push_back(pack(xu[a].a, xu[a].b, ys[b].a, ys[b].b, z, std::min(z + zc_i, c.nz - pml)));
+  // PML tiles over the whole z range, then the interior tiles' top and bottom z faces
+  for (int z = H; z < c.nz - H; z += zc_p)
+    for (const auto &t : ptiles) {
+      int4 w = t.first;
+      w.z = z;
+      w.w = std::min(z + zc_p, c.nz - H);
+      if (t.second) w.y = (int)((unsigned)w.y | 0x80000000u);
+      ip.push_back(w);
+    }
+  for (int face = 0; face < 2; face++) {
+    const int za = face == 0 ? H : c.nz - pml, zb = face == 0 ? pml : c.nz - H;
+    const int zc = std::max(1, std::min(zpml, zc_p));
+    for (int z = za; z < zb; z += zc)
+      for (size_t a = 0; a < xs.size(); a++)
+        for (size_t b = 0; b < ys.size(); b++)
+          if (xi[a] && yi[b]) ip.push_back(pack(xs[a].a, xs[a].b, ys[b].a, ys[b].b, z, std::min(z + zc, zb)));
+  }
+  int rc = upload_list(wi, it);
+  if (rc) return rc;
+  return upload_list(wp, ip);
+}
+
+static int build_work(qp3d_plan *p)
+{
   int occ_g = 1, occ_i = 1, occ_p = 1;
-  with_storage(c.storage, [&](auto sp) {
+  with_storage(p->cfg.storage, [&](auto sp) {
     using S = decltype(sp);
     occ_g = occupancy_of(p->tile_g, [](auto t) { return reinterpret_cast<const void *>(&k_gradmax<S, decltype(t)>); });
     occ_i = occupancy_of(p->tile_u,
@@ -236,88 +231,12 @@ static int build_work(qp3d_plan *p)
     occ_p = occupancy_of(p->tile_p,
                          [](auto t) { return reinterpret_cast<const void *>(&k_update<S, decltype(t), true>); });
   });
-  // gradient-max: uniform tiles over [5,n-5)^3
-  {
-    const int TX = tile_tx(p->tile_g);
-    std::vector<Seg> xs, ys;
-    cut(xs, H, c.nx - H, TX);
-    cut(ys, H, c.ny - H, TY);
-    const int len = c.nz - 2 * H;
-    const int zc = choose_zchunk((int)(xs.size() * ys.size()), len, occ_g * ncu, c.zchunk);
-    std::vector<int4> items;
-    for (int z = H; z < c.nz - H; z += zc)
-      for (const Seg &x : xs)
-        for (const Seg &y : ys) items.push_back(pack(x.a, x.b, y.a, y.b, z, std::min(z + zc, c.nz - H)));
-    int rc = upload_list(p->wl_grad, items);
-    if (rc) return rc;
-  }
-  // update: tiles aligned to the PML boundary in x and y, z split at the PML planes
-  {
-    const int pml = c.pml;
-    auto segs = [](std::vector<Seg> &out, std::vector<bool> &inner, int n, int pml, int w) {
-      const int bounds[4] = {H, pml, n - pml, n - H};
-      for (int s = 0; s < 3; s++) {
-        const size_t before = out.size();
-        cut(out, bounds[s], bounds[s + 1], w);
-        inner.insert(inner.end(), out.size() - before, s == 1);
-      }
-    };
-    // interior kernel: its own x tiling of the interior; PML kernel: x tiling with its tile width
-    std::vector<Seg> xu, xs, ys;
-    std::vector<bool> xui, xi, yi;
-    segs(xu, xui, c.nx, pml, tile_tx(p->tile_u));
-    segs(xs, xi, c.nx, pml, tile_tx(p->tile_p));
-    segs(ys, yi, c.ny, pml, TY);
-    int nint_tiles = 0;
-    for (size_t a = 0; a < xu.size(); a++)
-      for (size_t b = 0; b < ys.size(); b++) nint_tiles += xui[a] && yi[b];
-    // The two y faces of an x-interior tile column share one folded tile when each face fits
-    // half a wavefront with its 5-cell halo.
-    const bool fold = pml <= 32;
-    std::vector<std::pair<int4, bool>> ptiles;  // PML xy tiles over the whole z range
-    for (size_t a = 0; a < xs.size(); a++) {
-      if (!xi[a]) {
-        for (size_t b = 0; b < ys.size(); b++)
-          ptiles.push_back({pack(xs[a].a, xs[a].b, ys[b].a, ys[b].b, 0, 0), false});
-      } else if (fold) {
-        ptiles.push_back({pack(xs[a].a, xs[a].b, H, pml, 0, 0), true});
-      } else {
-        for (size_t b = 0; b < ys.size(); b++)
-          if (!yi[b]) ptiles.push_back({pack(xs[a].a, xs[a].b, ys[b].a, ys[b].b, 0, 0), false});
-      }
-    }
-    const int zmid = c.nz - 2 * pml, zpml = pml - H;
-    const int zc_i = choose_zchunk(nint_tiles, zmid, occ_i * ncu, c.zchunk);
-    const int zc_p = choose_zchunk((int)ptiles.size(), c.nz - 2 * H, occ_p * ncu, c.zchunk);
-    std::vector<int4> it, ip;
-    for (int z = pml; z < c.nz - pml; z += zc_i)
-      for (size_t a = 0; a < xu.size(); a++)
-        for (size_t b = 0; b < ys.size(); b++)
-          if (xui[a] && yi[b]) it.push_back(pack(xu[a].a, xu[a].b, ys[b].a, ys[b].b, z, std::min(z + zc_i, c.nz - pml)));
-    // PML tiles over the whole z range, then the interior tiles' top and bottom z faces
-    for (int z = H; z < c.nz - H; z += zc_p)
-      for (const auto &t : ptiles) {
-        int4 w = t.first;
-        w.z = z;
-        w.w = std::min(z + zc_p, c.nz - H);
-        if (t.second) w.y = (int)((unsigned)w.y | 0x80000000u);
-        ip.push_back(w);
-      }
-    for (int face = 0; face < 2; face++) {
-      const int za = face == 0 ? H : c.nz - pml, zb = face == 0 ? pml : c.nz - H;
-      const int zc = std::max(1, std::min(zpml, zc_p));
-      for (int z = za; z < zb; z += zc)
-        for (size_t a = 0; a < xs.size(); a++)
-          for (size_t b = 0; b < ys.size(); b++)
-            if (xi[a] && yi[b]) ip.push_back(pack(xs[a].a, xs[a].b, ys[b].a, ys[b].b, z, std::min(z + zc, zb)));
-    }
-    int rc = upload_list(p->wl_int, it);
-    if (rc) return rc;
-    rc = upload_list(p->wl_pml, ip);
-    if (rc) return rc;
-    p->n_int_cells = p->wl_int.cells;
-    p->n_pml_cells = p->wl_pml.cells;
-  }
+  int rc = qp3d_build_uniform_list(p, p->tile_g, occ_g, p->wl_grad);
+  if (rc) return rc;
+  rc = qp3d_build_update_lists(p, p->tile_u, p->tile_p, occ_i, occ_p, p->wl_int, p->wl_pml);
+  if (rc) return rc;
+  p->n_int_cells = p->wl_int.cells;
+  p->n_pml_cells = p->wl_pml.cells;
   return QP3D_OK;
 }
 
@@ -618,15 +537,16 @@ static int harvest_events(qp3d_plan *p)
 
 // Enqueues time step k. `st` is the step index the kernels use (k itself, or a device index
 // plus an offset when the launches are being captured; k then only sets the slot parity).
-// While capturing, the snapshot and the instrumentation events are left out.
+// A bare step (graph capture, recomputation) leaves out the snapshot and the instrumentation
+// events.
 template <class S>
-static int launch_step_s(qp3d_plan *p, int k, Step st, bool capture)
+static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare)
 {
   using T = typename S::T;
   const qp3d_config &cf = p->cfg;
   const int prev = k & 1, cur = prev ^ 1;  // u[prev] holds u(k-1) and receives u(k+1)
   hipEvent_t *e = nullptr;
-  if (p->instrument && !capture) {
+  if (p->instrument && !bare) {
     if (p->ev.size() < 4 * (size_t)cf.nt) {
       size_t old = p->ev.size();
       p->ev.resize(4 * (size_t)cf.nt);
@@ -700,7 +620,7 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool capture)
                        (const float *)p->sbox, (const float *)p->wav, st);
     HIP_TRY(hipGetLastError());
   }
-  if (!capture && cf.snapflag == 1 && k % cf.ntsnap == 0) {
+  if (!bare && cf.snapflag == 1 && k % cf.ntsnap == 0) {
     const int si = k / cf.ntsnap;
     if (si < (int)p->snaps.size()) {
       hipLaunchKernelGGL(k_snapshot<S>, dim3((p->hy + 255) / 256, p->hx, p->hz), dim3(256), 0, p->stream,
@@ -712,12 +632,14 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool capture)
   return QP3D_OK;
 }
 
-static int launch_step(qp3d_plan *p, int k, Step st, bool capture)
+static int launch_step(qp3d_plan *p, int k, Step st, bool bare)
 {
   int rc = QP3D_OK;
-  with_storage(p->cfg.storage, [&](auto sp) { rc = launch_step_s<decltype(sp)>(p, k, st, capture); });
+  with_storage(p->cfg.storage, [&](auto sp) { rc = launch_step_s<decltype(sp)>(p, k, st, bare); });
   return rc;
 }
+
+int qp3d_forward_step(qp3d_plan *p, int k) { return launch_step(p, k, Step{nullptr, k}, true); }
 
 static bool snapshot_in(const qp3d_plan *p, int k0, int k1)
 {

@@ -1,0 +1,117 @@
+// Internal header of the qP3D HIP library: the plan object and the helpers shared by the
+// forward time loop (qp3d_plan.hip) and the adjoint gradient (qp3d_fwi.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "qp3d.h"
+#include "qp3d_kernels.hpp"
+
+using namespace qp3d;
+
+#define HIP_TRY(expr)                                                                                  \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess) {                                                                            \
+      qp3d_set_error("%s failed at %s:%d: %s", #expr, __FILE__, __LINE__, hipGetErrorString(e_));     \
+      return QP3D_ERR_HIP;                                                                             \
+    }                                                                                                  \
+  } while (0)
+
+namespace qp3d_impl {
+
+constexpr int GRAPH_LEN = 10;  // time steps per captured graph
+
+struct WorkList {
+  int4 *d = nullptr;
+  int n = 0;       // launched workgroups (multiple of 8, padding items are empty)
+  long cells = 0;  // cells updated by one launch
+};
+
+}  // namespace qp3d_impl
+
+using qp3d_impl::WorkList;
+using qp3d_impl::GRAPH_LEN;
+
+struct qp3d_plan {
+  qp3d_config cfg{};
+  int hx = 0, hy = 0, hz = 0;
+  long N = 0;
+  Geo geo{};
+  Coef coef{};
+  int tile_u = 0, tile_g = 0, tile_p = 0;
+  bool two_streams = false;
+  hipStream_t stream2 = nullptr;                          // PML updates when two_streams
+  hipEvent_t ev_grad = nullptr, ev_pml = nullptr;         // cross-stream ordering, no timing
+  WorkList wl_grad, wl_int, wl_pml;
+  long n_int_cells = 0, n_pml_cells = 0;
+  hipStream_t stream = nullptr;
+
+  // u[2], w1[2], w3[2], p[2], w21: 9 fields of N cells of esz bytes (float, bfloat16 or half)
+  char *state = nullptr;
+  size_t esz = 4;
+  void *u[2]{}, *w1[2]{}, *w3[2]{}, *pp[2]{}, *w21 = nullptr;
+  int scale_log2 = 0;  // half storage: fields are stored times 2^scale_log2
+  float *model = nullptr;  // vdt2, eps, epsd
+  float *vdt2 = nullptr, *eps = nullptr, *epsd = nullptr;
+  float4 *pmltab = nullptr;
+  unsigned *gmax = nullptr;  // [nt]
+  int *flag = nullptr;
+  float *rec = nullptr;      // [nt][hx][hy]
+  float *sbox = nullptr;
+  int sb[6]{};               // z0 z1 x0 x1 y0 y1
+  bool src_on = false, src_rim = false;
+  float *wav = nullptr;      // [nt] wavelet on the device
+  int *kdev = nullptr;       // step index read by graph-replayed launches
+
+  // use_graph: GRAPH_LEN consecutive steps (starting at an even step) captured once and
+  // replayed with the step index set on the device before each replay
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+
+  std::vector<float *> snaps;
+  std::vector<int> snap_steps;
+  int nsnap_taken = 0;
+
+  bool uploaded = false;
+  int steps_done = 0;  // one past the last step enqueued since the last reset
+
+  bool instrument = false;
+  std::vector<hipEvent_t> ev;   // 4 per step: before gradmax, after gradmax, after interior, after PML
+  std::vector<int> ev_pending;  // steps with recorded events not yet harvested
+  double t_grad = 0, t_int = 0, t_pml = 0, t_upd = 0;
+  long n_launch = 0;
+};
+
+// Interior-x extent (rows) of tile shape t.
+inline int tile_tx(int t)
+{
+  static const int tx[] = {
+#define QP3D_TX(wx, rx) wx * rx,
+      QP3D_TILES(QP3D_TX)
+#undef QP3D_TX
+  };
+  return tx[t];
+}
+
+// Calls f with a value of the storage policy of a QP3D_STORAGE_* mode.
+template <class F>
+inline void with_storage(int storage, F f)
+{
+  if (storage == QP3D_STORAGE_BF16)
+    f(StoreBF16{});
+  else if (storage == QP3D_STORAGE_FP16)
+    f(StoreF16{});
+  else
+    f(StoreF32{});
+}
+
+// Work lists (qp3d_plan.hip). Interior and PML update lists for the given tile shapes and
+// resident-workgroup capacities per CU; a uniform list over [5,n-5)^3 for a tile shape.
+int qp3d_build_update_lists(const qp3d_plan *p, int tile_i, int tile_p, int occ_i, int occ_p, WorkList &wi,
+                            WorkList &wp);
+int qp3d_build_uniform_list(const qp3d_plan *p, int tile, int occ, WorkList &w);
+// Enqueues forward time step k with direct launches (no capture, no snapshot, no events).
+int qp3d_forward_step(qp3d_plan *p, int k);

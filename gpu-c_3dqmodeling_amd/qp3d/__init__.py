@@ -3,10 +3,12 @@
     import qp3d
     s = qp3d.config_setup(hx=61, hy=61, nz0=61, pml=20, nt=200, seed=0)
     rec = qp3d.modeling3d(s)            # the reference's modeling3d() on the GPU
+
+qp3d.autograd (imported separately, it needs torch) holds the torch.autograd shim.
 """
 from ._lib import EXPORTED, LIB_PATH, Qp3dError, load
-from .plan import Plan, modeling3d
+from .plan import Fwi, Plan, modeling3d
 from .setup import Setup, build_setup, config_setup, fd_tables, file_to_grid, genmodel
 
-__all__ = ["EXPORTED", "LIB_PATH", "Qp3dError", "load", "Plan", "modeling3d", "Setup", "build_setup",
+__all__ = ["EXPORTED", "LIB_PATH", "Qp3dError", "load", "Fwi", "Plan", "modeling3d", "Setup", "build_setup",
            "config_setup", "fd_tables", "file_to_grid", "genmodel"]

@@ -70,6 +70,11 @@ _SIGS = [
     ("qp3d_plan_dims", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("qp3d_plan_kernel_times", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     ("qp3d_plan_step_bytes", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    ("qp3d_fwi_create", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    ("qp3d_fwi_destroy", C.c_int, [C.c_void_p]),
+    ("qp3d_fwi_forward", C.c_int, [C.c_void_p]),
+    ("qp3d_fwi_gradient", C.c_int, [C.c_void_p, _F] + [C.POINTER(C.c_double)] * 3),
+    ("qp3d_fwi_times", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]

@@ -129,3 +129,57 @@ class Plan:
 
     def __exit__(self, *a):
         self.close()
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Fwi:
+    """Adjoint-state gradient on top of a Plan (qp3d_fwi_* in include/qp3d.h): a checkpointed
+    forward run, then the gradient of 1/2 |record - observed|^2 w.r.t. the padded-grid vp,
+    epsilon and delta, by segment recomputation and the adjoint kernels."""
+
+    def __init__(self, plan, interval=50, ckpt_storage="fp32"):
+        self.plan = plan
+        self.lib = plan.lib
+        h = C.c_void_p()
+        check(self.lib.qp3d_fwi_create(plan.h, int(interval), STORAGE[ckpt_storage], C.byref(h)))
+        self.h = h
+
+    def forward(self):
+        """Runs the shot from rest with checkpoints and returns the record [nt][hx][hy]."""
+        check(self.lib.qp3d_fwi_forward(self.h))
+        return self.plan.record()
+
+    def gradient(self, resid):
+        """resid = record - observed, [nt][hx][hy]; returns (g_vp, g_eps, g_del) [nz][nx][ny] float64."""
+        s = self.plan.s
+        r = np.ascontiguousarray(resid, np.float32).reshape(-1)
+        if r.size != s.nt * s.hx * s.hy:
+            raise ValueError("residual must be [nt][hx][hy]")
+        out = [np.zeros((s.nz, s.nx, s.ny), np.float64) for _ in range(3)]
+        check(self.lib.qp3d_fwi_gradient(self.h, fptr(r), *[_dptr(o) for o in out]))
+        return tuple(out)
+
+    def times(self):
+        ms = (C.c_double * 3)()
+        check(self.lib.qp3d_fwi_times(self.h, ms))
+        return {"forward": ms[0], "recompute": ms[1], "adjoint": ms[2]}
+
+    def close(self):
+        if self.h:
+            self.lib.qp3d_fwi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

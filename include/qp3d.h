@@ -146,6 +146,26 @@ int qp3d_plan_kernel_times(qp3d_plan *p, double *ms);
  * bytes[3], bytes[4] = cells of the interior and the PML update launches. */
 int qp3d_plan_step_bytes(const qp3d_plan *p, double *bytes);
 
+/* ---- adjoint-state FWI gradient ---------------------------------------------------------
+ * No reference counterpart (the reference models forward only). Gradient of
+ * J = 1/2 sum_k |record_k - observed_k|^2 with respect to the padded-grid vp, epsilon and delta
+ * that were uploaded to the plan, for the discrete forward operator as implemented (PML
+ * profiles held fixed, the 0.01*G threshold of the scalar operator treated as a constant).
+ * The forward run stores the state every `interval` steps (checkpoint storage FP32 or BF16);
+ * the gradient recomputes each segment from its checkpoint and back-propagates the residual
+ * with the adjoint kernels, summing in FP32 within a segment and in FP64 over time.
+ * interval >= nt stores every step of the run (full-wavefield storage). The plan must use
+ * FP32 storage and one stream. */
+typedef struct qp3d_fwi qp3d_fwi;
+int qp3d_fwi_create(qp3d_plan *p, int interval, int ckpt_storage, qp3d_fwi **out);
+int qp3d_fwi_destroy(qp3d_fwi *f);
+/* Forward run from rest with checkpoints; record, maxima and state as after qp3d_plan_run. */
+int qp3d_fwi_forward(qp3d_fwi *f);
+/* resid = record - observed, host [nt][hx][hy]; outputs host [nz][nx][ny] each. */
+int qp3d_fwi_gradient(qp3d_fwi *f, const float *resid, double *g_vp, double *g_eps, double *g_del);
+/* ms[0] checkpointed forward run, ms[1] recomputation, ms[2] adjoint sweep (device time). */
+int qp3d_fwi_times(const qp3d_fwi *f, double *ms);
+
 #ifdef __cplusplus
 }
 #endif
