@@ -21,6 +21,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 for p in (REPO, os.path.join(REPO, "gpu-c_3dqmodeling_amd")):
     if p not in sys.path:
@@ -77,6 +79,225 @@ def cpu_baseline(setup, plan, start, nsteps):
                       f"{start}..{start + nsteps - 1} from the GPU state at step {start}; {dt:.1f} s"}
 
 
+def smoothed_setup(qp3d, hx, hy, nz0, pml, nt, seed, sigma, fm=25.0, src_depth=0):
+    """Setup of the layered-plus-random model rule smoothed with a Gaussian of `sigma` cells."""
+    from scipy.ndimage import gaussian_filter
+    vp, ep, de = qp3d.genmodel(hx, hy, nz0, seed)
+    sm = [gaussian_filter(a.astype(np.float64), sigma, mode="nearest").astype(np.float32) for a in (vp, ep, de)]
+    g = [qp3d.file_to_grid(a) for a in sm]
+    return qp3d.build_setup(*g, pml, nt, 12.5, 12.5, 12.5, 0.001, fm, src_depth=src_depth)
+
+
+def bench_fwi(args):
+    """Adjoint-state FWI gradient: 256^3 interior, pml 32, 2000 steps, one shot at the surface
+    centre, 256x256 surface receivers; observed data from the seed-2 model, gradient at that
+    model smoothed with sigma = 15 cells; checkpoints every 50 steps (FP32 and BF16) and, with
+    --fwi-full, full-wavefield storage. Then the finite-difference check on a 64^3 grid, 400
+    steps, random joint directions in (vp, eps, del), against FP64 oracle misfits."""
+    import qp3d
+    n = 256 if args.n == 512 else args.n
+    nt = 2000 if args.nt == 3000 else args.nt
+    t = time.perf_counter()
+    s_true = qp3d.config_setup(hx=n, hy=n, nz0=n, pml=args.pml, nt=nt, seed=2)
+    s0 = smoothed_setup(qp3d, n, n, n, args.pml, nt, seed=2, sigma=15.0)
+    log(f"fwi setup {s0.nx}^3, nt {nt}, {time.perf_counter() - t:.1f} s")
+    with qp3d.Plan(s_true) as p:
+        obs = p.forward()
+        t = time.perf_counter()
+        p.forward()
+        t_fwd = time.perf_counter() - t
+    log(f"plain forward shot {t_fwd:.3f} s")
+    runs = {}
+    ref = None
+    cases = [("ckpt50_fp32", 50, "fp32"), ("ckpt50_bf16", 50, "bf16")]
+    if args.fwi_full:
+        cases.append(("full_storage", nt, "fp32"))
+    for key, interval, ck in cases:
+        try:
+            with qp3d.Plan(s0) as p, qp3d.Fwi(p, interval=interval, ckpt_storage=ck) as f:
+                f.forward()
+                p.sync()
+                t = time.perf_counter()
+                rec = f.forward()
+                g = f.gradient(rec - obs)
+                wall = time.perf_counter() - t
+                tm = f.times()
+        except qp3d.Qp3dError as e:
+            log(f"{key}: {e}")
+            runs[key] = {"error": str(e)}
+            continue
+        r = {"s_per_shot": round(wall, 3), "x_forward": round(wall / t_fwd, 2),
+             "ms_forward": round(tm["forward"], 1), "ms_recompute": round(tm["recompute"], 1),
+             "ms_adjoint": round(tm["adjoint"], 1), "misfit": float(0.5 * np.sum((rec.astype(np.float64) - obs) ** 2))}
+        if ref is None:
+            ref = g
+        else:
+            r["rel_diff_vs_ckpt50_fp32"] = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(g, ref)]
+        runs[key] = r
+        log(f"{key}: {r}")
+        del g
+    fd = fd_study(qp3d, args.fd_dirs)
+    out = {"metric": "FWI gradient s/shot (256^3, 2000 steps, checkpoints every 50)",
+           "value": runs["ckpt50_fp32"]["s_per_shot"], "unit": "s/shot", "n_gpus": 1,
+           "higher_is_better": False, "dtype": "fp32 (FP64 gradient accumulation)", "data": "synthetic",
+           "config": {"workload": "adjoint-state FWI gradient w.r.t. vp, eps, del; 256^3 interior, pml 32, "
+                                  "2000 steps, seed-2 true model, start model smoothed sigma=15",
+                      "padded": [s0.nx, s0.ny, s0.nz], "plain_forward_s": round(t_fwd, 3), "runs": runs,
+                      "fd_check": fd}}
+    print(json.dumps(out), flush=True)
+
+
+def fd_study(qp3d, ndirs, n=64, nt=400, pml=20, rel=1e-5):
+    """Directional derivatives of the GPU adjoint against central differences of FP64 oracle
+    misfits on the 64^3 grid: random smooth perturbations of one of vp, eps, del in turn, at
+    the smoothed seed-2 model and at the same model made elliptic (del = eps, where S = 1 on
+    both sides of the scalar operator's threshold and the misfit is smooth)."""
+    import dataclasses
+    from scipy.ndimage import gaussian_filter
+    from oracle import oracle as O
+    s_true = qp3d.config_setup(hx=n, hy=n, nz0=n, pml=pml, nt=nt, seed=2)
+    with qp3d.Plan(s_true) as p:
+        obs = p.forward().astype(np.float64)
+    base_s = smoothed_setup(qp3d, n, n, n, pml, nt, seed=2, sigma=15.0)
+    out = {"grid": [base_s.nx, base_s.ny, base_s.nz], "nt": nt, "step": rel}
+    rng = np.random.default_rng(7)
+    t0 = time.perf_counter()
+    for model in ("vti", "elliptic"):
+        s0 = base_s if model == "vti" else dataclasses.replace(base_s, dele=base_s.eps.copy())
+        with qp3d.Plan(s0) as p, qp3d.Fwi(p, interval=50) as f:
+            rec = f.forward().astype(np.float64)
+            g = f.gradient((rec - obs).astype(np.float32))
+        nd = ndirs if model == "vti" else max(3, ndirs // 4)
+        errs = {"vp": [], "eps": [], "dele": []}
+        for i in range(nd):
+            name = ("vp", "eps", "dele")[i % 3]
+            scale = float(np.abs(s0.vp).max()) if name == "vp" else 1.0
+            d = gaussian_filter(rng.standard_normal((s0.nz, s0.nx, s0.ny)), 2.0)
+            d = d / np.abs(d).max()
+            h = rel * scale
+            base = getattr(s0, name).astype(np.float64)
+            mp, mm = (base + h * d).astype(np.float32), (base - h * d).astype(np.float32)
+            jp = O.modeling3d(dataclasses.replace(s0, **{name: mp}), "f64").astype(np.float64)
+            jm = O.modeling3d(dataclasses.replace(s0, **{name: mm}), "f64").astype(np.float64)
+            dj = 0.5 * np.sum((jp - obs) ** 2) - 0.5 * np.sum((jm - obs) ** 2)
+            ad = float(np.sum(g[("vp", "eps", "dele").index(name)] * (mp.astype(np.float64) - mm.astype(np.float64))))
+            errs[name].append(abs(ad - dj) / abs(dj))
+            log(f"fd {model} {name} {i}: adjoint {ad:.8e} fd {dj:.8e} rel {errs[name][-1]:.2e} "
+                f"({time.perf_counter() - t0:.0f} s)")
+        out[model] = {k: {"n": len(v), "max_rel_err": float(max(v)), "median_rel_err": float(np.median(v))}
+                      for k, v in errs.items() if v}
+    return out
+
+
+def bench_precision(args):
+    """FP32, BF16 and FP16 storage against the FP64 oracle on a 256^3 crop of the 512^3 model
+    (centre in x and y, top 256 cells in z), pml 32, 1500 steps: relative L2 gather error and
+    peak snapshot error (steps 500 and 1000). Throughput of each storage at 512^3 comes from
+    the forward workload with --storage."""
+    import qp3d
+    from oracle import oracle as O
+    n, nt, crop = 512, 1500, 256
+    vp, ep, de = qp3d.genmodel(n, n, n, 1)
+    a, b = (n - crop) // 2, (n + crop) // 2
+    g = [qp3d.file_to_grid(np.ascontiguousarray(f[a:b, a:b, :crop])) for f in (vp, ep, de)]
+    s = qp3d.build_setup(*g, args.pml, nt, 12.5, 12.5, 12.5, 0.001, 25.0)
+    log(f"precision crop {s.nx}^3, nt {nt}")
+    res = {}
+    gpu = {}
+    for st in ("fp32", "bf16", "fp16"):
+        with qp3d.Plan(s, storage=st, snapflag=1, ntsnap=500) as p:
+            t = time.perf_counter()
+            rec = p.forward()
+            res[st] = {"s_per_shot": round(time.perf_counter() - t, 3)}
+            gpu[st] = (rec.astype(np.float64), {k: v.astype(np.float64) for k, v in p.snapshots()})
+    o = O.OracleState(s, "f64", nthreads=O.default_threads())
+    orec = np.zeros((nt, s.hx, s.hy))
+    osnap = {}
+    t0 = time.perf_counter()
+    pm = s.pml
+    for k in range(nt):
+        o.step(s.wavelet[k])
+        u = o.field("u2")
+        orec[k] = u[s.sz, pm:s.nx - pm, pm:s.ny - pm]
+        if k % 500 == 0:
+            osnap[k] = np.array(u[pm:s.nz - pm, pm:s.nx - pm, pm:s.ny - pm])
+        if k % 50 == 0:
+            log(f"oracle f64 step {k} ({time.perf_counter() - t0:.0f} s)")
+    o.close()
+    for st, (rec, snaps) in gpu.items():
+        res[st]["gather_rel_l2"] = float(np.linalg.norm(rec - orec) / np.linalg.norm(orec))
+        res[st]["snapshot_peak_err"] = {k: float(np.abs(snaps[k] - osnap[k]).max() / np.abs(osnap[k]).max())
+                                        for k in (500, 1000) if k in snaps}
+    out = {"metric": "relative L2 gather error vs FP64 oracle (256^3 crop, 1500 steps)",
+           "value": res["bf16"]["gather_rel_l2"], "unit": "relative L2", "n_gpus": 1, "higher_is_better": False,
+           "dtype": "fp32 arithmetic; storage fp32/bf16/fp16", "data": "synthetic",
+           "config": {"workload": "256^3 crop of the seed-1 512^3 model, pml 32, 1500 steps",
+                      "padded": [s.nx, s.ny, s.nz], "results": res}}
+    print(json.dumps(out), flush=True)
+
+
+def bench_batch(args):
+    """Batched shots: an 800x800x300 model (seed 3), shots on a random surface grid (seed 3),
+    each with its own window of 160-320 cells in x and y by 300 in z (pml 32), 20 Hz Ricker at
+    the window centre, the full surface receiver plane of the window, 2500 steps; all shots in
+    one launch per stage and step, replayed from a captured graph, tile shapes from the
+    autotuner. The same shots run one after another (each plan alone, also with its graph)
+    give the baseline."""
+    import qp3d
+    nt = 2500 if args.nt == 3000 else args.nt
+    rng = np.random.default_rng(3)
+    t = time.perf_counter()
+    model = qp3d.genmodel(800, 800, 300, 3)
+    setups = []
+    for i in range(args.shots):
+        wx, wy = (int(v) for v in rng.integers(160, 321, size=2))
+        sx, sy = (int(v) for v in rng.integers(0, 800, size=2))
+        x0 = min(max(sx - wx // 2, 0), 800 - wx)
+        y0 = min(max(sy - wy // 2, 0), 800 - wy)
+        setups.append(qp3d.window_setup(model, x0, y0, wx, wy, args.pml, nt, 20.0))
+    del model
+    log(f"batch setup: {args.shots} shots in {time.perf_counter() - t:.1f} s")
+    plans = [qp3d.Plan(s, storage=args.storage, use_graph=True) for s in setups]
+    cells = sum(s.n_upd for s in setups) * nt
+    out_runs = {}
+    with qp3d.Batch(plans, use_graph=True) as b:
+        t = time.perf_counter()
+        tune = b.autotune(steps=4)
+        log(f"autotune {tune} in {time.perf_counter() - t:.1f} s")
+        b.reset()
+        b.run(0, 20)
+        b.sync()
+        t0 = time.perf_counter()
+        b.reset()
+        b.run()
+        b.sync()
+        wall = time.perf_counter() - t0
+        kb = b.step_bytes()
+    log(f"batched: {wall:.2f} s")
+    t0 = time.perf_counter()
+    for p in plans:
+        p.reset()
+        p.run()
+        p.sync()
+    seq = time.perf_counter() - t0
+    log(f"one after another: {seq:.2f} s")
+    for p in plans:
+        p.close()
+    step_bytes = kb["gradmax"] + kb["update"]
+    out = {"metric": f"Gcell-updates/s ({args.shots} shots batched, 2500 steps)", "value": round(cells / wall / 1e9, 3),
+           "unit": "Gcell-updates/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(wall * 1e3, 1),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+           "data": "synthetic (layered VTI + smooth random perturbation, seed 3)",
+           "config": {"workload": f"{args.shots} shots, windows 160-320 x 160-320 x 300 of an 800x800x300 model, "
+                                  f"pml {args.pml}, {nt} steps, 20 Hz, storage {args.storage}",
+                      "cells_updated_per_step": kb["cells_updated"], "autotuned_tiles": tune["tiles"],
+                      "autotune_ms_per_step": [round(x, 4) for x in tune["ms"]],
+                      "batched_s": round(wall, 3), "sequential_s": round(seq, 3),
+                      "speedup_vs_sequential": round(seq / wall, 3),
+                      "hbm_gbs_algorithmic": round(step_bytes * nt / wall / 1e9, 1)}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,11 +318,23 @@ def main():
                     help="storage of wavefields and PML state (arithmetic is always FP32)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run a 1 GiB device-to-device copy (the reference kernel for scaling HBM counters)")
+    ap.add_argument("--shots", type=int, default=32, help="batch: number of shots")
+    ap.add_argument("--workload", default="forward", choices=["forward", "fwi", "precision", "batch"],
+                    help="forward: the 512^3 shot (default); fwi: the 256^3 adjoint gradient; "
+                         "precision: FP32/BF16/FP16 storage against the FP64 oracle on a 256^3 crop")
+    ap.add_argument("--fwi-full", action="store_true", help="fwi: also run with full-wavefield storage")
+    ap.add_argument("--fd-dirs", type=int, default=20, help="fwi: directions of the finite-difference check")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the shot-parallel and z-slab "
               f"designs for several GPUs are described, not built)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "fwi":
+        return bench_fwi(args)
+    if args.workload == "precision":
+        return bench_precision(args)
+    if args.workload == "batch":
+        return bench_batch(args)
 
     import qp3d
 

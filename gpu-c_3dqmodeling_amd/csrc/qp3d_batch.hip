@@ -1,0 +1,384 @@
+// Many shots in one launch per stage: the gradient-max pass, the interior update and the PML
+// update of every shot of a batch go out as three launches per time step whose work items
+// carry the shot's index. Each shot keeps its own plan (grid, model, source, PML profiles,
+// wavefields, record); all shots advance in lockstep. The step sequence can be captured once
+// in a hipGraph, and an autotuner picks each stage's tile shape by timing the candidates.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdio>
+#include <vector>
+
+#include "qp3d_plan_impl.hpp"
+
+struct qp3d_batch {
+  std::vector<qp3d_plan *> plans;
+  std::vector<unsigned> gen;  // each plan's upload count when the batch was built
+  int nt = 0, storage = 0, device = 0;
+  bool use_graph = false;
+  hipStream_t stream = nullptr;
+  int tile[3] = {0, 5, 1};  // gradient max, interior update, PML update (QP3D_TILES index)
+  WorkList wl[3];
+  void *args_g[2] = {}, *args_u[2] = {};  // per-parity argument blocks of every shot
+  int *kdev = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipEvent_t ev[2] = {};
+  int steps_done = 0;
+  bool timing = false;  // autotuner: events around stage `timed_stage`
+  int timed_stage = 0;
+  double timed_ms = 0;
+};
+
+namespace {
+
+template <class S>
+void launch_grad_b(int tile, const WorkList &w, hipStream_t st, const GradArgs<S> *A, Step s)
+{
+  int i = 0;
+#define QP3D_L(wx, rx)                                                                                 \
+  if (tile == i++) {                                                                                   \
+    hipLaunchKernelGGL((k_gradmax_batch<S, Tile<wx, rx>>), dim3(w.n), dim3(TY, wx), 0, st, A, w.d, s); \
+    return;                                                                                            \
+  }
+  QP3D_TILES(QP3D_L)
+#undef QP3D_L
+}
+
+template <class S, bool PML>
+void launch_upd_b(int tile, const WorkList &w, hipStream_t st, const UpdArgs<S> *A, Step s)
+{
+  int i = 0;
+#define QP3D_L(wx, rx)                                                                                       \
+  if (tile == i++) {                                                                                         \
+    hipLaunchKernelGGL((k_update_batch<S, Tile<wx, rx>, PML>), dim3(w.n), dim3(TY, wx), 0, st, A, w.d, s);  \
+    return;                                                                                                  \
+  }
+  QP3D_TILES(QP3D_L)
+#undef QP3D_L
+}
+
+template <class S>
+int occupancy_b(int stage, int tile)
+{
+  int i = 0, occ = 1;
+#define QP3D_O(wx, rx)                                                                                 \
+  if (tile == i++) {                                                                                   \
+    const void *fn = stage == 0 ? reinterpret_cast<const void *>(&k_gradmax_batch<S, Tile<wx, rx>>)   \
+                     : stage == 1 ? reinterpret_cast<const void *>(&k_update_batch<S, Tile<wx, rx>, false>) \
+                                  : reinterpret_cast<const void *>(&k_update_batch<S, Tile<wx, rx>, true>);  \
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, TY * wx, 0) != hipSuccess) occ = 1;     \
+  }
+  QP3D_TILES(QP3D_O)
+#undef QP3D_O
+  return std::max(occ, 1);
+}
+
+}  // namespace
+
+// Work list of one stage over all shots, items in batch format.
+static int build_stage(qp3d_batch *b, int stage)
+{
+  int occ = 1;
+  with_storage(b->storage, [&](auto sp) { occ = occupancy_b<decltype(sp)>(stage, b->tile[stage]); });
+  const int n = (int)b->plans.size();
+  const int cap = std::max(1, occ * qp3d_cu_count(b->plans[0]) / n);  // the shots share the machine
+  std::vector<int4> all;
+  for (int i = 0; i < n; i++) {
+    std::vector<int4> it, ip;
+    if (stage == 0)
+      qp3d_items_uniform(b->plans[i], b->tile[0], cap, it);
+    else
+      qp3d_items_update(b->plans[i], b->tile[1], b->tile[2], cap, cap, it, ip);
+    const std::vector<int4> &v = stage == 2 ? ip : it;
+    for (int4 w : v) all.push_back(make_int4(w.x, w.y, w.z | (w.w << 16), i));
+  }
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  return qp3d_upload_items(b->wl[stage], all, true);
+}
+
+template <class S>
+static int upload_args(qp3d_batch *b)
+{
+  const size_t n = b->plans.size();
+  for (int par = 0; par < 2; par++) {
+    std::vector<GradArgs<S>> g(n);
+    std::vector<UpdArgs<S>> u(n);
+    for (size_t i = 0; i < n; i++) make_args<S>(b->plans[i], par, Step{nullptr, 0}, g[i], u[i]);
+    HIP_TRY(hipMalloc(&b->args_g[par], sizeof(GradArgs<S>) * n));
+    HIP_TRY(hipMalloc(&b->args_u[par], sizeof(UpdArgs<S>) * n));
+    HIP_TRY(hipMemcpy(b->args_g[par], g.data(), sizeof(GradArgs<S>) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->args_u[par], u.data(), sizeof(UpdArgs<S>) * n, hipMemcpyHostToDevice));
+  }
+  return QP3D_OK;
+}
+
+template <class S>
+static int batch_step_s(qp3d_batch *b, int k, Step st)
+{
+  using T = typename S::T;
+  const int par = k & 1;
+  for (int s = 0; s < 3; s++) {
+    const bool tm = b->timing && s == b->timed_stage;
+    if (tm) HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+    if (b->wl[s].n) {
+      if (s == 0)
+        launch_grad_b<S>(b->tile[0], b->wl[0], b->stream, (const GradArgs<S> *)b->args_g[par], st);
+      else if (s == 1)
+        launch_upd_b<S, false>(b->tile[1], b->wl[1], b->stream, (const UpdArgs<S> *)b->args_u[par], st);
+      else
+        launch_upd_b<S, true>(b->tile[2], b->wl[2], b->stream, (const UpdArgs<S> *)b->args_u[par], st);
+      HIP_TRY(hipGetLastError());
+    }
+    if (tm) {
+      HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+      HIP_TRY(hipEventSynchronize(b->ev[1]));
+      float ms = 0;
+      HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+      b->timed_ms += ms;
+    }
+  }
+  // source cells on the frozen rim of any shot (u_next = u_cur + s w)
+  for (qp3d_plan *p : b->plans) {
+    if (!p->src_rim) continue;
+    const int prev = k & 1, cur = prev ^ 1;
+    const long nb = (long)(p->sb[1] - p->sb[0] + 1) * (p->sb[3] - p->sb[2] + 1) * (p->sb[5] - p->sb[4] + 1);
+    const int blocks = (int)std::min<long>((nb + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_rim_source<S>, dim3(blocks), dim3(256), 0, b->stream, (const T *)p->u[cur], (T *)p->u[prev],
+                       p->geo, p->sb[0], p->sb[1], p->sb[2], p->sb[3], p->sb[4], p->sb[5], (const float *)p->sbox,
+                       (const float *)p->wav, st);
+    HIP_TRY(hipGetLastError());
+  }
+  return QP3D_OK;
+}
+
+static int batch_step(qp3d_batch *b, int k, Step st)
+{
+  int rc = QP3D_OK;
+  with_storage(b->storage, [&](auto sp) { rc = batch_step_s<decltype(sp)>(b, k, st); });
+  return rc;
+}
+
+static void batch_free(qp3d_batch *b)
+{
+  if (!b) return;
+  for (WorkList &w : b->wl) (void)hipFree(w.d);
+  for (int i = 0; i < 2; i++) {
+    (void)hipFree(b->args_g[i]);
+    (void)hipFree(b->args_u[i]);
+  }
+  (void)hipFree(b->kdev);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (hipEvent_t e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+extern "C" int qp3d_batch_create(qp3d_plan *const *plans, int nshots, int use_graph, qp3d_batch **out)
+{
+  if (!plans || !out || nshots < 1) {
+    qp3d_set_error("batch_create: need at least one plan");
+    return QP3D_ERR_ARG;
+  }
+  *out = nullptr;
+  const qp3d_config &c0 = plans[0]->cfg;
+  for (int i = 0; i < nshots; i++) {
+    const qp3d_plan *p = plans[i];
+    if (!p || !p->uploaded) {
+      qp3d_set_error("batch_create: plan %d is null or has no model uploaded", i);
+      return QP3D_ERR_STATE;
+    }
+    if (p->cfg.nt != c0.nt || p->cfg.storage != c0.storage || p->cfg.device != c0.device || p->cfg.nz >= 65536) {
+      qp3d_set_error("batch_create: plan %d differs in nt, storage or device from plan 0", i);
+      return QP3D_ERR_ARG;
+    }
+  }
+  HIP_TRY(hipSetDevice(c0.device));
+  qp3d_batch *b = new qp3d_batch();
+  b->plans.assign(plans, plans + nshots);
+  for (int i = 0; i < nshots; i++) b->gen.push_back(plans[i]->upload_gen);
+  b->nt = c0.nt;
+  b->storage = c0.storage;
+  b->device = c0.device;
+  b->use_graph = use_graph != 0;
+  auto fail = [&](int rc) {
+    batch_free(b);
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev[0]) != hipSuccess || hipEventCreate(&b->ev[1]) != hipSuccess ||
+      hipMalloc((void **)&b->kdev, sizeof(int)) != hipSuccess) {
+    qp3d_set_error("batch_create: stream, events or step index allocation failed");
+    return fail(QP3D_ERR_HIP);
+  }
+  int rc = QP3D_OK;
+  with_storage(b->storage, [&](auto sp) { rc = upload_args<decltype(sp)>(b); });
+  if (rc) return fail(rc);
+  for (int s = 0; s < 3; s++)
+    if ((rc = build_stage(b, s))) return fail(rc);
+  // the plans' own streams may still hold uploads
+  for (qp3d_plan *p : b->plans) HIP_TRY(hipStreamSynchronize(p->stream));
+  *out = b;
+  return QP3D_OK;
+}
+
+extern "C" int qp3d_batch_destroy(qp3d_batch *b)
+{
+  if (!b) return QP3D_OK;
+  (void)hipSetDevice(b->device);
+  (void)hipStreamSynchronize(b->stream);
+  batch_free(b);
+  return QP3D_OK;
+}
+
+// Zeroes every shot's wavefields, PML state, maxima, flag and record on the batch stream.
+extern "C" int qp3d_batch_reset(qp3d_batch *b)
+{
+  if (!b) return QP3D_ERR_ARG;
+  HIP_TRY(hipSetDevice(b->device));
+  for (qp3d_plan *p : b->plans) {
+    HIP_TRY(hipMemsetAsync(p->state, 0, p->esz * 9 * (size_t)p->N, b->stream));
+    HIP_TRY(hipMemsetAsync(p->gmax, 0, sizeof(unsigned) * (size_t)b->nt, b->stream));
+    HIP_TRY(hipMemsetAsync(p->flag, 0, sizeof(int), b->stream));
+    if (p->rec) HIP_TRY(hipMemsetAsync(p->rec, 0, sizeof(float) * (size_t)b->nt * p->hx * p->hy, b->stream));
+    p->steps_done = 0;
+  }
+  b->steps_done = 0;
+  return QP3D_OK;
+}
+
+static int batch_capture(qp3d_batch *b)
+{
+  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = QP3D_OK;
+  for (int i = 0; i < GRAPH_LEN && rc == QP3D_OK; i++) rc = batch_step(b, i, Step{b->kdev, i});
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc == QP3D_OK && e != hipSuccess) {
+    qp3d_set_error("batch_run: graph capture failed: %s", hipGetErrorString(e));
+    rc = QP3D_ERR_HIP;
+  }
+  if (rc == QP3D_OK && (e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0)) != hipSuccess) {
+    qp3d_set_error("batch_run: graph instantiation failed: %s", hipGetErrorString(e));
+    b->gexec = nullptr;
+    rc = QP3D_ERR_HIP;
+  }
+  if (rc == QP3D_OK)
+    b->graph = g;
+  else if (g)
+    (void)hipGraphDestroy(g);
+  return rc;
+}
+
+// Enqueues time steps [k0, k1) of every shot (asynchronous on the batch's stream).
+extern "C" int qp3d_batch_run(qp3d_batch *b, int k0, int k1)
+{
+  if (!b || k0 < 0 || k1 > b->nt || k0 > k1) {
+    qp3d_set_error("batch_run: bad arguments");
+    return QP3D_ERR_ARG;
+  }
+  for (size_t i = 0; i < b->plans.size(); i++)
+    if (b->plans[i]->upload_gen != b->gen[i]) {
+      qp3d_set_error("batch_run: plan %zu was uploaded again after the batch was built; build a new batch", i);
+      return QP3D_ERR_STATE;
+    }
+  HIP_TRY(hipSetDevice(b->device));
+  const bool graph = b->use_graph && !b->timing;
+  for (int k = k0; k < k1;) {
+    if (graph && (k & 1) == 0 && k + GRAPH_LEN <= k1) {
+      if (!b->gexec) {
+        int rc = batch_capture(b);
+        if (rc) return rc;
+      }
+      hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, b->stream, b->kdev, k);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipGraphLaunch(b->gexec, b->stream));
+      k += GRAPH_LEN;
+    } else {
+      int rc = batch_step(b, k, Step{nullptr, k});
+      if (rc) return rc;
+      k++;
+    }
+  }
+  b->steps_done = k1;
+  for (qp3d_plan *p : b->plans) p->steps_done = k1;
+  return QP3D_OK;
+}
+
+// Waits for the batch; QP3D_ERR_NONFINITE names the first shot whose wavefield blew up.
+extern "C" int qp3d_batch_sync(qp3d_batch *b)
+{
+  if (!b) return QP3D_ERR_ARG;
+  HIP_TRY(hipSetDevice(b->device));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (size_t i = 0; i < b->plans.size(); i++) {
+    int flag = 0;
+    HIP_TRY(hipMemcpy(&flag, b->plans[i]->flag, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag) {
+      qp3d_set_error("batch: the wavefield of shot %zu became non-finite (inf/NaN)", i);
+      return QP3D_ERR_NONFINITE;
+    }
+  }
+  return QP3D_OK;
+}
+
+// Times every tile shape for each stage over `steps` time steps from rest, keeps the fastest,
+// and resets the batch. tiles_out (3 ints, may be NULL) receives the choice (1-based, as in
+// qp3d_config.tile_*); ms_out (3 doubles, may be NULL) the chosen stages' mean ms per step.
+extern "C" int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, double *ms_out)
+{
+  if (!b || steps < 1 || steps + 2 > b->nt) {
+    qp3d_set_error("batch_autotune: need 1 <= steps <= nt - 2");
+    return QP3D_ERR_ARG;
+  }
+  HIP_TRY(hipSetDevice(b->device));
+  int rc;
+  for (int s = 0; s < 3; s++) {
+    double best = DBL_MAX;
+    int bt = b->tile[s];
+    for (int t = 0; t < NTILES; t++) {
+      b->tile[s] = t;
+      if ((rc = build_stage(b, s))) return rc;
+      if ((rc = qp3d_batch_reset(b))) return rc;
+      if ((rc = qp3d_batch_run(b, 0, 2))) return rc;  // warm-up and a non-zero field
+      b->timing = true;
+      b->timed_stage = s;
+      b->timed_ms = 0;
+      rc = qp3d_batch_run(b, 2, 2 + steps);
+      b->timing = false;
+      if (rc) return rc;
+      const double ms = b->timed_ms / steps;
+      if (ms < best) {
+        best = ms;
+        bt = t;
+      }
+    }
+    b->tile[s] = bt;
+    if ((rc = build_stage(b, s))) return rc;
+    if (ms_out) ms_out[s] = best;
+    if (tiles_out) tiles_out[s] = bt + 1;
+  }
+  return qp3d_batch_reset(b);
+}
+
+// Algorithmic HBM bytes of one batched time step (every shot), as qp3d_plan_step_bytes sums.
+extern "C" int qp3d_batch_step_bytes(const qp3d_batch *b, double *bytes)
+{
+  if (!b || !bytes) return QP3D_ERR_ARG;
+  bytes[0] = bytes[1] = bytes[2] = bytes[3] = 0;
+  for (const qp3d_plan *p : b->plans) {
+    double v[5];
+    int rc = qp3d_plan_step_bytes(p, v);
+    if (rc) return rc;
+    bytes[0] += v[0];
+    bytes[1] += v[1] + v[2];
+    bytes[2] += (double)(p->cfg.nx - 10) * (p->cfg.ny - 10) * (p->cfg.nz - 10);  // updated cells
+    bytes[3] += (double)p->hx * p->hy * p->hz;                                 // interior cells
+  }
+  return QP3D_OK;
+}

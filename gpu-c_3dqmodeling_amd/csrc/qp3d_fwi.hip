@@ -40,6 +40,8 @@ struct qp3d_fwi {
   float *resid = nullptr;        // [nt][hx][hy]
   WorkList wl_i, wl_p, wl_a;
   bool forward_done = false;
+  int ubuf_seg = -1;             // segment whose u_k ubuf holds (the forward run keeps the last)
+  unsigned gen = 0;              // the plan's upload count at the forward run
   hipEvent_t ev[4] = {};
   double ms_forward = 0, ms_recompute = 0, ms_adjoint = 0;
 };
@@ -179,10 +181,17 @@ extern "C" int qp3d_fwi_forward(qp3d_fwi *f)
   int rc = qp3d_plan_reset(p);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(f->ev[0], p->stream));
+  const int klast = (f->nseg - 1) * f->interval;  // steps of the last segment keep u_k
+  const size_t N = (size_t)p->N;
   for (int k = 0; k < p->cfg.nt; k++) {
     if (k % f->interval == 0 && k > 0 && (rc = save_ckpt(f, k / f->interval))) return rc;
+    if (k >= klast)
+      HIP_TRY(hipMemcpyAsync(f->ubuf + (size_t)(k - klast) * N, p->u[(k & 1) ^ 1], sizeof(float) * N,
+                             hipMemcpyDeviceToDevice, p->stream));
     if ((rc = qp3d_forward_step(p, k))) return rc;
   }
+  f->ubuf_seg = f->nseg - 1;
+  f->gen = p->upload_gen;
   p->steps_done = p->cfg.nt;
   HIP_TRY(hipEventRecord(f->ev[1], p->stream));
   rc = qp3d_plan_sync(p);
@@ -246,8 +255,8 @@ extern "C" int qp3d_fwi_gradient(qp3d_fwi *f, const float *resid, double *g_vp, 
     qp3d_set_error("fwi_gradient: null argument");
     return QP3D_ERR_ARG;
   }
-  if (!f->forward_done) {
-    qp3d_set_error("fwi_gradient: run qp3d_fwi_forward first");
+  if (!f->forward_done || f->gen != f->p->upload_gen) {
+    qp3d_set_error("fwi_gradient: run qp3d_fwi_forward first (and again after every upload)");
     return QP3D_ERR_STATE;
   }
   qp3d_plan *p = f->p;
@@ -263,12 +272,15 @@ extern "C" int qp3d_fwi_gradient(qp3d_fwi *f, const float *resid, double *g_vp, 
   for (int s = f->nseg - 1; s >= 0; s--) {
     const int k0 = s * f->interval, k1 = std::min(k0 + f->interval, c.nt);
     HIP_TRY(hipEventRecord(f->ev[0], p->stream));
-    if ((rc = load_ckpt(f, s))) return rc;
-    for (int k = k0; k < k1; k++) {
-      const int cur = (k & 1) ^ 1;  // slot of u_k in the forward state
-      HIP_TRY(hipMemcpyAsync(f->ubuf + (size_t)(k - k0) * N, p->u[cur], sizeof(float) * N, hipMemcpyDeviceToDevice,
-                             p->stream));
-      if ((rc = qp3d_forward_step(p, k))) return rc;
+    if (s != f->ubuf_seg) {
+      if ((rc = load_ckpt(f, s))) return rc;
+      for (int k = k0; k < k1; k++) {
+        const int cur = (k & 1) ^ 1;  // slot of u_k in the forward state
+        HIP_TRY(hipMemcpyAsync(f->ubuf + (size_t)(k - k0) * N, p->u[cur], sizeof(float) * N,
+                               hipMemcpyDeviceToDevice, p->stream));
+        if ((rc = qp3d_forward_step(p, k))) return rc;
+      }
+      f->ubuf_seg = s;
     }
     HIP_TRY(hipEventRecord(f->ev[1], p->stream));
     HIP_TRY(hipMemsetAsync(f->gacc, 0, sizeof(float) * 3 * N, p->stream));
@@ -294,8 +306,8 @@ extern "C" int qp3d_fwi_gradient(qp3d_fwi *f, const float *resid, double *g_vp, 
   HIP_TRY(hipStreamSynchronize(p->stream));
   f->ms_recompute = ms_re;
   f->ms_adjoint = ms_adj;
-  // the plan's state now holds the end of segment 0's recomputation
-  p->steps_done = std::min(f->interval, c.nt);
+  // the plan's state holds the end of the last segment recomputed (or of the forward run)
+  if (f->nseg > 1) p->steps_done = std::min(f->interval, c.nt);
   return QP3D_OK;
 }
 

@@ -318,8 +318,9 @@ struct Unroll<N, N> {
   }
 };
 
+// One work item of the gradient-max pass (w: packed tile, kstep: time step).
 template <class S, class T>
-__global__ __launch_bounds__(T::NT) void k_gradmax(GradArgs<S> a)
+__device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int kstep)
 {
   __shared__ float sm[2][T::LX][T::LY];
   __shared__ float red[T::WX];
@@ -327,7 +328,7 @@ __global__ __launch_bounds__(T::NT) void k_gradmax(GradArgs<S> a)
   const Geo &g = a.g;
   int x0, x1, y0, y1, z0, z1;
   bool fold;
-  unpack(a.work[blockIdx.x], x0, x1, y0, y1, z0, z1, fold);
+  unpack(w, x0, x1, y0, y1, z0, z1, fold);
   if (z0 >= z1) return;
   const Lane<T> L(g, x0, x1, y0, y1, fold);
   const bool ycmp = L.yon;
@@ -410,9 +411,33 @@ __global__ __launch_bounds__(T::NT) void k_gradmax(GradArgs<S> a)
       v = fmaxf(v, red[w]);
       bb |= redbad[w];
     }
-    atomicMax(a.gmax + a.step.get(), __float_as_uint(v));
+    atomicMax(a.gmax + kstep, __float_as_uint(v));
     if (bb) atomicOr(a.flag, 1);
   }
+}
+
+// Batched launches (several grids in one launch): item .z packs z0 | z1 << 16 and .w is the
+// index of the grid's argument block.
+__device__ __forceinline__ int4 batch_item(const int4 *work, int &shot)
+{
+  const int4 w = work[blockIdx.x];
+  shot = w.w;
+  return make_int4(w.x, w.y, w.z & 0xffff, (int)((unsigned)w.z >> 16));
+}
+
+template <class S, class T>
+__global__ __launch_bounds__(T::NT) void k_gradmax(GradArgs<S> a)
+{
+  gradmax_tile<S, T>(a, a.work[blockIdx.x], a.step.get());
+}
+
+template <class S, class T>
+__global__ __launch_bounds__(T::NT) void k_gradmax_batch(const GradArgs<S> *A, const int4 *work, Step st)
+{
+  int shot;
+  const int4 w = batch_item(work, shot);
+  const GradArgs<S> a = A[shot];
+  gradmax_tile<S, T>(a, w, st.get());
 }
 
 // Per-cell inputs of k_update other than u_cur: model, and either u_prev (interior tiles) or
@@ -447,14 +472,15 @@ struct CellLoad {
   }
 };
 
+// One work item of the update pass (w: packed tile, kstep: time step).
 template <class S, class T, bool PML>
-__global__ __launch_bounds__(T::NT) void k_update(UpdArgs<S> a)
+__device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kstep)
 {
   __shared__ float sm[2][T::LX][T::LY];
   const Geo &g = a.g;
   int x0, x1, y0, y1, z0, z1;
   bool fold;
-  unpack(a.work[blockIdx.x], x0, x1, y0, y1, z0, z1, fold);
+  unpack(w, x0, x1, y0, y1, z0, z1, fold);
   if (z0 >= z1) return;
   const Lane<T> L(g, x0, x1, y0, y1, fold);
   const int y = L.y;
@@ -466,7 +492,6 @@ __global__ __launch_bounds__(T::NT) void k_update(UpdArgs<S> a)
 
   // S threshold: |grad u| > 0.01*G compared in double (GPU_kernel.cu:99); for a float
   // magnitude this equals mode > rd(0.01*G).
-  const int kstep = a.step.get();
   const float G = __uint_as_float(a.gmax[kstep]);
   const float wk = a.wav[kstep];
   float *const rec = a.rec + (size_t)kstep * (g.nx - 2 * g.pml) * (g.ny - 2 * g.pml);
@@ -636,6 +661,21 @@ __global__ __launch_bounds__(T::NT) void k_update(UpdArgs<S> a)
       }
     }
   }
+}
+
+template <class S, class T, bool PML>
+__global__ __launch_bounds__(T::NT) void k_update(UpdArgs<S> a)
+{
+  update_tile<S, T, PML>(a, a.work[blockIdx.x], a.step.get());
+}
+
+template <class S, class T, bool PML>
+__global__ __launch_bounds__(T::NT) void k_update_batch(const UpdArgs<S> *A, const int4 *work, Step st)
+{
+  int shot;
+  const int4 w = batch_item(work, shot);
+  const UpdArgs<S> a = A[shot];  // one copy: stores to the fields must not force reloads
+  update_tile<S, T, PML>(a, w, st.get());
 }
 
 // Source cells outside [5,n-5)^3 are never updated by the stencil; the reference's u3 keeps

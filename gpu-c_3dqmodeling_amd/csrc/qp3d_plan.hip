@@ -111,30 +111,36 @@ std::vector<int4> xcd_order(const std::vector<int4> &logical)
 }
 }  // namespace
 
-static int upload_list(WorkList &w, const std::vector<int4> &logical)
+int qp3d_upload_items(WorkList &w, const std::vector<int4> &logical, bool batch)
 {
   std::vector<int4> v = xcd_order(logical);
   w.n = (int)v.size();
   w.cells = 0;
   for (const int4 &t : logical) {
     const long ny_ = (long)((((unsigned)t.y >> 16) & 0x7fff) - (t.y & 0xffff)) * (t.y < 0 ? 2 : 1);
-    w.cells += (long)(((unsigned)t.x >> 16) - (t.x & 0xffff)) * ny_ * (t.w - t.z);
+    const long nz_ = batch ? (long)(((unsigned)t.z >> 16) - (t.z & 0xffff)) : (long)(t.w - t.z);
+    w.cells += (long)(((unsigned)t.x >> 16) - (t.x & 0xffff)) * ny_ * nz_;
   }
+  (void)hipFree(w.d);
+  w.d = nullptr;
   if (w.n == 0) return QP3D_OK;
   HIP_TRY(hipMalloc((void **)&w.d, sizeof(int4) * v.size()));
   HIP_TRY(hipMemcpy(w.d, v.data(), sizeof(int4) * v.size(), hipMemcpyHostToDevice));
   return QP3D_OK;
 }
 
-static int cu_count(const qp3d_plan *p)
+static int upload_list(WorkList &w, const std::vector<int4> &logical) { return qp3d_upload_items(w, logical, false); }
+
+int qp3d_cu_count(const qp3d_plan *p)
 {
   int ncu = 256;
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->cfg.device);
   return ncu;
 }
 
-// Uniform tiles over [5,n-5)^3 (the gradient-max pass).
-int qp3d_build_uniform_list(const qp3d_plan *p, int tile, int occ, WorkList &w)
+// Uniform tiles over [5,n-5)^3 (the gradient-max pass); `capacity` is the number of
+// workgroups resident at once that the items should fill.
+void qp3d_items_uniform(const qp3d_plan *p, int tile, int capacity, std::vector<int4> &items)
 {
   const qp3d_config &c = p->cfg;
   const int TX = tile_tx(tile);
@@ -142,21 +148,25 @@ int qp3d_build_uniform_list(const qp3d_plan *p, int tile, int occ, WorkList &w)
   cut(xs, H, c.nx - H, TX);
   cut(ys, H, c.ny - H, TY);
   const int len = c.nz - 2 * H;
-  const int zc = choose_zchunk((int)(xs.size() * ys.size()), len, occ * cu_count(p), c.zchunk);
-  std::vector<int4> items;
+  const int zc = choose_zchunk((int)(xs.size() * ys.size()), len, capacity, c.zchunk);
   for (int z = H; z < c.nz - H; z += zc)
     for (const Seg &x : xs)
       for (const Seg &y : ys) items.push_back(pack(x.a, x.b, y.a, y.b, z, std::min(z + zc, c.nz - H)));
+}
+
+int qp3d_build_uniform_list(const qp3d_plan *p, int tile, int occ, WorkList &w)
+{
+  std::vector<int4> items;
+  qp3d_items_uniform(p, tile, occ * qp3d_cu_count(p), items);
   return upload_list(w, items);
 }
 
 // Update tiles aligned to the PML boundary in x and y, z split at the PML planes: interior
 // tiles (tile shape tile_i) hold only interior cells, PML tiles (tile_p) one face family each.
-int qp3d_build_update_lists(const qp3d_plan *p, int tile_i, int tile_p, int occ_i, int occ_p, WorkList &wi,
-                            WorkList &wp)
+void qp3d_items_update(const qp3d_plan *p, int tile_i, int tile_p, int cap_i, int cap_p, std::vector<int4> &it,
+                       std::vector<int4> &ip)
 {
   const qp3d_config &c = p->cfg;
-  const int ncu = cu_count(p);
   const int pml = c.pml;
   auto segs = [](std::vector<Seg> &out, std::vector<bool> &inner, int n, int pml, int w) {
     const int bounds[4] = {H, pml, n - pml, n - H};
@@ -191,9 +201,8 @@ int qp3d_build_update_lists(const qp3d_plan *p, int tile_i, int tile_p, int occ_
     }
   }
   const int zmid = c.nz - 2 * pml, zpml = pml - H;
-  const int zc_i = choose_zchunk(nint_tiles, zmid, occ_i * ncu, c.zchunk);
-  const int zc_p = choose_zchunk((int)ptiles.size(), c.nz - 2 * H, occ_p * ncu, c.zchunk);
-  std::vector<int4> it, ip;
+  const int zc_i = choose_zchunk(nint_tiles, zmid, cap_i, c.zchunk);
+  const int zc_p = choose_zchunk((int)ptiles.size(), c.nz - 2 * H, cap_p, c.zchunk);
   for (int z = pml; z < c.nz - pml; z += zc_i)
     for (size_t a = 0; a < xu.size(); a++)
       for (size_t b = 0; b < ys.size(); b++)
@@ -215,6 +224,14 @@ int qp3d_build_update_lists(const qp3d_plan *p, int tile_i, int tile_p, int occ_
         for (size_t b = 0; b < ys.size(); b++)
           if (xi[a] && yi[b]) ip.push_back(pack(xs[a].a, xs[a].b, ys[b].a, ys[b].b, z, std::min(z + zc, zb)));
   }
+}
+
+int qp3d_build_update_lists(const qp3d_plan *p, int tile_i, int tile_p, int occ_i, int occ_p, WorkList &wi,
+                            WorkList &wp)
+{
+  std::vector<int4> it, ip;
+  const int ncu = qp3d_cu_count(p);
+  qp3d_items_update(p, tile_i, tile_p, occ_i * ncu, occ_p * ncu, it, ip);
   int rc = upload_list(wi, it);
   if (rc) return rc;
   return upload_list(wp, ip);
@@ -493,6 +510,7 @@ extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps,
   p->gexec = nullptr;
   p->graph = nullptr;
   p->uploaded = true;
+  p->upload_gen++;
   return QP3D_OK;
 }
 
@@ -558,13 +576,8 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare)
   if (p->two_streams) HIP_TRY(hipStreamWaitEvent(p->stream, p->ev_pml, 0));
   if (e) HIP_TRY(hipEventRecord(e[0], p->stream));
   GradArgs<S> ga;
-  ga.g = p->geo;
-  ga.c = p->coef;
-  ga.ucur = (const T *)p->u[cur];
-  ga.gmax = p->gmax;
-  ga.flag = p->flag;
-  ga.work = p->wl_grad.d;
-  ga.step = st;
+  UpdArgs<S> ua;
+  make_args<S>(p, k & 1, st, ga, ua);
   launch_gradmax(p->tile_g, p->wl_grad, p->stream, ga);
   HIP_TRY(hipGetLastError());
   if (e) HIP_TRY(hipEventRecord(e[1], p->stream));
@@ -573,26 +586,6 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare)
     HIP_TRY(hipStreamWaitEvent(p->stream2, p->ev_grad, 0));
   }
 
-  UpdArgs<S> ua;
-  ua.g = p->geo;
-  ua.c = p->coef;
-  ua.ucur = (const T *)p->u[cur];
-  ua.unext = (T *)p->u[prev];
-  ua.vdt2 = p->vdt2; ua.eps = p->eps; ua.epsd = p->epsd;
-  ua.w1p = (T *)p->w1[prev]; ua.w1c = (const T *)p->w1[cur];
-  ua.w3p = (T *)p->w3[prev]; ua.w3c = (const T *)p->w3[cur];
-  ua.pp = (T *)p->pp[prev]; ua.pc = (const T *)p->pp[cur];
-  ua.w21 = (T *)p->w21;
-  ua.pmltab = p->pmltab;
-  ua.gmax = p->gmax;
-  ua.wav = p->wav;
-  ua.step = st;
-  ua.src_on = p->src_on ? 1 : 0;
-  ua.sb_z0 = p->sb[0]; ua.sb_z1 = p->sb[1]; ua.sb_x0 = p->sb[2]; ua.sb_x1 = p->sb[3];
-  ua.sb_y0 = p->sb[4]; ua.sb_y1 = p->sb[5];
-  ua.sbox = p->sbox;
-  ua.rec = p->rec;
-  ua.sz = cf.sz;
   if (p->wl_int.n) {
     ua.work = p->wl_int.d;
     launch_update<S, false>(p->tile_u, p->wl_int, p->stream, ua);

@@ -76,6 +76,7 @@ struct qp3d_plan {
   int nsnap_taken = 0;
 
   bool uploaded = false;
+  unsigned upload_gen = 0;  // counts uploads (objects built on the plan check it)
   int steps_done = 0;  // one past the last step enqueued since the last reset
 
   bool instrument = false;
@@ -108,10 +109,52 @@ inline void with_storage(int storage, F f)
     f(StoreF32{});
 }
 
+// Kernel arguments of a step of parity `prev` (slot prev holds u(k-1) and receives u(k+1)).
+template <class S>
+inline void make_args(const qp3d_plan *p, int prev, Step st, GradArgs<S> &ga, UpdArgs<S> &ua)
+{
+  using T = typename S::T;
+  const int cur = prev ^ 1;
+  ga.g = p->geo;
+  ga.c = p->coef;
+  ga.ucur = (const T *)p->u[cur];
+  ga.gmax = p->gmax;
+  ga.flag = p->flag;
+  ga.work = p->wl_grad.d;
+  ga.step = st;
+  ua.g = p->geo;
+  ua.c = p->coef;
+  ua.ucur = (const T *)p->u[cur];
+  ua.unext = (T *)p->u[prev];
+  ua.vdt2 = p->vdt2; ua.eps = p->eps; ua.epsd = p->epsd;
+  ua.w1p = (T *)p->w1[prev]; ua.w1c = (const T *)p->w1[cur];
+  ua.w3p = (T *)p->w3[prev]; ua.w3c = (const T *)p->w3[cur];
+  ua.pp = (T *)p->pp[prev]; ua.pc = (const T *)p->pp[cur];
+  ua.w21 = (T *)p->w21;
+  ua.pmltab = p->pmltab;
+  ua.gmax = p->gmax;
+  ua.wav = p->wav;
+  ua.step = st;
+  ua.src_on = p->src_on ? 1 : 0;
+  ua.sb_z0 = p->sb[0]; ua.sb_z1 = p->sb[1]; ua.sb_x0 = p->sb[2]; ua.sb_x1 = p->sb[3];
+  ua.sb_y0 = p->sb[4]; ua.sb_y1 = p->sb[5];
+  ua.sbox = p->sbox;
+  ua.rec = p->rec;
+  ua.sz = p->cfg.sz;
+  ua.work = nullptr;
+}
+
 // Work lists (qp3d_plan.hip). Interior and PML update lists for the given tile shapes and
 // resident-workgroup capacities per CU; a uniform list over [5,n-5)^3 for a tile shape.
 int qp3d_build_update_lists(const qp3d_plan *p, int tile_i, int tile_p, int occ_i, int occ_p, WorkList &wi,
                             WorkList &wp);
 int qp3d_build_uniform_list(const qp3d_plan *p, int tile, int occ, WorkList &w);
+// The same items in host memory (logical order) for `capacity` resident workgroups.
+void qp3d_items_uniform(const qp3d_plan *p, int tile, int capacity, std::vector<int4> &items);
+void qp3d_items_update(const qp3d_plan *p, int tile_i, int tile_p, int cap_i, int cap_p, std::vector<int4> &it,
+                       std::vector<int4> &ip);
+// Uploads items in XCD-aware order (batch: items hold z0 | z1 << 16 and a grid index).
+int qp3d_upload_items(WorkList &w, const std::vector<int4> &logical, bool batch);
+int qp3d_cu_count(const qp3d_plan *p);
 // Enqueues forward time step k with direct launches (no capture, no snapshot, no events).
 int qp3d_forward_step(qp3d_plan *p, int k);

@@ -75,6 +75,13 @@ _SIGS = [
     ("qp3d_fwi_forward", C.c_int, [C.c_void_p]),
     ("qp3d_fwi_gradient", C.c_int, [C.c_void_p, _F] + [C.POINTER(C.c_double)] * 3),
     ("qp3d_fwi_times", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    ("qp3d_batch_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    ("qp3d_batch_destroy", C.c_int, [C.c_void_p]),
+    ("qp3d_batch_reset", C.c_int, [C.c_void_p]),
+    ("qp3d_batch_run", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("qp3d_batch_sync", C.c_int, [C.c_void_p]),
+    ("qp3d_batch_autotune", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    ("qp3d_batch_step_bytes", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]

@@ -183,3 +183,60 @@ class Fwi:
 
     def __exit__(self, *a):
         self.close()
+
+
+class Batch:
+    """Several shots (one Plan each) advanced in lockstep with one launch per stage and step
+    (qp3d_batch_* in include/qp3d.h)."""
+
+    def __init__(self, plans, use_graph=False):
+        self.plans = list(plans)
+        self.lib = self.plans[0].lib
+        arr = (C.c_void_p * len(self.plans))(*[p.h.value for p in self.plans])
+        h = C.c_void_p()
+        check(self.lib.qp3d_batch_create(arr, len(self.plans), int(use_graph), C.byref(h)))
+        self.h = h
+
+    def reset(self):
+        check(self.lib.qp3d_batch_reset(self.h))
+
+    def run(self, k0=0, k1=None):
+        check(self.lib.qp3d_batch_run(self.h, k0, self.plans[0].s.nt if k1 is None else k1))
+
+    def sync(self):
+        check(self.lib.qp3d_batch_sync(self.h))
+
+    def forward(self):
+        """All shots from rest; returns their records."""
+        self.reset()
+        self.run()
+        self.sync()
+        return [p.record() for p in self.plans]
+
+    def autotune(self, steps=5):
+        tiles = (C.c_int * 3)()
+        ms = (C.c_double * 3)()
+        check(self.lib.qp3d_batch_autotune(self.h, int(steps), tiles, ms))
+        return {"tiles": list(tiles), "ms": list(ms)}
+
+    def step_bytes(self):
+        b = (C.c_double * 4)()
+        check(self.lib.qp3d_batch_step_bytes(self.h, b))
+        return {"gradmax": b[0], "update": b[1], "cells_updated": b[2], "cells_interior": b[3]}
+
+    def close(self):
+        if self.h:
+            self.lib.qp3d_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -122,3 +122,13 @@ def config_setup(hx, hy, nz0, pml, nt, seed, h=12.5, dt=0.001, fm=25.0, src_dept
     vpf, epf, def_ = genmodel(hx, hy, nz0, seed, sigma, amp)
     return build_setup(file_to_grid(vpf), file_to_grid(epf), file_to_grid(def_), pml, nt, h, h, h, dt, fm,
                        src_depth=src_depth)
+
+
+def window_setup(model, x0, y0, hx, hy, pml, nt, fm, h=12.5, dt=0.001, src_depth=0):
+    """Setup of one shot of a survey (Main.c:179-201): the window [x0, x0+hx) x [y0, y0+hy) x
+    [0, nz0) of a model held in model-file layout [y][x][z] (as readmodel3d reads it), with the
+    source at the window centre."""
+    win = [np.ascontiguousarray(a[y0:y0 + hy, x0:x0 + hx, :]) for a in model]
+    if any(w.shape[:2] != (hy, hx) for w in win):
+        raise ValueError("window outside the model")
+    return build_setup(*[file_to_grid(w) for w in win], pml, nt, h, h, h, dt, fm, src_depth=src_depth)

@@ -166,6 +166,25 @@ int qp3d_fwi_gradient(qp3d_fwi *f, const float *resid, double *g_vp, double *g_e
 /* ms[0] checkpointed forward run, ms[1] recomputation, ms[2] adjoint sweep (device time). */
 int qp3d_fwi_times(const qp3d_fwi *f, double *ms);
 
+/* ---- batched shots ------------------------------------------------------------------------
+ * No reference counterpart (the reference runs its shots one after another, Main.c:179-247).
+ * A batch advances several plans (one per shot: own grid window, model, source, PML profiles
+ * and record; same nt, storage and device) in lockstep with one launch per stage and time step
+ * for all shots. Build it after the plans' uploads; a plan uploaded again needs a new batch.
+ * use_graph: steps run as replays of a captured 10-step graph. The plans' own run/sync are not
+ * used while the batch drives them; records, fields and maxima are read through the plans. */
+typedef struct qp3d_batch qp3d_batch;
+int qp3d_batch_create(qp3d_plan *const *plans, int nshots, int use_graph, qp3d_batch **out);
+int qp3d_batch_destroy(qp3d_batch *b);
+int qp3d_batch_reset(qp3d_batch *b);
+int qp3d_batch_run(qp3d_batch *b, int k0, int k1);
+int qp3d_batch_sync(qp3d_batch *b);
+/* Picks each stage's tile shape by timing every candidate over `steps` steps, then resets. */
+int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, double *ms_out);
+/* bytes[0] gradient-max bytes, bytes[1] update bytes per step (all shots), bytes[2] updated
+ * cells, bytes[3] interior cells. */
+int qp3d_batch_step_bytes(const qp3d_batch *b, double *bytes);
+
 #ifdef __cplusplus
 }
 #endif
