@@ -394,8 +394,8 @@ def main():
     plan.close()
 
     out = {
-        "metric": "Gcell-updates/s (512^3 FP32 forward, 3000 steps)" if args.storage == "fp32" else
-                  f"Gcell-updates/s (512^3 forward, {args.storage.upper()} storage / FP32 arithmetic, 3000 steps)",
+        "metric": f"Gcell-updates/s ({args.n}^3 FP32 forward, {s.nt} steps)" if args.storage == "fp32" else
+                  f"Gcell-updates/s ({args.n}^3 forward, {args.storage.upper()} storage / FP32 arithmetic, {s.nt} steps)",
         "value": round(value, 3),
         "unit": "Gcell-updates/s",
         "n_gpus": 1,
@@ -407,9 +407,9 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (layered VTI + smooth random perturbation, seed 1)",
-        "config": {"workload": "3D pure-qP VTI forward shot, 512^3 interior, pml 32 (576^3 padded), "
-                               "3000 steps, dt 1 ms, dx 12.5 m, 25 Hz Ricker, 512x512 receivers, "
-                               "snapshots every 500 steps",
+        "config": {"workload": f"3D pure-qP VTI forward shot, {args.n}^3 interior, pml {args.pml} "
+                               f"({s.nx}^3 padded), {s.nt} steps, dt 1 ms, dx 12.5 m, 25 Hz Ricker, "
+                               f"{args.n}x{args.n} receivers, snapshots every 500 steps",
                    "padded": [s.nx, s.ny, s.nz], "nt": s.nt, "n_upd": s.n_upd, "n_int": s.n_int,
                    "gcells_int_per_s": round(s.n_int * s.nt * args.steps / wall / 1e9, 3),
                    "ms_per_time_step": round(wall / args.steps / s.nt * 1e3, 5),

@@ -94,13 +94,13 @@ __device__ __forceinline__ void shift_queue(float (&q)[T::RX][QN])
 template <class T, bool PML>
 __global__ __launch_bounds__(T::NT) void k_adj_prod(AdjArgs a)
 {
-  __shared__ float sm[2][T::LX][T::LY];
+  __shared__ float sm[2][T::LX + 1][T::LY];
   const Geo &g = a.g;
   int x0, x1, y0, y1, z0, z1;
   bool fold;
   unpack(a.work[blockIdx.x], x0, x1, y0, y1, z0, z1, fold);
   if (z0 >= z1) return;
-  const Lane<T> L(g, x0, x1, y0, y1, fold);
+  Lane<T> L(g, x0, x1, y0, y1, fold);
   const int y = L.y;
   const int nx = g.nx, ny = g.ny, nz = g.nz, pml = g.pml;
   const int hy = ny - 2 * pml;
@@ -303,13 +303,13 @@ struct ApplyArgs {
 template <class T>
 __global__ __launch_bounds__(T::NT) void k_adj_apply(ApplyArgs a)
 {
-  __shared__ float sm[2][4][T::LX][T::LY];  // Pxx, Qx, Pyy, Qy
+  __shared__ float sm[2][4][T::LX + 1][T::LY];  // Pxx, Qx, Pyy, Qy
   const Geo &g = a.g;
   int x0, x1, y0, y1, z0, z1;
   bool fold;
   unpack(a.work[blockIdx.x], x0, x1, y0, y1, z0, z1, fold);
   if (z0 >= z1) return;
-  const Lane<T> L(g, x0, x1, y0, y1, fold);
+  Lane<T> L(g, x0, x1, y0, y1, fold);
   bool rows_on[T::RX];
 #pragma unroll
   for (int r = 0; r < T::RX; r++) rows_on[r] = L.yon && x0 + L.tx + T::WX * r < x1;

@@ -17,7 +17,7 @@ struct qp3d_batch {
   int nt = 0, storage = 0, device = 0;
   bool use_graph = false;
   hipStream_t stream = nullptr;
-  int tile[3] = {0, 5, 1};  // gradient max, interior update, PML update (QP3D_TILES index)
+  int tile[3] = {2, 0, 5};  // gradient max, interior update, PML update (QP3D_TILES index)
   WorkList wl[3];
   void *args_g[2] = {}, *args_u[2] = {};  // per-parity argument blocks of every shot
   int *kdev = nullptr;
@@ -242,7 +242,7 @@ extern "C" int qp3d_batch_reset(qp3d_batch *b)
   if (!b) return QP3D_ERR_ARG;
   HIP_TRY(hipSetDevice(b->device));
   for (qp3d_plan *p : b->plans) {
-    HIP_TRY(hipMemsetAsync(p->state, 0, p->esz * 9 * (size_t)p->N, b->stream));
+    HIP_TRY(hipMemsetAsync(p->state, 0, p->state_bytes, b->stream));
     HIP_TRY(hipMemsetAsync(p->gmax, 0, sizeof(unsigned) * (size_t)b->nt, b->stream));
     HIP_TRY(hipMemsetAsync(p->flag, 0, sizeof(int), b->stream));
     if (p->rec) HIP_TRY(hipMemsetAsync(p->rec, 0, sizeof(float) * (size_t)b->nt * p->hx * p->hy, b->stream));

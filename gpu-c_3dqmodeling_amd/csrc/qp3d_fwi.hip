@@ -100,7 +100,7 @@ extern "C" int qp3d_fwi_create(qp3d_plan *p, int interval, int ckpt_storage, qp3
     return 0;
   };
   const size_t N = (size_t)p->N;
-  const size_t cb = 9 * N * (ckpt_storage == QP3D_STORAGE_FP32 ? 4 : 2);
+  const size_t cb = p->state_bytes / (ckpt_storage == QP3D_STORAGE_FP32 ? 1 : 2);
   int rc;
   for (int s = 1; s < f->nseg; s++) {  // segment 0 starts from rest
     void *d = nullptr;
@@ -140,9 +140,9 @@ static int grid_of(long n) { return (int)std::min<long>((n + 255) / 256, 8192); 
 static int save_ckpt(qp3d_fwi *f, int s)
 {
   qp3d_plan *p = f->p;
-  const long n = 9 * p->N;
+  const long n = (long)(p->state_bytes / sizeof(float));  // the whole state block, pads included
   if (f->ckpt_storage == QP3D_STORAGE_FP32) {
-    HIP_TRY(hipMemcpyAsync(f->ckpt[s - 1], p->state, sizeof(float) * n, hipMemcpyDeviceToDevice, p->stream));
+    HIP_TRY(hipMemcpyAsync(f->ckpt[s - 1], p->state, p->state_bytes, hipMemcpyDeviceToDevice, p->stream));
   } else {
     hipLaunchKernelGGL(k_pack_bf16, dim3(grid_of(n)), dim3(256), 0, p->stream, (unsigned short *)f->ckpt[s - 1],
                        (const float *)p->state, n);
@@ -154,11 +154,11 @@ static int save_ckpt(qp3d_fwi *f, int s)
 static int load_ckpt(qp3d_fwi *f, int s)
 {
   qp3d_plan *p = f->p;
-  const long n = 9 * p->N;
+  const long n = (long)(p->state_bytes / sizeof(float));
   if (s == 0) {
-    HIP_TRY(hipMemsetAsync(p->state, 0, sizeof(float) * n, p->stream));
+    HIP_TRY(hipMemsetAsync(p->state, 0, p->state_bytes, p->stream));
   } else if (f->ckpt_storage == QP3D_STORAGE_FP32) {
-    HIP_TRY(hipMemcpyAsync(p->state, f->ckpt[s - 1], sizeof(float) * n, hipMemcpyDeviceToDevice, p->stream));
+    HIP_TRY(hipMemcpyAsync(p->state, f->ckpt[s - 1], p->state_bytes, hipMemcpyDeviceToDevice, p->stream));
   } else {
     hipLaunchKernelGGL(k_unpack_bf16, dim3(grid_of(n)), dim3(256), 0, p->stream, (float *)p->state,
                        (const unsigned short *)f->ckpt[s - 1], n);

@@ -124,6 +124,7 @@ struct Coef {
   float ix, iy, iz;          // 1/dx, 1/dy, 1/dz
   float ixx, iyy, izz;       // 1/dx^2, 1/dy^2, 1/dz^2
   float dt;
+  float g1x[H], g1y[H], g1z[H];  // c1 / dx, c1 / dy, c1 / dz (gradient-max pass)
 };
 
 // Time-step index of a launch: *kdev + koff when kdev is set (launches replayed from a
@@ -187,6 +188,8 @@ struct Lane {
   unsigned off[T::RX];   // the same, clamped into the tile, for per-cell fields
   unsigned hxo[T::NXH];  // x-halo byte offsets
   unsigned hyo;          // y-halo byte offset
+  int hyL;               // LDS slot (row * LY + column) of the lane's y-halo value; lanes
+                         // without one write a spare slot past the plane (LX * LY)
   bool hx_on[T::NXH];    // wave-uniform
   bool hy_on;
 
@@ -230,6 +233,20 @@ struct Lane {
     if (!fold) gy = min(y0 - H + (col < H ? col : TY + col), y1 + H - 1);
     else gy = col < H ? y0 - H + col : g.ny - g.pml - H + (col - H);
     hyo = 4u * ((unsigned)min(x0 + row, x1 + H - 1) * g.ny + gy);
+    hyL = hy_on ? (H + row) * T::LY + halo_col(col, fold) : T::LX * T::LY;
+  }
+
+  // Re-issues the 32-bit offsets inside the march (an empty asm with read-write operands, no
+  // instruction). Without it the compiler hoists their zero-extension out of the loop and then
+  // forms every load address in 64-bit vector registers; with it each load keeps the
+  // scalar-base + 32-bit vector-offset form.
+  __device__ __forceinline__ void refresh()
+  {
+#pragma unroll
+    for (int r = 0; r < T::RX; r++) asm volatile("" : "+v"(qoff[r]), "+v"(off[r]));
+#pragma unroll
+    for (int i = 0; i < T::NXH; i++) asm volatile("" : "+v"(hxo[i]));
+    asm volatile("" : "+v"(hyo));
   }
 
   // LDS column of y-halo value `col` (0..9) of this lane's load.
@@ -256,11 +273,12 @@ struct PlaneLoad {
     for (int r = 0; r < T::RX; r++) qn[r] = S::ld(uq, L.qoff[r]);
 #pragma unroll
     for (int i = 0; i < T::NXH; i++)
-      if (L.hx_on[i]) hx[i] = S::ld(uz, L.hxo[i]);
+      if (__builtin_amdgcn_readfirstlane(L.hx_on[i])) hx[i] = S::ld(uz, L.hxo[i]);
     if (L.hy_on) hy = S::ld(uz, L.hyo);
   }
 
-  // Write plane z (centre values from the queue, halos from this load) into an LDS buffer.
+  // Write plane z (centre values from the queue, halos from this load) into an LDS buffer
+  // (LX + 1 rows: the last holds the spare slot).
   __device__ __forceinline__ void stage(float (*sm)[T::LY], const float (&centre)[T::RX], const Lane<T> &L) const
   {
 #pragma unroll
@@ -268,13 +286,9 @@ struct PlaneLoad {
 #pragma unroll
     for (int i = 0; i < T::NXH; i++) {
       const int h = L.tx + T::WX * i;
-      if (L.hx_on[i]) sm[h < H ? h : T::TX + h][L.Y] = hx[i];
+      if (__builtin_amdgcn_readfirstlane(L.hx_on[i])) sm[h < H ? h : T::TX + h][L.Y] = hx[i];
     }
-    if (L.hy_on) {
-      const int t = L.tx * TY + L.ty;
-      const int row = t / (2 * H), col = t - row * (2 * H);
-      sm[H + row][Lane<T>::halo_col(col, L.fold)] = hy;
-    }
+    (&sm[0][0])[L.hyL] = hy;  // lanes without a y-halo value write the spare slot
   }
 };
 
@@ -322,7 +336,7 @@ struct Unroll<N, N> {
 template <class S, class T>
 __device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int kstep)
 {
-  __shared__ float sm[2][T::LX][T::LY];
+  __shared__ float sm[2][T::LX + 1][T::LY];
   __shared__ float red[T::WX];
   __shared__ int redbad[T::WX];
   const Geo &g = a.g;
@@ -330,13 +344,15 @@ __device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int k
   bool fold;
   unpack(w, x0, x1, y0, y1, z0, z1, fold);
   if (z0 >= z1) return;
-  const Lane<T> L(g, x0, x1, y0, y1, fold);
+  Lane<T> L(g, x0, x1, y0, y1, fold);
   const bool ycmp = L.yon;
   bool rows_on[T::RX];
 #pragma unroll
   for (int r = 0; r < T::RX; r++) rows_on[r] = ycmp && x0 + L.tx + T::WX * r < x1;
   float q[T::RX][QN];
   prime_queue<S, T>(q, a.ucur, g, L, z0);
+  // The march is unrolled by 22 (twice the queue length), so the LDS buffer of every plane
+  // (it & 1) is known at compile time.
   PlaneLoad<S, T> P;
   P.issue(a.ucur, g, L, z0);
   float centre[T::RX];
@@ -347,34 +363,43 @@ __device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int k
   }
   P.stage(sm[0], centre, L);
   __syncthreads();
-  // per-axis maxima of |sum_m c1_m (u+m - u-m)|; fmaxf ignores NaN like the reference's
-  // '>=' scan (CPU_function.c:18), and x*0 accumulates NaN for any non-finite gradient.
-  float mx = 0.f, my = 0.f, mz = 0.f, nanacc = 0.f;
-  int b = 0;
+  // max of |sum_m (c1_m/d) (u+m - u-m)| over the three axes; fmaxf ignores NaN like the
+  // reference's '>=' scan (CPU_function.c:18), and x*0 accumulates NaN for any non-finite
+  // gradient.
+  float m = 0.f, nanacc = 0.f;
+#ifdef GRAD_MASKMUL
+  float rmask[T::RX];
+#pragma unroll
+  for (int r = 0; r < T::RX; r++) rmask[r] = rows_on[r] ? 1.f : 0.f;
+#endif
   int zb = z0;
   auto plane = [&](auto itc) {
     constexpr int it = decltype(itc)::value;
     {
       const int z = zb + it;
       const bool more = z + 1 < z1;
+      L.refresh();
       if (more) P.issue(a.ucur, g, L, z + 1);
-      float (*s)[T::LY] = sm[b];
+      float (*s)[T::LY] = sm[it & 1];
 #pragma unroll
       for (int r = 0; r < T::RX; r++) {
         const int X = H + L.tx + T::WX * r, Y = L.Y;
         float ux = 0.f, uy = 0.f, uz = 0.f;
 #pragma unroll
         for (int k = 1; k <= H; k++) {
-          ux = fmaf(a.c.c1[k - 1], s[X + k][Y] - s[X - k][Y], ux);
-          uy = fmaf(a.c.c1[k - 1], s[X][Y + k] - s[X][Y - k], uy);
-          uz = fmaf(a.c.c1[k - 1], q[r][(H + k + it) % QN] - q[r][(H - k + it) % QN], uz);
+          ux = fmaf(a.c.g1x[k - 1], s[X + k][Y] - s[X - k][Y], ux);
+          uy = fmaf(a.c.g1y[k - 1], s[X][Y + k] - s[X][Y - k], uy);
+          uz = fmaf(a.c.g1z[k - 1], q[r][(H + k + it) % QN] - q[r][(H - k + it) % QN], uz);
         }
+#ifdef GRAD_MASKMUL
+        m = fmaxf(m, fmaxf(fabsf(ux), fmaxf(fabsf(uy), fabsf(uz))) * rmask[r]);
+        nanacc = fmaf(ux, 0.f, nanacc);
+#else
         if (rows_on[r]) {
-          mx = fmaxf(mx, fabsf(ux));
-          my = fmaxf(my, fabsf(uy));
-          mz = fmaxf(mz, fabsf(uz));
+          m = fmaxf(m, fmaxf(fabsf(ux), fmaxf(fabsf(uy), fabsf(uz))));
           nanacc = fmaf(ux, 0.f, nanacc);
         }
+#endif
       }
       if (more) {
 #pragma unroll
@@ -382,15 +407,12 @@ __device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int k
           q[r][it % QN] = P.qn[r];
           centre[r] = q[r][(H + 1 + it) % QN];
         }
-        P.stage(sm[b ^ 1], centre, L);
+        P.stage(sm[(it + 1) & 1], centre, L);
         __syncthreads();
-        b ^= 1;
       }
     }
   };
-  for (; zb < z1; zb += QN) Unroll<0, QN>::run(plane, z1 - zb);
-  // scaling by 1/d is monotonic, so max(|g|)/d = max(|g|/d)
-  float m = fmaxf(fmaxf(mx * a.c.ix, my * a.c.iy), mz * a.c.iz);
+  for (; zb < z1; zb += 2 * QN) Unroll<0, 2 * QN>::run(plane, z1 - zb);
   int bad = !(nanacc == 0.f) || !(m <= 3.4e38f);
   // workgroup max -> one atomic per workgroup
 #pragma unroll
@@ -476,13 +498,13 @@ struct CellLoad {
 template <class S, class T, bool PML>
 __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kstep)
 {
-  __shared__ float sm[2][T::LX][T::LY];
+  __shared__ float sm[2][T::LX + 1][T::LY];
   const Geo &g = a.g;
   int x0, x1, y0, y1, z0, z1;
   bool fold;
   unpack(w, x0, x1, y0, y1, z0, z1, fold);
   if (z0 >= z1) return;
-  const Lane<T> L(g, x0, x1, y0, y1, fold);
+  Lane<T> L(g, x0, x1, y0, y1, fold);
   const int y = L.y;
   const bool ycmp = L.yon;
   const int nx = g.nx, ny = g.ny, nz = g.nz, pml = g.pml;
@@ -529,8 +551,11 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
   prime_queue<S, T>(q, a.ucur, g, L, z0);
   PlaneLoad<S, T> P;
   P.issue(a.ucur, g, L, z0);
-  CellLoad<S, T, PML> C, Cn;
-  C.issue(a, L, z0);
+  // per-cell loads, double-buffered: the interior march (unrolled by 22) addresses the buffer
+  // of plane `it` as CL[it & 1] at compile time; the rolled PML march copies the next into
+  // the current
+  CellLoad<S, T, PML> CL[2];
+  CL[0].issue(a, L, z0);
   float centre[T::RX];
 #pragma unroll
   for (int r = 0; r < T::RX; r++) {
@@ -546,11 +571,15 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
     {
       const int z = zb + it;
       const bool more = z + 1 < z1;
+      L.refresh();
+      CellLoad<S, T, PML> &C = CL[PML ? 0 : (it & 1)];
+      CellLoad<S, T, PML> &Cn = CL[PML ? 1 : ((it + 1) & 1)];
       if (more) {
         P.issue(a.ucur, g, L, z + 1);
         Cn.issue(a, L, z + 1);
       }
-      float (*s)[T::LY] = sm[b];
+      // interior: unrolled by 22, the buffer of plane `it` is it & 1 at compile time
+      float (*s)[T::LY] = sm[PML ? b : (it & 1)];
       const bool src_plane = src_tile && z >= a.sb_z0 && z <= a.sb_z1;
       const bool rec_plane = rec_tile && z == a.sz;
       float4 tz;
@@ -638,15 +667,15 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
           q[r][it % QN] = P.qn[r];
           centre[r] = q[r][(H + 1 + it) % QN];
         }
-        P.stage(sm[b ^ 1], centre, L);
-        C = Cn;
+        P.stage(sm[PML ? (b ^ 1) : ((it + 1) & 1)], centre, L);
+        if constexpr (PML) C = Cn;
         __syncthreads();
         b ^= 1;
       }
     }
   };
   if constexpr (!PML) {
-    for (; zb < z1; zb += QN) Unroll<0, QN>::run(plane, z1 - zb);
+    for (; zb < z1; zb += 2 * QN) Unroll<0, 2 * QN>::run(plane, z1 - zb);
   } else {
     // the PML kernel carries seven more state values per row; a rolled loop with a shifted
     // queue keeps it free of spills

@@ -315,9 +315,11 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
   p->N = (long)c.nx * c.ny * c.nz;
   p->geo.nx = c.nx; p->geo.ny = c.ny; p->geo.nz = c.nz; p->geo.pml = c.pml;
   p->geo.pln = (unsigned)c.nx * (unsigned)c.ny;
-  p->tile_u = c.tile_update > 0 ? c.tile_update - 1 : 5;  // 16x2 by default
-  p->tile_g = c.tile_grad > 0 ? c.tile_grad - 1 : 0;      // 8x2 by default
-  p->tile_p = c.tile_pml > 0 ? c.tile_pml - 1 : 1;        // 16x1 by default
+  // defaults measured fastest on MI355X at 576^3 (FP32): 8x2 interior, 4x2 gradient max,
+  // 16x2 PML
+  p->tile_u = c.tile_update > 0 ? c.tile_update - 1 : 0;
+  p->tile_g = c.tile_grad > 0 ? c.tile_grad - 1 : 2;
+  p->tile_p = c.tile_pml > 0 ? c.tile_pml - 1 : 5;
   p->two_streams = c.streams == 2 && !c.use_graph;  // a captured loop runs on one stream
 
   auto fail = [&](int rc) {
@@ -351,7 +353,9 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
   }
   int rc;
   p->esz = c.storage == QP3D_STORAGE_FP32 ? 4 : 2;
-  const size_t fs = p->esz * (size_t)p->N;
+  const size_t fs = field_stride(p->esz * (size_t)p->N);
+  p->fstride = fs;
+  p->state_bytes = 9 * fs;
   if ((rc = dmalloc((void **)&p->state, 9 * fs, "wavefields and PML state"))) return fail(rc);
   char *s = p->state;
   p->u[0] = s; p->u[1] = s + fs;
@@ -359,8 +363,9 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
   p->w3[0] = s + 4 * fs; p->w3[1] = s + 5 * fs;
   p->pp[0] = s + 6 * fs; p->pp[1] = s + 7 * fs;
   p->w21 = s + 8 * fs;
-  if ((rc = dmalloc((void **)&p->model, 3 * fb, "model"))) return fail(rc);
-  p->vdt2 = p->model; p->eps = p->model + p->N; p->epsd = p->model + 2 * p->N;
+  const size_t ms = field_stride(fb) / sizeof(float);  // floats between model fields
+  if ((rc = dmalloc((void **)&p->model, 3 * ms * sizeof(float), "model"))) return fail(rc);
+  p->vdt2 = p->model; p->eps = p->model + ms; p->epsd = p->model + 2 * ms;
   if ((rc = dmalloc((void **)&p->pmltab, sizeof(float4) * 3 * c.pml, "PML table"))) return fail(rc);
   if ((rc = dmalloc((void **)&p->gmax, sizeof(unsigned) * (size_t)(c.nt > 0 ? c.nt : 1), "step maxima")))
     return fail(rc);
@@ -435,7 +440,9 @@ extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps,
     h[N + i] = eps[i];
     h[2 * N + i] = eps[i] - del[i];
   }
-  HIP_TRY(hipMemcpy(p->model, h.data(), sizeof(float) * 3 * (size_t)N, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(p->vdt2, h.data(), sizeof(float) * (size_t)N, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(p->eps, h.data() + N, sizeof(float) * (size_t)N, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(p->epsd, h.data() + 2 * N, sizeof(float) * (size_t)N, hipMemcpyHostToDevice));
   // PML coefficient table per face family (x: dlr, y: dfb, z: dtb) and layer j:
   //   K1 = (1+d dt)^2 - 3, K2 = 2 d dt - 1, K3 = 1 - dt d, E = d'   (GPU_kernel.cu:568-576)
   std::vector<float4> tab(3 * (size_t)cf.pml);
@@ -462,6 +469,11 @@ extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps,
   k.iyy = (float)(1.0 / ((double)cf.dy * cf.dy));
   k.izz = (float)(1.0 / ((double)cf.dz * cf.dz));
   k.s2 = 2.0f * (k.c2[0] + k.c2[1] + k.c2[2] + k.c2[3] + k.c2[4]);
+  for (int m = 0; m < H; m++) {
+    k.g1x[m] = k.c1[m] * k.ix;
+    k.g1y[m] = k.c1[m] * k.iy;
+    k.g1z[m] = k.c1[m] * k.iz;
+  }
   k.dt = cf.dt;
   // source: bounding box of the non-zero cells of the full-grid spatial source
   int z0 = cf.nz, z1 = -1, x0 = cf.nx, x1 = -1, y0 = cf.ny, y1 = -1;
@@ -521,7 +533,7 @@ extern "C" int qp3d_plan_reset(qp3d_plan *p)
     return QP3D_ERR_ARG;
   }
   HIP_TRY(hipSetDevice(p->cfg.device));
-  HIP_TRY(hipMemsetAsync(p->state, 0, p->esz * 9 * (size_t)p->N, p->stream));
+  HIP_TRY(hipMemsetAsync(p->state, 0, p->state_bytes, p->stream));
   HIP_TRY(hipMemsetAsync(p->gmax, 0, sizeof(unsigned) * (size_t)(p->cfg.nt > 0 ? p->cfg.nt : 1), p->stream));
   HIP_TRY(hipMemsetAsync(p->flag, 0, sizeof(int), p->stream));
   if (p->rec) HIP_TRY(hipMemsetAsync(p->rec, 0, sizeof(float) * (size_t)p->cfg.nt * p->hx * p->hy, p->stream));

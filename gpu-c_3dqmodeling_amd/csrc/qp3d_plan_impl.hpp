@@ -52,6 +52,8 @@ struct qp3d_plan {
   // u[2], w1[2], w3[2], p[2], w21: 9 fields of N cells of esz bytes (float, bfloat16 or half)
   char *state = nullptr;
   size_t esz = 4;
+  size_t fstride = 0;      // bytes from one state field to the next (N*esz plus a pad)
+  size_t state_bytes = 0;  // 9 * fstride
   void *u[2]{}, *w1[2]{}, *w3[2]{}, *pp[2]{}, *w21 = nullptr;
   int scale_log2 = 0;  // half storage: fields are stored times 2^scale_log2
   float *model = nullptr;  // vdt2, eps, epsd
@@ -85,6 +87,12 @@ struct qp3d_plan {
   double t_grad = 0, t_int = 0, t_pml = 0, t_upd = 0;
   long n_launch = 0;
 };
+
+// Byte stride between fields that one kernel streams side by side: the size rounded up to 256
+// bytes plus 12.25 KiB, so that the same cell of different fields does not sit at the same
+// offset modulo large powers of two (fields of exactly 2^k-multiple sizes would otherwise start
+// on the same HBM channel and bank together).
+inline size_t field_stride(size_t bytes) { return (bytes + 255) / 256 * 256 + 12544; }
 
 // Interior-x extent (rows) of tile shape t.
 inline int tile_tx(int t)
