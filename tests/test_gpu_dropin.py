@@ -1,0 +1,114 @@
+"""The drop-in driver qp3d_main (Main.c:24-246 on top of the library) against the same shot
+loop run with the CPU oracle: parameter file, model windows read from model files, the direct
+wave, full-wave and muted records appended per shot, the xz / yz slices of the first shot and
+the snapshots, compared file by file (same sizes; FP32 values within relative L2 1e-5)."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+import qp3d
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+MAIN = os.path.join(os.path.dirname(qp3d.LIB_PATH), "qp3d_main")
+
+PAR = """Time sampling rate[dt-ms]--record time length[nt]
+0.001\t{nt}\t\t{ntsnap}
+X-direction distance--y-direction distance--z-direction distance[zmax-m]
+{Nx0}\t  {Ny0}     {Nz0}
+X-direction sampling internal--y-direction sampling internal--z-direction sampling internal[dz-m]
+12.5\t12.5   \t12.5
+Ricker wavelet dominant frequency--source amplitude--source decay coefficient[alp]
+25.0\t1    \t1
+First source position at x-direction[spx-m] y-direction[spy-m]and z-direction[spz-m]
+{sx0}\t\t{sy0}     \t0
+First geophone position at x-direction[gx-m] y-direction[gy-m]and z-direction[gz-m]
+0  \t\t0  \t\t0
+Nshot(x,y)--shot_interval(x,y)
+{nxs}  \t\t{nys}  \t\t{dxs}  \t{dys}
+Offset[m]-the distance of offset in x and y
+{hx}  \t{hy}
+The number of PML absorbing boundary layers[npml]
+{pml}
+Model-parameters-vp-epsilon-delta-filename
+model/vp.dat\tmodel/epsilon.dat\tmodel/delta.dat
+Record-fullwave-directwave-mute-filename
+output/Record_fullwave.dat   \toutput/Record_direct.dat   output/Record_mute.dat
+Record-fullwave-xoz and yoz-slice
+output/recordxz.dat   \toutput/recordyz.dat
+Snapshot-fullwve-directwave-filename
+output/snapshot_direct  \toutput/snapshot_fullwave
+Single-shot or multi-shots (single = 0, multi = 1)
+1
+Mute directwave or not(yes = 1, no = 0)
+1
+Save fullwave wavefield snapshot or not(yes = 1, no = 0)
+1
+Save directwave wavefield snapshot or not(yes = 1, no = 0)
+0
+"""
+
+
+def oracle_run(d, p, model):
+    """The shot loop of Main.c:163-247 with the oracle's modeling3d (homogeneous 2500 m/s
+    direct-wave model, as the driver documents)."""
+    os.makedirs(os.path.join(d, "output"))
+    nx, ny, nz = p["hx"] + 2 * p["pml"], p["hy"] + 2 * p["pml"], p["Nz0"] + 2 * p["pml"]
+    z = np.zeros((p["Nz0"], p["hx"], p["hy"]), np.float32)
+    sd = qp3d.build_setup(z + 2500.0, z, z, p["pml"], p["nt"], 12.5, 12.5, 12.5, 0.001, 25.0)
+    rec_dir = O.modeling3d(sd, "f32")
+    rec_dir.tofile(os.path.join(d, "output/Record_direct.dat"))
+    for ix in range(p["nxs"]):
+        for iy in range(p["nys"]):
+            sx, sy = p["sx0"] + ix * p["dxs"], p["sy0"] + iy * p["dys"]
+            fnx, fny = sx - (p["hx"] - 1) // 2, sy - (p["hy"] - 1) // 2
+            win = [qp3d.file_to_grid(np.ascontiguousarray(m[fny:fny + p["hy"], fnx:fnx + p["hx"], :])) for m in model]
+            s = qp3d.build_setup(*win, p["pml"], p["nt"], 12.5, 12.5, 12.5, 0.001, 25.0)
+            full = O.modeling3d(s, "f32", snapflag=1, ntsnap=p["ntsnap"],
+                                snap_file=os.path.join(d, "output/snapshot_fullwave"))
+            with open(os.path.join(d, "output/Record_fullwave.dat"), "ab") as f:
+                full.tofile(f)
+            with open(os.path.join(d, "output/Record_mute.dat"), "ab") as f:
+                (full - rec_dir).tofile(f)
+            if ix == 0 and iy == 0:
+                sx1, sy1 = (p["hx"] - 1) // 2, (p["hy"] - 1) // 2
+                np.ascontiguousarray(full[:, :, sy1]).tofile(os.path.join(d, "output/recordxz.dat"))
+                np.ascontiguousarray(full[:, sx1, :]).tofile(os.path.join(d, "output/recordyz.dat"))
+
+
+def test_qp3d_main_matches_the_oracle_driver(tmp_path):
+    assert os.path.exists(MAIN), "qp3d_main is not built"
+    p = dict(nt=80, ntsnap=40, Nx0=40, Ny0=36, Nz0=24, sx0=14, sy0=15, nxs=2, nys=1, dxs=8, dys=6, hx=21, hy=19,
+             pml=10)
+    model = qp3d.genmodel(p["Nx0"], p["Ny0"], p["Nz0"], 5)  # file layout [y][x][z]
+    g, c = tmp_path / "gpu", tmp_path / "cpu"
+    os.makedirs(g / "model")
+    os.makedirs(g / "output")
+    for name, a in zip(("vp", "epsilon", "delta"), model):
+        a.tofile(g / "model" / f"{name}.dat")
+    (g / "parameter.txt").write_text(PAR.format(**p))
+    print("running qp3d_main", flush=True)
+    r = subprocess.run([MAIN, "parameter.txt"], cwd=g, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-500:], r.stderr[-500:], flush=True)
+    assert r.returncode == 0
+    print("running the oracle driver", flush=True)
+    oracle_run(str(c), p, model)
+    names = sorted(os.listdir(c / "output"))
+    assert sorted(os.listdir(g / "output")) == names
+    assert "snapshot_fullwave40.dat" in names and "snapshot_fullwave_xz0.dat" in names
+    for n in names:
+        a = np.fromfile(g / "output" / n, np.float32)
+        b = np.fromfile(c / "output" / n, np.float32)
+        assert a.size == b.size, n
+        if np.any(b):
+            assert rel_l2(a, b) <= TOL, n
+        else:
+            assert not np.any(a), n
+    R = p["nt"] * p["hx"] * p["hy"]
+    assert os.path.getsize(g / "output/Record_fullwave.dat") == 4 * R * p["nxs"] * p["nys"]
