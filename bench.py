@@ -136,7 +136,7 @@ def bench_fwi(args):
         runs[key] = r
         log(f"{key}: {r}")
         del g
-    fd = fd_study(qp3d, args.fd_dirs)
+    fd = fd_study(qp3d, args.fd_dirs) if args.fd_dirs > 0 else None
     out = {"metric": "FWI gradient s/shot (256^3, 2000 steps, checkpoints every 50)",
            "value": runs["ckpt50_fp32"]["s_per_shot"], "unit": "s/shot", "n_gpus": 1,
            "higher_is_better": False, "dtype": "fp32 (FP64 gradient accumulation)", "data": "synthetic",

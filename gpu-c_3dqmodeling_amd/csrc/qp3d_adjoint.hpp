@@ -91,6 +91,39 @@ __device__ __forceinline__ void shift_queue(float (&q)[T::RX][QN])
     for (int j = 0; j < 2 * H; j++) q[r][j] = q[r][j + 1];
 }
 
+// Per-cell inputs of k_adj_prod for one plane, issued a plane ahead of their use: the model,
+// the adjoint of u (newest level; current level for interior cells), the PML adjoint states
+// and the gradient sums.
+template <class T, bool PML>
+struct AdjCellLoad {
+  static constexpr int NS = PML ? 7 : 0;
+  float v2[T::RX], e[T::RX], ed[T::RX], an[T::RX], ac[T::RX];
+  float st[T::RX][PML ? 7 : 1];  // o1n o1c pin pic o3n o3c o21
+  float gs[T::RX][3];            // gv2 ge ged
+
+  __device__ __forceinline__ void issue(const AdjArgs &a, const Lane<T> &L, int z)
+  {
+    const size_t zo = (size_t)z * a.g.pln;
+#pragma unroll
+    for (int r = 0; r < T::RX; r++) {
+      const unsigned o = L.off[r];
+      v2[r] = *bat(a.vdt2 + zo, o);
+      e[r] = *bat(a.eps + zo, o);
+      ed[r] = *bat(a.epsd + zo, o);
+      an[r] = *bat(a.an + zo, o);
+      if constexpr (!PML) ac[r] = *bat(a.ac + zo, o);
+      if constexpr (PML) {
+        const float *src[7] = {a.o1n, a.o1c, a.pin, a.pic, a.o3n, a.o3c, a.o21};
+#pragma unroll
+        for (int j = 0; j < 7; j++) st[r][j] = *bat(src[j] + zo, o);
+      }
+      gs[r][0] = *bat(a.gv2 + zo, o);
+      gs[r][1] = *bat(a.ge + zo, o);
+      gs[r][2] = *bat(a.ged + zo, o);
+    }
+  }
+};
+
 template <class T, bool PML>
 __global__ __launch_bounds__(T::NT) void k_adj_prod(AdjArgs a)
 {
@@ -144,11 +177,17 @@ __global__ __launch_bounds__(T::NT) void k_adj_prod(AdjArgs a)
     centre[r] = q[r][H];
   }
   P.stage(sm[0], centre, L);
+  AdjCellLoad<T, PML> C, Cn;
+  C.issue(a, L, z0);
   __syncthreads();
   int b = 0;
   for (int z = z0; z < z1; z++) {
     const bool more = z + 1 < z1;
-    if (more) P.issue(a.u, g, L, z + 1);
+    L.refresh();
+    if (more) {
+      P.issue(a.u, g, L, z + 1);
+      Cn.issue(a, L, z + 1);
+    }
     float (*s)[T::LY] = sm[b];
     float4 tz = make_float4(0.f, 0.f, 0.f, 0.f);
     if constexpr (PML) {
@@ -180,7 +219,7 @@ __global__ __launch_bounds__(T::NT) void k_adj_prod(AdjArgs a)
       const float Uzz = fmaf(-a.c.s2, c0, Lzz) * a.c.izz;
       ux *= a.c.ix; uy *= a.c.iy; uz *= a.c.iz;
       const unsigned o = L.off[r];
-      const float e = *bat(a.eps + zo, o), ed = *bat(a.epsd + zo, o), v2 = *bat(a.vdt2 + zo, o);
+      const float e = C.e[r], ed = C.ed[r], v2 = C.v2[r];
 
       // S and its derivatives (the forward's arithmetic for S itself)
       float Sv = 1.f, dSde = 0.f, dSded = 0.f, dSdqa = 0.f, dSdb = 0.f;
@@ -206,7 +245,7 @@ __global__ __launch_bounds__(T::NT) void k_adj_prod(AdjArgs a)
         }
       }
       const float e1 = fmaf(2.f, e, 1.f);
-      float mu = at(a.an, zo, o);
+      float mu = C.an[r];
       if (rec_plane && x >= pml && x < nx - pml && y >= pml && y < ny - pml)
         mu += a.resid[(x - pml) * hy + (y - pml)];
 
@@ -218,21 +257,21 @@ __global__ __launch_bounds__(T::NT) void k_adj_prod(AdjArgs a)
         dS = mu * v2 * Pt;
         gv2 = mu * Sv * Pt;
         ge = 2.f * w * (Uxx + Uyy);
-        at(a.ac, zo, o) += 2.f * mu;
+        at(a.ac, zo, o) = C.ac[r] + 2.f * mu;
         at(a.an, zo, o) = -mu;
       } else {
         const float4 t = fam == 2 ? tz : trow[r];
         const float B = Sv * v2;
-        const float w1b = at(a.o1n, zo, o) + mu;
-        const float w21b = at(a.o21, zo, o) + mu;
-        const float w3b = at(a.o3n, zo, o) + mu;
-        const float pb = fmaf(a.c.dt, w21b, at(a.pin, zo, o));
-        at(a.o1c, zo, o) += -t.x * w1b;
+        const float w1b = C.st[r][0] + mu;
+        const float w21b = C.st[r][6] + mu;
+        const float w3b = C.st[r][4] + mu;
+        const float pb = fmaf(a.c.dt, w21b, C.st[r][2]);
+        at(a.o1c, zo, o) = fmaf(-t.x, w1b, C.st[r][1]);
         at(a.o1n, zo, o) = t.y * w1b;
-        at(a.pic, zo, o) += -t.x * pb;
+        at(a.pic, zo, o) = fmaf(-t.x, pb, C.st[r][3]);
         at(a.pin, zo, o) = t.y * pb;
         at(a.o21, zo, o) = t.z * w21b;
-        at(a.o3c, zo, o) += 2.f * w3b;
+        at(a.o3c, zo, o) = fmaf(2.f, w3b, C.st[r][5]);
         at(a.o3n, zo, o) = -w3b;
         at(a.an, zo, o) = 0.f;  // u_{k-1} is not read by PML cells
         float dCn, dB, de1;
@@ -271,11 +310,12 @@ __global__ __launch_bounds__(T::NT) void k_adj_prod(AdjArgs a)
       }
       at(a.pxx, zo, o) = Pxx; at(a.pyy, zo, o) = Pyy; at(a.pzz, zo, o) = Pzz;
       at(a.qx, zo, o) = Qx; at(a.qy, zo, o) = Qy; at(a.qz, zo, o) = Qz;
-      at(a.gv2, zo, o) += gv2;
-      at(a.ge, zo, o) += ge;
-      at(a.ged, zo, o) += ged;
+      at(a.gv2, zo, o) = C.gs[r][0] + gv2;
+      at(a.ge, zo, o) = C.gs[r][1] + ge;
+      at(a.ged, zo, o) = C.gs[r][2] + ged;
     }
     if (more) {
+      C = Cn;
       shift_queue<T>(q);
 #pragma unroll
       for (int r = 0; r < T::RX; r++) {
@@ -341,6 +381,7 @@ __global__ __launch_bounds__(T::NT) void k_adj_apply(ApplyArgs a)
   int b = 0;
   for (int z = z0; z < z1; z++) {
     const bool more = z + 1 < z1;
+    L.refresh();
     if (more) issue(z + 1);
     const size_t zo = (size_t)z * g.pln;
 #pragma unroll
