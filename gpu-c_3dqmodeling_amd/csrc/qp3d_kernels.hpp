@@ -367,11 +367,6 @@ __device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int k
   // reference's '>=' scan (CPU_function.c:18), and x*0 accumulates NaN for any non-finite
   // gradient.
   float m = 0.f, nanacc = 0.f;
-#ifdef GRAD_MASKMUL
-  float rmask[T::RX];
-#pragma unroll
-  for (int r = 0; r < T::RX; r++) rmask[r] = rows_on[r] ? 1.f : 0.f;
-#endif
   int zb = z0;
   auto plane = [&](auto itc) {
     constexpr int it = decltype(itc)::value;
@@ -391,15 +386,10 @@ __device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int k
           uy = fmaf(a.c.g1y[k - 1], s[X][Y + k] - s[X][Y - k], uy);
           uz = fmaf(a.c.g1z[k - 1], q[r][(H + k + it) % QN] - q[r][(H - k + it) % QN], uz);
         }
-#ifdef GRAD_MASKMUL
-        m = fmaxf(m, fmaxf(fabsf(ux), fmaxf(fabsf(uy), fabsf(uz))) * rmask[r]);
-        nanacc = fmaf(ux, 0.f, nanacc);
-#else
         if (rows_on[r]) {
           m = fmaxf(m, fmaxf(fabsf(ux), fmaxf(fabsf(uy), fabsf(uz))));
           nanacc = fmaf(ux, 0.f, nanacc);
         }
-#endif
       }
       if (more) {
 #pragma unroll
@@ -462,6 +452,7 @@ __global__ __launch_bounds__(T::NT) void k_gradmax_batch(const GradArgs<S> *A, c
   gradmax_tile<S, T>(a, w, st.get());
 }
 
+
 // Per-cell inputs of k_update other than u_cur: model, and either u_prev (interior tiles) or
 // the seven PML state values (PML tiles).
 template <class S, class T, bool PML>
@@ -489,7 +480,7 @@ struct CellLoad {
 #pragma unroll
       for (int j = 0; j < 7; j++)
 #pragma unroll
-        for (int r = 0; r < T::RX; r++) st[r][j] = S::ld_stream(src[j], L.off[r]);
+        for (int r = 0; r < T::RX; r++) st[r][j] = S::ld(src[j], L.off[r]);  // cached: 6% faster than non-temporal
     }
   }
 };
