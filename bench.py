@@ -247,10 +247,10 @@ def bench_batch(args):
     nt = 2500 if args.nt == 3000 else args.nt
     rng = np.random.default_rng(3)
     t = time.perf_counter()
-    model = qp3d.genmodel(800, 800, 300, 3)
+    model = qp3d.genmodel(800, 800, args.depth, 3)
     setups = []
     for i in range(args.shots):
-        wx, wy = (int(v) for v in rng.integers(160, 321, size=2))
+        wx, wy = (int(v) for v in rng.integers(args.win[0], args.win[1] + 1, size=2))
         sx, sy = (int(v) for v in rng.integers(0, 800, size=2))
         x0 = min(max(sx - wx // 2, 0), 800 - wx)
         y0 = min(max(sy - wy // 2, 0), 800 - wy)
@@ -284,11 +284,12 @@ def bench_batch(args):
     for p in plans:
         p.close()
     step_bytes = kb["gradmax"] + kb["update"]
-    out = {"metric": f"Gcell-updates/s ({args.shots} shots batched, 2500 steps)", "value": round(cells / wall / 1e9, 3),
+    out = {"metric": f"Gcell-updates/s ({args.shots} shots batched, {nt} steps)", "value": round(cells / wall / 1e9, 3),
            "unit": "Gcell-updates/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(wall * 1e3, 1),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
            "data": "synthetic (layered VTI + smooth random perturbation, seed 3)",
-           "config": {"workload": f"{args.shots} shots, windows 160-320 x 160-320 x 300 of an 800x800x300 model, "
+           "config": {"workload": f"{args.shots} shots, windows {args.win[0]}-{args.win[1]} x {args.win[0]}-{args.win[1]} "
+                                  f"x {args.depth} of an 800x800x{args.depth} model, "
                                   f"pml {args.pml}, {nt} steps, 20 Hz, storage {args.storage}",
                       "cells_updated_per_step": kb["cells_updated"], "autotuned_tiles": tune["tiles"],
                       "autotune_ms_per_step": [round(x, 4) for x in tune["ms"]],
@@ -313,12 +314,15 @@ def main():
     ap.add_argument("--tile-update", type=int, default=0)
     ap.add_argument("--tile-grad", type=int, default=0)
     ap.add_argument("--tile-pml", type=int, default=0)
+    ap.add_argument("--tile-pml-y", type=int, default=0)
     ap.add_argument("--streams", type=int, default=0, help="2: interior and PML updates concurrent; 1: serial")
     ap.add_argument("--storage", default="fp32", choices=["fp32", "bf16", "fp16"],
                     help="storage of wavefields and PML state (arithmetic is always FP32)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run a 1 GiB device-to-device copy (the reference kernel for scaling HBM counters)")
     ap.add_argument("--shots", type=int, default=32, help="batch: number of shots")
+    ap.add_argument("--win", type=int, nargs=2, default=[160, 320], help="batch: window size range in x and y")
+    ap.add_argument("--depth", type=int, default=300, help="batch: model depth (cells)")
     ap.add_argument("--workload", default="forward", choices=["forward", "fwi", "precision", "batch"],
                     help="forward: the 512^3 shot (default); fwi: the 256^3 adjoint gradient; "
                          "precision: FP32/BF16/FP16 storage against the FP64 oracle on a 256^3 crop")
@@ -345,7 +349,8 @@ def main():
     log(f"setup {s.nx}x{s.ny}x{s.nz} in {time.perf_counter() - t:.1f} s")
     ntsnap = 500
     plan = qp3d.Plan(s, snapflag=1, ntsnap=ntsnap, use_graph=bool(args.graph), tile_update=args.tile_update,
-                     tile_grad=args.tile_grad, tile_pml=args.tile_pml, streams=args.streams, storage=args.storage)
+                     tile_grad=args.tile_grad, tile_pml=args.tile_pml, tile_pml_y=args.tile_pml_y,
+                     streams=args.streams, storage=args.storage)
     log("model uploaded")
     for i in range(args.warmup):
         t = time.perf_counter()

@@ -89,7 +89,7 @@ static int build_stage(qp3d_batch *b, int stage)
     if (stage == 0)
       qp3d_items_uniform(b->plans[i], b->tile[0], cap, it);
     else
-      qp3d_items_update(b->plans[i], b->tile[1], b->tile[2], cap, cap, it, ip);
+      qp3d_items_update(b->plans[i], b->tile[1], b->tile[2], cap, cap, it, ip, nullptr, 0, 0);
     const std::vector<int4> &v = stage == 2 ? ip : it;
     for (int4 w : v) all.push_back(make_int4(w.x, w.y, w.z | (w.w << 16), i));
   }

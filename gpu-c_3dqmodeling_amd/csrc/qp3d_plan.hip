@@ -164,7 +164,7 @@ int qp3d_build_uniform_list(const qp3d_plan *p, int tile, int occ, WorkList &w)
 // Update tiles aligned to the PML boundary in x and y, z split at the PML planes: interior
 // tiles (tile shape tile_i) hold only interior cells, PML tiles (tile_p) one face family each.
 void qp3d_items_update(const qp3d_plan *p, int tile_i, int tile_p, int cap_i, int cap_p, std::vector<int4> &it,
-                       std::vector<int4> &ip)
+                       std::vector<int4> &ip, std::vector<int4> *ipy, int tile_py, int cap_py)
 {
   const qp3d_config &c = p->cfg;
   const int pml = c.pml;
@@ -177,10 +177,12 @@ void qp3d_items_update(const qp3d_plan *p, int tile_i, int tile_p, int cap_i, in
     }
   };
   // interior kernel: its own x tiling of the interior; PML kernel: x tiling with its tile width
-  std::vector<Seg> xu, xs, ys;
-  std::vector<bool> xui, xi, yi;
+  // (the y faces with their own when they go to a launch of their own)
+  std::vector<Seg> xu, xs, xsy, ys;
+  std::vector<bool> xui, xi, xiy, yi;
   segs(xu, xui, c.nx, pml, tile_tx(tile_i));
   segs(xs, xi, c.nx, pml, tile_tx(tile_p));
+  segs(xsy, xiy, c.nx, pml, tile_tx(ipy ? tile_py : tile_p));
   segs(ys, yi, c.ny, pml, TY);
   int nint_tiles = 0;
   for (size_t a = 0; a < xu.size(); a++)
@@ -188,21 +190,38 @@ void qp3d_items_update(const qp3d_plan *p, int tile_i, int tile_p, int cap_i, in
   // The two y faces of an x-interior tile column share one folded tile when each face fits
   // half a wavefront with its 5-cell halo.
   const bool fold = pml <= 32;
-  std::vector<std::pair<int4, bool>> ptiles;  // PML xy tiles over the whole z range
-  for (size_t a = 0; a < xs.size(); a++) {
-    if (!xi[a]) {
+  std::vector<std::pair<int4, bool>> ptiles, ytiles;  // PML xy tiles over the whole z range
+  for (size_t a = 0; a < xs.size(); a++)
+    if (!xi[a])
       for (size_t b = 0; b < ys.size(); b++)
         ptiles.push_back({pack(xs[a].a, xs[a].b, ys[b].a, ys[b].b, 0, 0), false});
-    } else if (fold) {
-      ptiles.push_back({pack(xs[a].a, xs[a].b, H, pml, 0, 0), true});
+  for (size_t a = 0; a < xsy.size(); a++) {
+    if (!xiy[a]) continue;
+    if (fold) {
+      ytiles.push_back({pack(xsy[a].a, xsy[a].b, H, pml, 0, 0), true});
     } else {
       for (size_t b = 0; b < ys.size(); b++)
-        if (!yi[b]) ptiles.push_back({pack(xs[a].a, xs[a].b, ys[b].a, ys[b].b, 0, 0), false});
+        if (!yi[b]) ytiles.push_back({pack(xsy[a].a, xsy[a].b, ys[b].a, ys[b].b, 0, 0), false});
     }
+  }
+  if (!ipy) {
+    ptiles.insert(ptiles.end(), ytiles.begin(), ytiles.end());
+    ytiles.clear();
   }
   const int zmid = c.nz - 2 * pml, zpml = pml - H;
   const int zc_i = choose_zchunk(nint_tiles, zmid, cap_i, c.zchunk);
   const int zc_p = choose_zchunk((int)ptiles.size(), c.nz - 2 * H, cap_p, c.zchunk);
+  if (ipy && !ytiles.empty()) {
+    const int zc_y = choose_zchunk((int)ytiles.size(), c.nz - 2 * H, cap_py, c.zchunk);
+    for (int z = H; z < c.nz - H; z += zc_y)
+      for (const auto &t : ytiles) {
+        int4 w = t.first;
+        w.z = z;
+        w.w = std::min(z + zc_y, c.nz - H);
+        if (t.second) w.y = (int)((unsigned)w.y | 0x80000000u);
+        ipy->push_back(w);
+      }
+  }
   for (int z = pml; z < c.nz - pml; z += zc_i)
     for (size_t a = 0; a < xu.size(); a++)
       for (size_t b = 0; b < ys.size(); b++)
@@ -231,7 +250,7 @@ int qp3d_build_update_lists(const qp3d_plan *p, int tile_i, int tile_p, int occ_
 {
   std::vector<int4> it, ip;
   const int ncu = qp3d_cu_count(p);
-  qp3d_items_update(p, tile_i, tile_p, occ_i * ncu, occ_p * ncu, it, ip);
+  qp3d_items_update(p, tile_i, tile_p, occ_i * ncu, occ_p * ncu, it, ip, nullptr, 0, 0);
   int rc = upload_list(wi, it);
   if (rc) return rc;
   return upload_list(wp, ip);
@@ -239,7 +258,7 @@ int qp3d_build_update_lists(const qp3d_plan *p, int tile_i, int tile_p, int occ_
 
 static int build_work(qp3d_plan *p)
 {
-  int occ_g = 1, occ_i = 1, occ_p = 1;
+  int occ_g = 1, occ_i = 1, occ_p = 1, occ_py = 1;
   with_storage(p->cfg.storage, [&](auto sp) {
     using S = decltype(sp);
     occ_g = occupancy_of(p->tile_g, [](auto t) { return reinterpret_cast<const void *>(&k_gradmax<S, decltype(t)>); });
@@ -247,13 +266,21 @@ static int build_work(qp3d_plan *p)
                          [](auto t) { return reinterpret_cast<const void *>(&k_update<S, decltype(t), false>); });
     occ_p = occupancy_of(p->tile_p,
                          [](auto t) { return reinterpret_cast<const void *>(&k_update<S, decltype(t), true>); });
+    occ_py = occupancy_of(p->tile_py,
+                          [](auto t) { return reinterpret_cast<const void *>(&k_update<S, decltype(t), true>); });
   });
   int rc = qp3d_build_uniform_list(p, p->tile_g, occ_g, p->wl_grad);
   if (rc) return rc;
-  rc = qp3d_build_update_lists(p, p->tile_u, p->tile_p, occ_i, occ_p, p->wl_int, p->wl_pml);
-  if (rc) return rc;
+  // The PML update is two launches: x and z faces, then the y faces (folded tiles fill half a
+  // wavefront per face and run best with a narrower tile; measured at 576^3).
+  std::vector<int4> it, ip, ipy;
+  const int ncu = qp3d_cu_count(p);
+  qp3d_items_update(p, p->tile_u, p->tile_p, occ_i * ncu, occ_p * ncu, it, ip, &ipy, p->tile_py, occ_py * ncu);
+  if ((rc = upload_list(p->wl_int, it))) return rc;
+  if ((rc = upload_list(p->wl_pml, ip))) return rc;
+  if ((rc = upload_list(p->wl_pmly, ipy))) return rc;
   p->n_int_cells = p->wl_int.cells;
-  p->n_pml_cells = p->wl_pml.cells;
+  p->n_pml_cells = p->wl_pml.cells + p->wl_pmly.cells;
   return QP3D_OK;
 }
 
@@ -286,7 +313,7 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
     return QP3D_ERR_ARG;
   }
   if (c.tile_update < 0 || c.tile_update > NTILES || c.tile_grad < 0 || c.tile_grad > NTILES || c.tile_pml < 0 ||
-      c.tile_pml > NTILES || c.streams < 0 || c.streams > 2) {
+      c.tile_pml > NTILES || c.tile_pml_y < 0 || c.tile_pml_y > NTILES || c.streams < 0 || c.streams > 2) {
     qp3d_set_error("plan_create: tile shapes must be in [0,%d] and streams in [0,2]", NTILES);
     return QP3D_ERR_ARG;
   }
@@ -320,6 +347,7 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
   p->tile_u = c.tile_update > 0 ? c.tile_update - 1 : 0;
   p->tile_g = c.tile_grad > 0 ? c.tile_grad - 1 : 2;
   p->tile_p = c.tile_pml > 0 ? c.tile_pml - 1 : 5;
+  p->tile_py = c.tile_pml_y > 0 ? c.tile_pml_y - 1 : 1;  // 16x1
   p->two_streams = c.streams == 2 && !c.use_graph;  // a captured loop runs on one stream
 
   auto fail = [&](int rc) {
@@ -408,6 +436,7 @@ extern "C" int qp3d_plan_destroy(qp3d_plan *p)
   (void)hipFree(p->wl_grad.d);
   (void)hipFree(p->wl_int.d);
   (void)hipFree(p->wl_pml.d);
+  (void)hipFree(p->wl_pmly.d);
   for (float *d : p->snaps) (void)hipFree(d);
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
   if (p->ev_grad) (void)hipEventDestroy(p->ev_grad);
@@ -608,6 +637,11 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare)
   if (p->wl_pml.n) {
     ua.work = p->wl_pml.d;
     launch_update<S, true>(p->tile_p, p->wl_pml, ps, ua);
+    HIP_TRY(hipGetLastError());
+  }
+  if (p->wl_pmly.n) {
+    ua.work = p->wl_pmly.d;
+    launch_update<S, true>(p->tile_py, p->wl_pmly, ps, ua);
     HIP_TRY(hipGetLastError());
   }
   if (e) {

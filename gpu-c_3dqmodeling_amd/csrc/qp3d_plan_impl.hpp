@@ -45,7 +45,8 @@ struct qp3d_plan {
   bool two_streams = false;
   hipStream_t stream2 = nullptr;                          // PML updates when two_streams
   hipEvent_t ev_grad = nullptr, ev_pml = nullptr;         // cross-stream ordering, no timing
-  WorkList wl_grad, wl_int, wl_pml;
+  WorkList wl_grad, wl_int, wl_pml, wl_pmly;  // wl_pml: x and z faces; wl_pmly: y faces
+  int tile_py = 1;
   long n_int_cells = 0, n_pml_cells = 0;
   hipStream_t stream = nullptr;
 
@@ -159,8 +160,10 @@ int qp3d_build_update_lists(const qp3d_plan *p, int tile_i, int tile_p, int occ_
 int qp3d_build_uniform_list(const qp3d_plan *p, int tile, int occ, WorkList &w);
 // The same items in host memory (logical order) for `capacity` resident workgroups.
 void qp3d_items_uniform(const qp3d_plan *p, int tile, int capacity, std::vector<int4> &items);
+// ipy == nullptr: the y-face PML tiles join ip; otherwise they go to *ipy with tile shape
+// tile_py (their own launch, filling cap_py workgroups).
 void qp3d_items_update(const qp3d_plan *p, int tile_i, int tile_p, int cap_i, int cap_p, std::vector<int4> &it,
-                       std::vector<int4> &ip);
+                       std::vector<int4> &ip, std::vector<int4> *ipy, int tile_py, int cap_py);
 // Uploads items in XCD-aware order (batch: items hold z0 | z1 << 16 and a grid index).
 int qp3d_upload_items(WorkList &w, const std::vector<int4> &logical, bool batch);
 int qp3d_cu_count(const qp3d_plan *p);

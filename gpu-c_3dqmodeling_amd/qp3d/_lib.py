@@ -38,6 +38,7 @@ class Config(C.Structure):
         ("sz", C.c_int), ("storage", C.c_int), ("use_graph", C.c_int), ("snapflag", C.c_int),
         ("ntsnap", C.c_int), ("zchunk", C.c_int), ("device", C.c_int),
         ("tile_update", C.c_int), ("tile_grad", C.c_int), ("tile_pml", C.c_int), ("streams", C.c_int),
+        ("tile_pml_y", C.c_int),
     ]
 
 

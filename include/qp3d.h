@@ -97,9 +97,11 @@ typedef struct qp3d_config {
   int device;    /* HIP device ordinal */
   int tile_update; /* tile shape of the interior update kernel, 1..QP3D_NTILES, 0 = default */
   int tile_grad;   /* tile shape of the gradient-max kernel, 1..QP3D_NTILES, 0 = default */
-  int tile_pml;    /* tile shape of the PML update kernel, 1..QP3D_NTILES, 0 = default */
+  int tile_pml;    /* tile shape of the PML update on the x and z faces, 1..QP3D_NTILES, 0 = default */
   int streams;     /* 2: run the interior and PML updates concurrently on two streams; 1: one stream;
                       0 = default (1: on MI355X the two compete for HBM and gain nothing) */
+  int tile_pml_y;  /* tile shape of the PML update on the y faces (a launch of its own),
+                      1..QP3D_NTILES, 0 = default */
 } qp3d_config;
 
 /* Tile shapes (wavefronts along x x rows per lane; a tile is 64 cells along y):
