@@ -17,8 +17,9 @@ struct qp3d_batch {
   int nt = 0, storage = 0, device = 0;
   bool use_graph = false;
   hipStream_t stream = nullptr;
-  int tile[3] = {2, 0, 5};  // gradient max, interior update, PML update (QP3D_TILES index)
-  WorkList wl[3];
+  // gradient max, interior update, PML update on x and z faces, on y faces (QP3D_TILES index)
+  int tile[4] = {2, 0, 5, 1};
+  WorkList wl[4];
   void *args_g[2] = {}, *args_u[2] = {};  // per-parity argument blocks of every shot
   int *kdev = nullptr;
   hipGraph_t graph = nullptr;
@@ -85,12 +86,12 @@ static int build_stage(qp3d_batch *b, int stage)
   const int cap = std::max(1, occ * qp3d_cu_count(b->plans[0]) / n);  // the shots share the machine
   std::vector<int4> all;
   for (int i = 0; i < n; i++) {
-    std::vector<int4> it, ip;
+    std::vector<int4> it, ip, ipy;
     if (stage == 0)
       qp3d_items_uniform(b->plans[i], b->tile[0], cap, it);
     else
-      qp3d_items_update(b->plans[i], b->tile[1], b->tile[2], cap, cap, it, ip, nullptr, 0, 0);
-    const std::vector<int4> &v = stage == 2 ? ip : it;
+      qp3d_items_update(b->plans[i], b->tile[1], b->tile[2], cap, cap, it, ip, &ipy, b->tile[3], cap);
+    const std::vector<int4> &v = stage == 3 ? ipy : stage == 2 ? ip : it;
     for (int4 w : v) all.push_back(make_int4(w.x, w.y, w.z | (w.w << 16), i));
   }
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
@@ -121,7 +122,7 @@ static int batch_step_s(qp3d_batch *b, int k, Step st)
 {
   using T = typename S::T;
   const int par = k & 1;
-  for (int s = 0; s < 3; s++) {
+  for (int s = 0; s < 4; s++) {
     const bool tm = b->timing && s == b->timed_stage;
     if (tm) HIP_TRY(hipEventRecord(b->ev[0], b->stream));
     if (b->wl[s].n) {
@@ -130,7 +131,7 @@ static int batch_step_s(qp3d_batch *b, int k, Step st)
       else if (s == 1)
         launch_upd_b<S, false>(b->tile[1], b->wl[1], b->stream, (const UpdArgs<S> *)b->args_u[par], st);
       else
-        launch_upd_b<S, true>(b->tile[2], b->wl[2], b->stream, (const UpdArgs<S> *)b->args_u[par], st);
+        launch_upd_b<S, true>(b->tile[s], b->wl[s], b->stream, (const UpdArgs<S> *)b->args_u[par], st);
       HIP_TRY(hipGetLastError());
     }
     if (tm) {
@@ -219,7 +220,7 @@ extern "C" int qp3d_batch_create(qp3d_plan *const *plans, int nshots, int use_gr
   int rc = QP3D_OK;
   with_storage(b->storage, [&](auto sp) { rc = upload_args<decltype(sp)>(b); });
   if (rc) return fail(rc);
-  for (int s = 0; s < 3; s++)
+  for (int s = 0; s < 4; s++)
     if ((rc = build_stage(b, s))) return fail(rc);
   // the plans' own streams may still hold uploads
   for (qp3d_plan *p : b->plans) HIP_TRY(hipStreamSynchronize(p->stream));
@@ -328,8 +329,9 @@ extern "C" int qp3d_batch_sync(qp3d_batch *b)
 }
 
 // Times every tile shape for each stage over `steps` time steps from rest, keeps the fastest,
-// and resets the batch. tiles_out (3 ints, may be NULL) receives the choice (1-based, as in
-// qp3d_config.tile_*); ms_out (3 doubles, may be NULL) the chosen stages' mean ms per step.
+// and resets the batch. tiles_out (4 ints, may be NULL) receives the choice (1-based, as in
+// qp3d_config.tile_*: gradient max, interior, PML x/z faces, PML y faces); ms_out (4 doubles,
+// may be NULL) the chosen stages' mean ms per step.
 extern "C" int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, double *ms_out)
 {
   if (!b || steps < 1 || steps + 2 > b->nt) {
@@ -338,7 +340,7 @@ extern "C" int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, dou
   }
   HIP_TRY(hipSetDevice(b->device));
   int rc;
-  for (int s = 0; s < 3; s++) {
+  for (int s = 0; s < 4; s++) {
     double best = DBL_MAX;
     int bt = b->tile[s];
     for (int t = 0; t < NTILES; t++) {

@@ -214,8 +214,8 @@ class Batch:
         return [p.record() for p in self.plans]
 
     def autotune(self, steps=5):
-        tiles = (C.c_int * 3)()
-        ms = (C.c_double * 3)()
+        tiles = (C.c_int * 4)()
+        ms = (C.c_double * 4)()
         check(self.lib.qp3d_batch_autotune(self.h, int(steps), tiles, ms))
         return {"tiles": list(tiles), "ms": list(ms)}
 

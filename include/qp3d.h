@@ -181,7 +181,8 @@ int qp3d_batch_destroy(qp3d_batch *b);
 int qp3d_batch_reset(qp3d_batch *b);
 int qp3d_batch_run(qp3d_batch *b, int k0, int k1);
 int qp3d_batch_sync(qp3d_batch *b);
-/* Picks each stage's tile shape by timing every candidate over `steps` steps, then resets. */
+/* Picks each stage's tile shape by timing every candidate over `steps` steps, then resets;
+ * tiles_out[4] / ms_out[4] (may be NULL): gradient max, interior, PML x/z faces, PML y faces. */
 int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, double *ms_out);
 /* bytes[0] gradient-max bytes, bytes[1] update bytes per step (all shots), bytes[2] updated
  * cells, bytes[3] interior cells. */
