@@ -38,7 +38,7 @@ struct qp3d_fwi {
   float *gacc = nullptr;         // per-segment FP32 sums w.r.t. v2, eps, eps-del (3 N)
   double *gsum = nullptr;        // FP64 sums (3 N)
   float *resid = nullptr;        // [nt][hx][hy]
-  WorkList wl_i, wl_p, wl_a;
+  WorkList wl_i, wl_p, wl_py, wl_a;  // interior, PML x and z faces, PML y faces, all of [5,n-5)^3
   bool forward_done = false;
   int ubuf_seg = -1;             // segment whose u_k ubuf holds (the forward run keeps the last)
   unsigned gen = 0;              // the plan's upload count at the forward run
@@ -58,6 +58,7 @@ static void fwi_free(qp3d_fwi *f)
   (void)hipFree(f->resid);
   (void)hipFree(f->wl_i.d);
   (void)hipFree(f->wl_p.d);
+  (void)hipFree(f->wl_py.d);
   (void)hipFree(f->wl_a.d);
   for (hipEvent_t e : f->ev)
     if (e) (void)hipEventDestroy(e);
@@ -118,7 +119,15 @@ extern "C" int qp3d_fwi_create(qp3d_plan *p, int interval, int ckpt_storage, qp3
   const int oi = occupancy(reinterpret_cast<const void *>(&k_adj_prod<TP, false>), TP::NT);
   const int op = occupancy(reinterpret_cast<const void *>(&k_adj_prod<TP, true>), TP::NT);
   const int oa = occupancy(reinterpret_cast<const void *>(&k_adj_apply<TA>), TA::NT);
-  if ((rc = qp3d_build_update_lists(p, TILE_PROD, TILE_PROD, oi, op, f->wl_i, f->wl_p))) return fail(rc);
+  {
+    // as in the forward plan, the y-face PML tiles go to a launch of their own
+    std::vector<int4> it, ip, ipy;
+    const int ncu = qp3d_cu_count(p);
+    qp3d_items_update(p, TILE_PROD, TILE_PROD, oi * ncu, op * ncu, it, ip, &ipy, TILE_PROD, op * ncu);
+    if ((rc = qp3d_upload_items(f->wl_i, it, false)) || (rc = qp3d_upload_items(f->wl_p, ip, false)) ||
+        (rc = qp3d_upload_items(f->wl_py, ipy, false)))
+      return fail(rc);
+  }
   if ((rc = qp3d_build_uniform_list(p, TILE_APPLY, oa, f->wl_a))) return fail(rc);
   for (hipEvent_t &e : f->ev)
     if (hipEventCreate(&e) != hipSuccess) return fail(QP3D_ERR_HIP);
@@ -231,9 +240,10 @@ static int adjoint_step(qp3d_fwi *f, int k, const float *uk)
     hipLaunchKernelGGL((k_adj_prod<TP, false>), dim3(f->wl_i.n), dim3(TY, TP::WX), 0, p->stream, a);
     HIP_TRY(hipGetLastError());
   }
-  if (f->wl_p.n) {
-    a.work = f->wl_p.d;
-    hipLaunchKernelGGL((k_adj_prod<TP, true>), dim3(f->wl_p.n), dim3(TY, TP::WX), 0, p->stream, a);
+  for (const WorkList *w : {&f->wl_p, &f->wl_py}) {
+    if (!w->n) continue;
+    a.work = w->d;
+    hipLaunchKernelGGL((k_adj_prod<TP, true>), dim3(w->n), dim3(TY, TP::WX), 0, p->stream, a);
     HIP_TRY(hipGetLastError());
   }
   ApplyArgs b;
