@@ -856,6 +856,38 @@ extern "C" int qp3d_plan_snapshot(const qp3d_plan *p, int i, int *step, float *o
   return QP3D_OK;
 }
 
+extern "C" int qp3d_work_items(const qp3d_config *cfg, int which, int capacity, int *out, int cap)
+{
+  if (!cfg || which < 0 || which > 3 || capacity < 1 || cap < 0 || (cap > 0 && !out)) {
+    qp3d_set_error("work_items: bad argument");
+    return QP3D_ERR_ARG;
+  }
+  const qp3d_config &c = *cfg;
+  if (c.pml < 6 || c.nx < 2 * c.pml + 1 || c.ny < 2 * c.pml + 1 || c.nz < 2 * c.pml + 1 || c.tile_update < 0 ||
+      c.tile_update > NTILES || c.tile_grad < 0 || c.tile_grad > NTILES || c.tile_pml < 0 || c.tile_pml > NTILES ||
+      c.tile_pml_y < 0 || c.tile_pml_y > NTILES) {
+    qp3d_set_error("work_items: bad configuration");
+    return QP3D_ERR_ARG;
+  }
+  qp3d_plan p;  // host-side description only: nothing is allocated
+  p.cfg = c;
+  const int tu = c.tile_update > 0 ? c.tile_update - 1 : 0, tg = c.tile_grad > 0 ? c.tile_grad - 1 : 2;
+  const int tp = c.tile_pml > 0 ? c.tile_pml - 1 : 5, tpy = c.tile_pml_y > 0 ? c.tile_pml_y - 1 : 1;
+  std::vector<int4> it, ip, ipy;
+  if (which == 0)
+    qp3d_items_uniform(&p, tg, capacity, it);
+  else
+    qp3d_items_update(&p, tu, tp, capacity, capacity, it, ip, &ipy, tpy, capacity);
+  const std::vector<int4> &v = which == 3 ? ipy : which == 2 ? ip : it;
+  for (size_t i = 0; i < v.size() && (int)i < cap; i++) {
+    out[4 * i] = v[i].x;
+    out[4 * i + 1] = v[i].y;
+    out[4 * i + 2] = v[i].z;
+    out[4 * i + 3] = v[i].w;
+  }
+  return (int)v.size();
+}
+
 extern "C" int qp3d_plan_dims(const qp3d_plan *p, int *hx, int *hy, int *hz)
 {
   if (!p) return QP3D_ERR_ARG;

@@ -68,6 +68,7 @@ _SIGS = [
     ("qp3d_plan_snapshot", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _F]),
     ("qp3d_plan_step_maxima", C.c_int, [C.c_void_p, _F]),
     ("qp3d_plan_set_instrument", C.c_int, [C.c_void_p, C.c_int]),
+    ("qp3d_work_items", C.c_int, [C.POINTER(Config), C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int]),
     ("qp3d_plan_dims", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("qp3d_plan_kernel_times", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     ("qp3d_plan_step_bytes", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),

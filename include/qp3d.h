@@ -133,6 +133,12 @@ int qp3d_plan_snapshot_count(const qp3d_plan *p);
 int qp3d_plan_snapshot(const qp3d_plan *p, int i, int *step, float *out);
 /* Per-step gradient maxima G[k] (float) of steps [0, nt). */
 int qp3d_plan_step_maxima(const qp3d_plan *p, float *out);
+/* Work items the plan would launch for a configuration, built on the host without a device
+ * (for tests and tools): which = 0 gradient-max pass, 1 interior update, 2 PML update on the
+ * x and z faces, 3 PML update on the y faces; capacity = resident workgroups the z-chunking
+ * fills. Writes min(count, cap) items of 4 ints (x0 | x1 << 16, y0 | y1 << 16 with bit 31 set
+ * for a folded y-face tile, z0, z1) in logical order and returns the count, or a QP3D_* error. */
+int qp3d_work_items(const qp3d_config *cfg, int which, int capacity, int *out, int cap);
 /* Interior extents hx, hy, hz of the plan's grid (any output may be NULL). */
 int qp3d_plan_dims(const qp3d_plan *p, int *hx, int *hy, int *hz);
 /* 1: bracket every stencil launch with HIP events on the plan's stream (and reset the
