@@ -1,8 +1,9 @@
 // gfx950 kernels of the qP3D time step.
 //
 // One time step of the reference (3dPureqPModeling_single_shot.cu:277-337) launches five
-// per-cell kernels and copies 16 B/cell over PCIe. Here a step is three launches of two
-// stencil passes over the wavefield, hand-tiled for CDNA4:
+// per-cell kernels and copies 16 B/cell over PCIe. Here a step is four launches of two
+// stencil passes over the wavefield (the PML update goes out as two launches: x and z faces,
+// y faces), hand-tiled for CDNA4:
 //   k_gradmax        max over the grid of |du/dx|,|du/dy|,|du/dz| (GPU_kernel.cu:14-33 grad and
 //                    the host absMaxval at 3dPureqPModeling_single_shot.cu:284-292), reduced on
 //                    device;
@@ -26,8 +27,8 @@
 // derivatives; each lane keeps an 11-deep register queue of its column in z for the z
 // derivatives. Every global load of plane z+1 (queue front, halos, model, previous wavefield,
 // PML state) is issued before plane z is computed, so HBM latency overlaps the arithmetic of
-// the plane before. Loads are unconditional: lanes outside the grid read a clamped cell and
-// never use it. A lane's offsets within a plane are fixed for the whole march and the plane
+// the plane before. Loads of lanes outside the tile read a clamped cell and are never used.
+// A lane's offsets within a plane are fixed for the whole march and the plane
 // base is a scalar, so each load is one instruction with no vector address arithmetic
 // (grids stay below 2^30 cells).
 #pragma once
