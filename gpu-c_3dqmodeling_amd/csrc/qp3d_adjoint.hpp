@@ -68,7 +68,7 @@ struct PlaneLoadC {
   __device__ __forceinline__ void stage(float (*sm)[T::LY], const Lane<T> &L) const
   {
 #pragma unroll
-    for (int r = 0; r < T::RX; r++) sm[H + L.tx + T::WX * r][L.Y] = c[r];
+    for (int r = 0; r < T::RX; r++) sm[H + L.row(r)][L.Y] = c[r];
 #pragma unroll
     for (int i = 0; i < T::NXH; i++) {
       const int h = L.tx + T::WX * i;
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(T::NT) void k_adj_prod(AdjArgs a)
   const int hy = ny - 2 * pml;
   bool rows_on[T::RX];
 #pragma unroll
-  for (int r = 0; r < T::RX; r++) rows_on[r] = L.yon && x0 + L.tx + T::WX * r < x1;
+  for (int r = 0; r < T::RX; r++) rows_on[r] = L.yon && x0 + L.row(r) < x1;
   const float G = __uint_as_float(*a.gmax);
   const float thr = __double2float_rd(1.0e-02 * (double)G);
 
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(T::NT) void k_adj_prod(AdjArgs a)
       sgn = x1 <= pml ? 1.f : -1.f;
 #pragma unroll
       for (int r = 0; r < T::RX; r++) {
-        const int x = min(x0 + L.tx + T::WX * r, x1 - 1);
+        const int x = min(x0 + L.row(r), x1 - 1);
         trow[r] = a.pmltab[x < pml ? pml - x : x - nx + pml];
       }
     } else if (fam == 1) {
@@ -198,8 +198,8 @@ __global__ __launch_bounds__(T::NT) void k_adj_prod(AdjArgs a)
 #pragma unroll
     for (int r = 0; r < T::RX; r++) {
       if (!rows_on[r]) continue;
-      const int X = H + L.tx + T::WX * r, Y = L.Y;
-      const int x = x0 + L.tx + T::WX * r;
+      const int X = H + L.row(r), Y = L.Y;
+      const int x = x0 + L.row(r);
       const float c0 = q[r][H];
       float Lxx = 0.f, Lyy = 0.f, Lzz = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
 #pragma unroll
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(T::NT) void k_adj_apply(ApplyArgs a)
   Lane<T> L(g, x0, x1, y0, y1, fold);
   bool rows_on[T::RX];
 #pragma unroll
-  for (int r = 0; r < T::RX; r++) rows_on[r] = L.yon && x0 + L.tx + T::WX * r < x1;
+  for (int r = 0; r < T::RX; r++) rows_on[r] = L.yon && x0 + L.row(r) < x1;
   const float *pl[4] = {a.pxx, a.qx, a.pyy, a.qy};
   float qzz[T::RX][QN], qqz[T::RX][QN];
   prime_queue<StoreF32, T>(qzz, a.pzz, g, L, z0);
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(T::NT) void k_adj_apply(ApplyArgs a)
 #pragma unroll
     for (int r = 0; r < T::RX; r++) {
       if (!rows_on[r]) continue;
-      const int X = H + L.tx + T::WX * r, Y = L.Y;
+      const int X = H + L.row(r), Y = L.Y;
       float (*sxx)[T::LY] = sm[b][0], (*sqx)[T::LY] = sm[b][1], (*syy)[T::LY] = sm[b][2], (*sqy)[T::LY] = sm[b][3];
       float lxx = 0.f, lyy = 0.f, lzz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
 #pragma unroll

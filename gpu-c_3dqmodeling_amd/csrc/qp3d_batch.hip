@@ -18,7 +18,7 @@ struct qp3d_batch {
   bool use_graph = false;
   hipStream_t stream = nullptr;
   // gradient max, interior update, PML update on x and z faces, on y faces (QP3D_TILES index)
-  int tile[4] = {2, 0, 5, 1};
+  int tile[4] = {6, 0, 5, 1};
   WorkList wl[4];
   void *args_g[2] = {}, *args_u[2] = {};  // per-parity argument blocks of every shot
   int *kdev = nullptr;

@@ -178,7 +178,11 @@ struct GradArgs {
 // Loads of u are exact up to 5 cells past the tile (the halo the stencil reads) and clamped
 // beyond; per-cell loads (model, previous wavefield, PML state) of lanes outside the tile are
 // clamped into the tile, so idle lanes add no memory traffic.
-template <class T>
+//
+// Rows: a lane's rows r = 0..RX-1 sit at tile row tx + WX*r (interleaved across the wavefronts)
+// or, with RUN, at RX*tx + r (a run of consecutive rows per wavefront, so the x stencil of the
+// run reads only the 10 rows around it from LDS and the rest from the lane's own registers).
+template <class T, bool RUN = false>
 struct Lane {
   int tx, ty;            // wavefront (x group) and lane in the workgroup
   int y, Y;              // y of the lane and its LDS column
@@ -193,6 +197,8 @@ struct Lane {
                          // without one write a spare slot past the plane (LX * LY)
   bool hx_on[T::NXH];    // wave-uniform
   bool hy_on;
+
+  __device__ __forceinline__ int row(int r) const { return RUN ? T::RX * tx + r : tx + T::WX * r; }
 
   __device__ __forceinline__ Lane(const Geo &g, int x0, int x1, int y0, int y1, bool fold_) : fold(fold_)
   {
@@ -216,7 +222,7 @@ struct Lane {
     }
 #pragma unroll
     for (int r = 0; r < T::RX; r++) {
-      const int x = x0 + tx + T::WX * r;
+      const int x = x0 + row(r);
       qoff[r] = 4u * ((unsigned)min(x, x1 + H - 1) * g.ny + yq);
       off[r] = 4u * ((unsigned)min(x, x1 - 1) * g.ny + yc);
     }
@@ -266,7 +272,8 @@ struct PlaneLoad {
   float hx[T::NXH];
   float hy;
 
-  __device__ __forceinline__ void issue(const typename S::T *u, const Geo &g, const Lane<T> &L, int z)
+  template <bool RUN>
+  __device__ __forceinline__ void issue(const typename S::T *u, const Geo &g, const Lane<T, RUN> &L, int z)
   {
     const typename S::T *uq = u + (size_t)(z + H) * g.pln;
     const typename S::T *uz = u + (size_t)z * g.pln;
@@ -280,10 +287,11 @@ struct PlaneLoad {
 
   // Write plane z (centre values from the queue, halos from this load) into an LDS buffer
   // (LX + 1 rows: the last holds the spare slot).
-  __device__ __forceinline__ void stage(float (*sm)[T::LY], const float (&centre)[T::RX], const Lane<T> &L) const
+  template <bool RUN>
+  __device__ __forceinline__ void stage(float (*sm)[T::LY], const float (&centre)[T::RX], const Lane<T, RUN> &L) const
   {
 #pragma unroll
-    for (int r = 0; r < T::RX; r++) sm[H + L.tx + T::WX * r][L.Y] = centre[r];
+    for (int r = 0; r < T::RX; r++) sm[H + L.row(r)][L.Y] = centre[r];
 #pragma unroll
     for (int i = 0; i < T::NXH; i++) {
       const int h = L.tx + T::WX * i;
@@ -293,9 +301,9 @@ struct PlaneLoad {
   }
 };
 
-template <class S, class T>
+template <class S, class T, bool RUN>
 __device__ __forceinline__ void prime_queue(float (&q)[T::RX][2 * H + 1], const typename S::T *u, const Geo &g,
-                                            const Lane<T> &L, int z0)
+                                            const Lane<T, RUN> &L, int z0)
 {
 #pragma unroll
   for (int j = 0; j < 2 * H; j++) {
@@ -333,7 +341,10 @@ struct Unroll<N, N> {
   }
 };
 
-// One work item of the gradient-max pass (w: packed tile, kstep: time step).
+// One work item of the gradient-max pass (w: packed tile, kstep: time step). Each lane owns a
+// run of RX consecutive rows: the x stencil of the run reads 10 rows from LDS and takes the
+// run's own rows from the lane's queue, which cuts the pass's LDS reads (its bottleneck) from
+// 10 + 10 to 10/RX + 10 per cell.
 template <class S, class T>
 __device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int kstep)
 {
@@ -345,11 +356,11 @@ __device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int k
   bool fold;
   unpack(w, x0, x1, y0, y1, z0, z1, fold);
   if (z0 >= z1) return;
-  Lane<T> L(g, x0, x1, y0, y1, fold);
+  Lane<T, true> L(g, x0, x1, y0, y1, fold);
   const bool ycmp = L.yon;
   bool rows_on[T::RX];
 #pragma unroll
-  for (int r = 0; r < T::RX; r++) rows_on[r] = ycmp && x0 + L.tx + T::WX * r < x1;
+  for (int r = 0; r < T::RX; r++) rows_on[r] = ycmp && x0 + L.row(r) < x1;
   float q[T::RX][QN];
   prime_queue<S, T>(q, a.ucur, g, L, z0);
   // The march is unrolled by 22 (twice the queue length), so the LDS buffer of every plane
@@ -377,13 +388,16 @@ __device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int k
       L.refresh();
       if (more) P.issue(a.ucur, g, L, z + 1);
       float (*s)[T::LY] = sm[it & 1];
+      const int XB = H + L.row(0);
+      // plane-z value of run row j (j outside [0, RX): LDS); the same float LDS holds
+      auto xv = [&](int j) -> float { return (j >= 0 && j < T::RX) ? q[j][(H + it) % QN] : s[XB + j][L.Y]; };
 #pragma unroll
       for (int r = 0; r < T::RX; r++) {
-        const int X = H + L.tx + T::WX * r, Y = L.Y;
+        const int X = XB + r, Y = L.Y;
         float ux = 0.f, uy = 0.f, uz = 0.f;
 #pragma unroll
         for (int k = 1; k <= H; k++) {
-          ux = fmaf(a.c.g1x[k - 1], s[X + k][Y] - s[X - k][Y], ux);
+          ux = fmaf(a.c.g1x[k - 1], xv(r + k) - xv(r - k), ux);
           uy = fmaf(a.c.g1y[k - 1], s[X][Y + k] - s[X][Y - k], uy);
           uz = fmaf(a.c.g1z[k - 1], q[r][(H + k + it) % QN] - q[r][(H - k + it) % QN], uz);
         }
@@ -502,7 +516,7 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
   const int nx = g.nx, ny = g.ny, nz = g.nz, pml = g.pml;
   bool rows_on[T::RX];
 #pragma unroll
-  for (int r = 0; r < T::RX; r++) rows_on[r] = ycmp && x0 + L.tx + T::WX * r < x1;
+  for (int r = 0; r < T::RX; r++) rows_on[r] = ycmp && x0 + L.row(r) < x1;
 
   // S threshold: |grad u| > 0.01*G compared in double (GPU_kernel.cu:99); for a float
   // magnitude this equals mode > rd(0.01*G).
@@ -525,7 +539,7 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
       sgn = x1 <= pml ? 1.f : -1.f;
 #pragma unroll
       for (int r = 0; r < T::RX; r++) {
-        const int x = min(x0 + L.tx + T::WX * r, x1 - 1);
+        const int x = min(x0 + L.row(r), x1 - 1);
         trow[r] = a.pmltab[x < pml ? pml - x : x - nx + pml];
       }
     } else if (fam == 1) {
@@ -582,8 +596,8 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
 #pragma unroll
       for (int r = 0; r < T::RX; r++) {
         if (!rows_on[r]) continue;
-        const int X = H + L.tx + T::WX * r, Y = L.Y;
-        const int x = x0 + L.tx + T::WX * r;
+        const int X = H + L.row(r), Y = L.Y;
+        const int x = x0 + L.row(r);
         const float c0 = q[r][(H + it) % QN];
         // second derivatives as sum_m c2_m (u+m + u-m) - 2 u0 sum_m c2_m, first derivatives as
         // sum_m c1_m (u+m - u-m); each pair's sum and difference are shared.
@@ -740,7 +754,7 @@ static __global__ void k_set_step(int *kdev, int k)
 }
 
 // Tile shapes the plan can launch (index = qp3d_config.tile_update / tile_grad - 1).
-#define QP3D_TILES(X) X(8, 2) X(16, 1) X(4, 2) X(8, 1) X(12, 1) X(16, 2)
-constexpr int NTILES = 6;
+#define QP3D_TILES(X) X(8, 2) X(16, 1) X(4, 2) X(8, 1) X(12, 1) X(16, 2) X(2, 4)
+constexpr int NTILES = 7;
 
 }  // namespace qp3d

@@ -345,7 +345,7 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
   // defaults measured fastest on MI355X at 576^3 (FP32): 8x2 interior, 4x2 gradient max,
   // 16x2 PML
   p->tile_u = c.tile_update > 0 ? c.tile_update - 1 : 0;
-  p->tile_g = c.tile_grad > 0 ? c.tile_grad - 1 : 2;
+  p->tile_g = c.tile_grad > 0 ? c.tile_grad - 1 : 6;
   p->tile_p = c.tile_pml > 0 ? c.tile_pml - 1 : 5;
   p->tile_py = c.tile_pml_y > 0 ? c.tile_pml_y - 1 : 1;  // 16x1
   p->two_streams = c.streams == 2 && !c.use_graph;  // a captured loop runs on one stream
@@ -871,7 +871,7 @@ extern "C" int qp3d_work_items(const qp3d_config *cfg, int which, int capacity, 
   }
   qp3d_plan p;  // host-side description only: nothing is allocated
   p.cfg = c;
-  const int tu = c.tile_update > 0 ? c.tile_update - 1 : 0, tg = c.tile_grad > 0 ? c.tile_grad - 1 : 2;
+  const int tu = c.tile_update > 0 ? c.tile_update - 1 : 0, tg = c.tile_grad > 0 ? c.tile_grad - 1 : 6;
   const int tp = c.tile_pml > 0 ? c.tile_pml - 1 : 5, tpy = c.tile_pml_y > 0 ? c.tile_pml_y - 1 : 1;
   std::vector<int4> it, ip, ipy;
   if (which == 0)

@@ -8,7 +8,7 @@ import pytest
 
 from qp3d._lib import QP3D_ERR_ARG, Config, check, load
 
-TX = {1: 16, 2: 16, 3: 8, 4: 8, 5: 12, 6: 32}  # rows of x per tile shape (8x2 16x1 4x2 8x1 12x1 16x2)
+TX = {1: 16, 2: 16, 3: 8, 4: 8, 5: 12, 6: 32, 7: 8}  # rows of x per tile shape (8x2 16x1 4x2 8x1 12x1 16x2 2x4)
 H = 5
 
 
@@ -41,6 +41,7 @@ CASES = [
     (90, 100, 80, 32, 1, 3, 5, 1, 0, 512),
     (95, 97, 88, 36, 2, 1, 6, 4, 7, 300),
     (85, 77, 90, 13, 4, 2, 1, 2, 0, 100),
+    (70, 75, 66, 12, 7, 7, 7, 7, 9, 200),
 ]
 
 
@@ -56,7 +57,7 @@ def test_work_items_cover_each_updated_cell_once(case):
     g = np.zeros_like(upd)
     x0, x1, y0, y1, z0, z1, fold = items(cfg, 0, cap)
     assert not fold.any()
-    assert np.all(x1 - x0 <= TX[tg or 3]) and np.all(y1 - y0 <= 64) and np.all(z1 > z0)
+    assert np.all(x1 - x0 <= TX[tg or 7]) and np.all(y1 - y0 <= 64) and np.all(z1 > z0)
     cover(g, x0, x1, y0, y1, z0, z1, fold, ny, pml)
     assert np.array_equal(g, upd)
     # update: interior + PML x/z faces + PML y faces
