@@ -47,8 +47,8 @@ struct AdjArgs {
 __device__ __forceinline__ float &at(float *p, size_t zo, unsigned b) { return *bat(p + zo, b); }
 
 // Loads of one plane z of a field that has no z queue: the lane's own rows plus its share of
-// the x and y halos.
-template <class T>
+// the x halo (XH) and the y halo (YH), for fields whose stencil runs along x or y.
+template <class T, bool XH = true, bool YH = true>
 struct PlaneLoadC {
   float c[T::RX];
   float hx[T::NXH];
@@ -59,22 +59,26 @@ struct PlaneLoadC {
     const float *fz = f + (size_t)z * g.pln;
 #pragma unroll
     for (int r = 0; r < T::RX; r++) c[r] = *bat(fz, L.qoff[r]);
+    if (XH) {
 #pragma unroll
-    for (int i = 0; i < T::NXH; i++)
-      if (L.hx_on[i]) hx[i] = *bat(fz, L.hxo[i]);
-    if (L.hy_on) hy = *bat(fz, L.hyo);
+      for (int i = 0; i < T::NXH; i++)
+        if (__builtin_amdgcn_readfirstlane(L.hx_on[i])) hx[i] = *bat(fz, L.hxo[i]);
+    }
+    if (YH && L.hy_on) hy = *bat(fz, L.hyo);
   }
 
   __device__ __forceinline__ void stage(float (*sm)[T::LY], const Lane<T> &L) const
   {
 #pragma unroll
     for (int r = 0; r < T::RX; r++) sm[H + L.row(r)][L.Y] = c[r];
+    if (XH) {
 #pragma unroll
-    for (int i = 0; i < T::NXH; i++) {
-      const int h = L.tx + T::WX * i;
-      if (L.hx_on[i]) sm[h < H ? h : T::TX + h][L.Y] = hx[i];
+      for (int i = 0; i < T::NXH; i++) {
+        const int h = L.tx + T::WX * i;
+        if (__builtin_amdgcn_readfirstlane(L.hx_on[i])) sm[h < H ? h : T::TX + h][L.Y] = hx[i];
+      }
     }
-    if (L.hy_on) {
+    if (YH && L.hy_on) {
       const int t = L.tx * TY + L.ty;
       const int row = t / (2 * H), col = t - row * (2 * H);
       sm[H + row][Lane<T>::halo_col(col, L.fold)] = hy;
@@ -353,74 +357,89 @@ __global__ __launch_bounds__(T::NT) void k_adj_apply(ApplyArgs a)
   bool rows_on[T::RX];
 #pragma unroll
   for (int r = 0; r < T::RX; r++) rows_on[r] = L.yon && x0 + L.row(r) < x1;
-  const float *pl[4] = {a.pxx, a.qx, a.pyy, a.qy};
   float qzz[T::RX][QN], qqz[T::RX][QN];
   prime_queue<StoreF32, T>(qzz, a.pzz, g, L, z0);
   prime_queue<StoreF32, T>(qqz, a.qz, g, L, z0);
-  PlaneLoadC<T> P[4];
+  // Pxx and Qx need only their x halos, Pyy and Qy only their y halos.
+  PlaneLoadC<T, true, false> Pxx, Pqx;
+  PlaneLoadC<T, false, true> Pyy, Pqy;
   float fzz[T::RX], fqz[T::RX];
+  float acn[T::RX], acv[T::RX];  // the adjoint of u at the cells, loaded a plane ahead
   auto issue = [&](int z) {
-#pragma unroll
-    for (int f = 0; f < 4; f++) P[f].issue(pl[f], g, L, z);
+    Pxx.issue(a.pxx, g, L, z);
+    Pqx.issue(a.qx, g, L, z);
+    Pyy.issue(a.pyy, g, L, z);
+    Pqy.issue(a.qy, g, L, z);
     const float *uzz = a.pzz + (size_t)(z + H) * g.pln, *uqz = a.qz + (size_t)(z + H) * g.pln;
+    const size_t zo = (size_t)z * g.pln;
 #pragma unroll
     for (int r = 0; r < T::RX; r++) {
       fzz[r] = *bat(uzz, L.qoff[r]);
       fqz[r] = *bat(uqz, L.qoff[r]);
+      acn[r] = at(a.ac, zo, L.off[r]);
     }
+  };
+  auto stage = [&](int buf) {
+    Pxx.stage(sm[buf][0], L);
+    Pqx.stage(sm[buf][1], L);
+    Pyy.stage(sm[buf][2], L);
+    Pqy.stage(sm[buf][3], L);
   };
   issue(z0);
 #pragma unroll
   for (int r = 0; r < T::RX; r++) {
     qzz[r][2 * H] = fzz[r];
     qqz[r][2 * H] = fqz[r];
+    acv[r] = acn[r];
   }
-#pragma unroll
-  for (int f = 0; f < 4; f++) P[f].stage(sm[0][f], L);
+  stage(0);
   __syncthreads();
-  int b = 0;
-  for (int z = z0; z < z1; z++) {
+  // The march is unrolled by 22 (twice the queue length): queue slots and the LDS buffer of
+  // every plane are static (the value of plane z-5+j sits in slot (j + it) % 11).
+  int zb = z0;
+  auto plane = [&](auto itc) {
+    constexpr int it = decltype(itc)::value;
+    const int z = zb + it;
     const bool more = z + 1 < z1;
     L.refresh();
     if (more) issue(z + 1);
     const size_t zo = (size_t)z * g.pln;
+    float (*sxx)[T::LY] = sm[it & 1][0], (*sqx)[T::LY] = sm[it & 1][1], (*syy)[T::LY] = sm[it & 1][2],
+          (*sqy)[T::LY] = sm[it & 1][3];
 #pragma unroll
     for (int r = 0; r < T::RX; r++) {
       if (!rows_on[r]) continue;
       const int X = H + L.row(r), Y = L.Y;
-      float (*sxx)[T::LY] = sm[b][0], (*sqx)[T::LY] = sm[b][1], (*syy)[T::LY] = sm[b][2], (*sqy)[T::LY] = sm[b][3];
       float lxx = 0.f, lyy = 0.f, lzz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
 #pragma unroll
       for (int k = 1; k <= H; k++) {
         lxx = fmaf(a.c.c2[k - 1], sxx[X + k][Y] + sxx[X - k][Y], lxx);
         lyy = fmaf(a.c.c2[k - 1], syy[X][Y + k] + syy[X][Y - k], lyy);
-        lzz = fmaf(a.c.c2[k - 1], qzz[r][H + k] + qzz[r][H - k], lzz);
+        lzz = fmaf(a.c.c2[k - 1], qzz[r][(H + k + it) % QN] + qzz[r][(H - k + it) % QN], lzz);
         dx = fmaf(a.c.c1[k - 1], sqx[X - k][Y] - sqx[X + k][Y], dx);
         dy = fmaf(a.c.c1[k - 1], sqy[X][Y - k] - sqy[X][Y + k], dy);
-        dz = fmaf(a.c.c1[k - 1], qqz[r][H - k] - qqz[r][H + k], dz);
+        dz = fmaf(a.c.c1[k - 1], qqz[r][(H - k + it) % QN] - qqz[r][(H + k + it) % QN], dz);
       }
       float acc = fmaf(-a.c.s2, sxx[X][Y], lxx) * a.c.ixx;
       acc = fmaf(fmaf(-a.c.s2, syy[X][Y], lyy), a.c.iyy, acc);
-      acc = fmaf(fmaf(-a.c.s2, qzz[r][H], lzz), a.c.izz, acc);
+      acc = fmaf(fmaf(-a.c.s2, qzz[r][(H + it) % QN], lzz), a.c.izz, acc);
       acc = fmaf(dx, a.c.ix, acc);
       acc = fmaf(dy, a.c.iy, acc);
       acc = fmaf(dz, a.c.iz, acc);
-      at(a.ac, zo, L.off[r]) += acc;
+      at(a.ac, zo, L.off[r]) = acv[r] + acc;
     }
     if (more) {
-      shift_queue<T>(qzz);
-      shift_queue<T>(qqz);
 #pragma unroll
       for (int r = 0; r < T::RX; r++) {
-        qzz[r][2 * H] = fzz[r];
-        qqz[r][2 * H] = fqz[r];
+        qzz[r][it % QN] = fzz[r];
+        qqz[r][it % QN] = fqz[r];
+        acv[r] = acn[r];
       }
-#pragma unroll
-      for (int f = 0; f < 4; f++) P[f].stage(sm[b ^ 1][f], L);
+      stage((it + 1) & 1);
       __syncthreads();
-      b ^= 1;
     }
-  }
+  };
+  for (; zb < z1; zb += 2 * QN) Unroll<0, 2 * QN>::run(plane, z1 - zb);
 }
 
 // gsum += gacc over n cells (FP64 accumulation over time of the per-segment FP32 sums).
