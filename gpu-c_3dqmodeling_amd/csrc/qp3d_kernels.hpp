@@ -754,7 +754,7 @@ static __global__ void k_set_step(int *kdev, int k)
 }
 
 // Tile shapes the plan can launch (index = qp3d_config.tile_update / tile_grad - 1).
-#define QP3D_TILES(X) X(8, 2) X(16, 1) X(4, 2) X(8, 1) X(12, 1) X(16, 2) X(2, 4)
+#define QP3D_TILES(X) X(8, 2) X(16, 1) X(4, 2) X(8, 1) X(12, 1) X(16, 2) X(4, 4)
 constexpr int NTILES = 7;
 
 }  // namespace qp3d

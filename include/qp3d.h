@@ -105,7 +105,7 @@ typedef struct qp3d_config {
 } qp3d_config;
 
 /* Tile shapes (wavefronts along x x rows per lane; a tile is 64 cells along y):
- * 1: 8x2  2: 16x1  3: 4x2  4: 8x1  5: 12x1  6: 16x2  7: 2x4 */
+ * 1: 8x2  2: 16x1  3: 4x2  4: 8x1  5: 12x1  6: 16x2  7: 4x4 */
 #define QP3D_NTILES 7
 
 typedef struct qp3d_plan qp3d_plan;

@@ -8,7 +8,7 @@ import pytest
 
 from qp3d._lib import QP3D_ERR_ARG, Config, check, load
 
-TX = {1: 16, 2: 16, 3: 8, 4: 8, 5: 12, 6: 32, 7: 8}  # rows of x per tile shape (8x2 16x1 4x2 8x1 12x1 16x2 2x4)
+TX = {1: 16, 2: 16, 3: 8, 4: 8, 5: 12, 6: 32, 7: 16}  # rows of x per tile shape (8x2 16x1 4x2 8x1 12x1 16x2 4x4)
 H = 5
 
 
