@@ -14,6 +14,10 @@ cpu_baseline: the CPU oracle (the reference's arithmetic, bit-exact with its CPU
 on the host's CPU share, on a 20-step window that starts from the GPU's state at step 300.
 
     python bench.py [--steps K] [--warmup W] [--cpu-steps S] [--no-cpu]
+
+Other workloads (--workload): fwi (adjoint-state gradient), precision (16-bit storage against
+the FP64 oracle), batch (many shot windows in one launch per stage), dropin (the same shot
+through modeling3d() with host buffers, PCIe transfers included).
 """
 import argparse
 import json
@@ -189,6 +193,28 @@ def fd_study(qp3d, ndirs, n=64, nt=400, pml=20, rel=1e-5):
     return out
 
 
+def bench_dropin(args):
+    """The drop-in entry point with host buffers: qp3d_modeling3d (the reference's modeling3d
+    signature) on the 512^3 shot, timed around the whole call (plan creation, model upload over
+    PCIe, nt steps, record copy back, teardown); one untimed call first."""
+    import qp3d
+    s = qp3d.config_setup(hx=args.n, hy=args.n, nz0=args.n, pml=args.pml, nt=args.nt, seed=1, fm=25.0, src_depth=0)
+    record = np.zeros((s.nt, s.hx, s.hy), np.float32)
+    t = time.perf_counter()
+    qp3d.modeling3d(s, record=record)
+    log(f"untimed drop-in call {time.perf_counter() - t:.3f} s")
+    t = time.perf_counter()
+    qp3d.modeling3d(s, record=record)
+    wall = time.perf_counter() - t
+    host_bytes = 4 * (4 * s.nx * s.ny * s.nz + record.size)
+    out = {"metric": f"Gcell-updates/s through modeling3d() with host buffers ({args.n}^3, {s.nt} steps)",
+           "value": round(s.n_upd * s.nt / wall / 1e9, 3), "unit": "Gcell-updates/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "fp32", "data": "synthetic (seed 1)",
+           "config": {"workload": "drop-in call including PCIe transfers", "padded": [s.nx, s.ny, s.nz], "nt": s.nt,
+                      "s_per_call": round(wall, 3), "host_bytes_moved": host_bytes}}
+    print(json.dumps(out), flush=True)
+
+
 def bench_precision(args):
     """FP32, BF16 and FP16 storage against the FP64 oracle on a 256^3 crop of the 512^3 model
     (centre in x and y, top 256 cells in z), pml 32, 1500 steps: relative L2 gather error and
@@ -323,9 +349,10 @@ def main():
     ap.add_argument("--shots", type=int, default=32, help="batch: number of shots")
     ap.add_argument("--win", type=int, nargs=2, default=[160, 320], help="batch: window size range in x and y")
     ap.add_argument("--depth", type=int, default=300, help="batch: model depth (cells)")
-    ap.add_argument("--workload", default="forward", choices=["forward", "fwi", "precision", "batch"],
+    ap.add_argument("--workload", default="forward", choices=["forward", "fwi", "precision", "batch", "dropin"],
                     help="forward: the 512^3 shot (default); fwi: the 256^3 adjoint gradient; "
-                         "precision: FP32/BF16/FP16 storage against the FP64 oracle on a 256^3 crop")
+                         "precision: FP32/BF16/FP16 storage against the FP64 oracle on a 256^3 crop; "
+                         "dropin: the 512^3 shot through modeling3d() with host buffers")
     ap.add_argument("--fwi-full", action="store_true", help="fwi: also run with full-wavefield storage")
     ap.add_argument("--fd-dirs", type=int, default=20, help="fwi: directions of the finite-difference check")
     args = ap.parse_args()
@@ -339,6 +366,8 @@ def main():
         return bench_precision(args)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "dropin":
+        return bench_dropin(args)
 
     import qp3d
 

@@ -1,6 +1,6 @@
 // Many shots in one launch per stage: the gradient-max pass, the interior update and the PML
-// update of every shot of a batch go out as three launches per time step whose work items
-// carry the shot's index. Each shot keeps its own plan (grid, model, source, PML profiles,
+// update (x and z faces, y faces) of every shot of a batch go out as four launches per time
+// step whose work items carry the shot's index. Each shot keeps its own plan (grid, model, source, PML profiles,
 // wavefields, record); all shots advance in lockstep. The step sequence can be captured once
 // in a hipGraph, and an autotuner picks each stage's tile shape by timing the candidates.
 #include <hip/hip_runtime.h>

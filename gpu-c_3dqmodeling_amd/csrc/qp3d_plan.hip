@@ -342,8 +342,8 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
   p->N = (long)c.nx * c.ny * c.nz;
   p->geo.nx = c.nx; p->geo.ny = c.ny; p->geo.nz = c.nz; p->geo.pml = c.pml;
   p->geo.pln = (unsigned)c.nx * (unsigned)c.ny;
-  // defaults measured fastest on MI355X at 576^3 (FP32): 8x2 interior, 4x2 gradient max,
-  // 16x2 PML
+  // defaults measured fastest on MI355X at 576^3 (FP32): 8x2 interior, 4x4 gradient max,
+  // 16x2 PML x and z faces, 16x1 PML y faces
   p->tile_u = c.tile_update > 0 ? c.tile_update - 1 : 0;
   p->tile_g = c.tile_grad > 0 ? c.tile_grad - 1 : 6;
   p->tile_p = c.tile_pml > 0 ? c.tile_pml - 1 : 5;
