@@ -341,6 +341,29 @@ struct Unroll<N, N> {
   }
 };
 
+// Scalar operator of a cell from its gradient (GPU_kernel.cu:93-111): 1 at or below the
+// threshold |grad u| <= thr; above it n = grad u / (|grad u| + 2.22e-17) and
+// S = (1 + sqrt(1 - 8 (eps-del) (nx^2+ny^2) nz^2 / (1 + 2 eps (nx^2+ny^2))^2)) / 2.
+// e = eps, ed = eps - del. The update kernel and the S-field snapshot share it.
+__device__ __forceinline__ float scalar_op(float ux, float uy, float uz, float e, float ed, float thr)
+{
+  float Sv = 1.f;
+  const float mode = __builtin_amdgcn_sqrtf(fmaf(ux, ux, fmaf(uy, uy, uz * uz)));
+  if (mode > thr) {
+    const float inv = __builtin_amdgcn_rcpf(mode + 2.22e-17f);
+    const float nxv = ux * inv, nyv = uy * inv, nzv = uz * inv;
+    const float qa = fmaf(nxv, nxv, nyv * nyv);
+    const float den = fmaf(2.f * e, qa, 1.f);
+    const float X8 = 8.f * ed * qa * (nzv * nzv) * __builtin_amdgcn_rcpf(den * den);
+    Sv = 0.5f + 0.5f * __builtin_amdgcn_sqrtf(1.f - X8);
+  }
+  return Sv;
+}
+
+// The threshold 0.01*G of the scalar operator: the reference compares |grad u| > 0.01*G in
+// double (GPU_kernel.cu:99); for a float magnitude this equals mode > rd(0.01*G).
+__device__ __forceinline__ float s_threshold(float G) { return __double2float_rd(1.0e-02 * (double)G); }
+
 // One work item of the gradient-max pass (w: packed tile, kstep: time step). Each lane owns a
 // run of RX consecutive rows: the x stencil of the run reads 10 rows from LDS and takes the
 // run's own rows from the lane's queue, which cuts the pass's LDS reads (its bottleneck) from
@@ -518,12 +541,10 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
 #pragma unroll
   for (int r = 0; r < T::RX; r++) rows_on[r] = ycmp && x0 + L.row(r) < x1;
 
-  // S threshold: |grad u| > 0.01*G compared in double (GPU_kernel.cu:99); for a float
-  // magnitude this equals mode > rd(0.01*G).
   const float G = __uint_as_float(a.gmax[kstep]);
   const float wk = a.wav[kstep];
   float *const rec = a.rec + (size_t)kstep * (g.nx - 2 * g.pml) * (g.ny - 2 * g.pml);
-  const float thr = __double2float_rd(1.0e-02 * (double)G);
+  const float thr = s_threshold(G);
   // source box and receiver plane tests, hoisted to the workgroup and the plane (uniform)
   const bool src_tile = a.src_on && a.sb_x0 < x1 && a.sb_x1 >= x0 &&
                         (fold || (a.sb_y0 < y1 && a.sb_y1 >= y0)) && a.sb_z0 < z1 && a.sb_z1 >= z0;
@@ -619,18 +640,7 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
         const float Uzz = fmaf(-a.c.s2, c0, Lzz) * a.c.izz;
         ux *= a.c.ix; uy *= a.c.iy; uz *= a.c.iz;
         const float e = C.e[r], ed = C.ed[r], v2 = C.v2[r];
-        // scalar operator (GPU_kernel.cu:93-111): n = grad u / (|grad u| + 2.22e-17) above the
-        // threshold, S = (1 + sqrt(1 - 8 (eps-del) (nx^2+ny^2) nz^2 / (1 + 2 eps (nx^2+ny^2))^2)) / 2.
-        float Sv = 1.f;
-        const float mode = __builtin_amdgcn_sqrtf(fmaf(ux, ux, fmaf(uy, uy, uz * uz)));
-        if (mode > thr) {
-          const float inv = __builtin_amdgcn_rcpf(mode + 2.22e-17f);
-          const float nxv = ux * inv, nyv = uy * inv, nzv = uz * inv;
-          const float qa = fmaf(nxv, nxv, nyv * nyv);
-          const float den = fmaf(2.f * e, qa, 1.f);
-          const float X8 = 8.f * ed * qa * (nzv * nzv) * __builtin_amdgcn_rcpf(den * den);
-          Sv = 0.5f + 0.5f * __builtin_amdgcn_sqrtf(1.f - X8);
-        }
+        const float Sv = scalar_op(ux, uy, uz, e, ed, thr);
         const float e1 = fmaf(2.f, e, 1.f);
         const unsigned o = L.off[r];
         float un;
@@ -745,6 +755,38 @@ __global__ void k_snapshot(const typename S::T *u, float *out, Geo g)
   if (y >= hy) return;
   out[((long)z * hx + x) * hy + y] =
       S::ld(u, 4u * ((unsigned)(z + g.pml) * g.pln + (unsigned)(x + g.pml) * g.ny + (unsigned)(y + g.pml)));
+}
+
+// The scalar-operator field S of step k over the whole grid [nz][nx][ny] (1 outside
+// [5,n-5)^3), from u_k and G_k with the update kernel's arithmetic, so it is the S that step
+// used. The reference computes it every step into a full-grid array and has its writer
+// commented out (3dPureqPModeling_single_shot.cu:381-409); here it is formed only for snapshot
+// steps that ask for it. One thread per cell; the stencil reads go through the caches.
+template <class S>
+__global__ void k_sfield(const typename S::T *u, const float *eps, const float *epsd, const unsigned *gmax, Step st,
+                         Geo g, Coef c, float *out)
+{
+  const float thr = s_threshold(__uint_as_float(gmax[st.get()]));
+  const long n = (long)g.pln * g.nz;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int z = (int)(i / g.pln);
+    const int rem = (int)(i - (long)z * g.pln);
+    const int x = rem / g.ny, y = rem - x * g.ny;
+    float Sv = 1.f;
+    if (z >= H && z < g.nz - H && x >= H && x < g.nx - H && y >= H && y < g.ny - H) {
+      const unsigned b = 4u * (unsigned)i, bx = 4u * (unsigned)g.ny, bz = 4u * g.pln;
+      float ux = 0.f, uy = 0.f, uz = 0.f;
+#pragma unroll
+      for (int k = 1; k <= H; k++) {
+        ux = fmaf(c.c1[k - 1], S::ld(u, b + k * bx) - S::ld(u, b - k * bx), ux);
+        uy = fmaf(c.c1[k - 1], S::ld(u, b + 4u * k) - S::ld(u, b - 4u * k), uy);
+        uz = fmaf(c.c1[k - 1], S::ld(u, b + k * bz) - S::ld(u, b - k * bz), uz);
+      }
+      ux *= c.ix; uy *= c.iy; uz *= c.iz;
+      Sv = scalar_op(ux, uy, uz, eps[i], epsd[i], thr);
+    }
+    out[i] = Sv;
+  }
 }
 
 // Sets the device step index read by graph-replayed launches.

@@ -312,6 +312,10 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
     qp3d_set_error("plan_create: ntsnap must be positive when snapflag == 1");
     return QP3D_ERR_ARG;
   }
+  if (c.snap_s != 0 && (c.snap_s != 1 || c.snapflag != 1)) {
+    qp3d_set_error("plan_create: snap_s = 1 needs snapflag == 1 (got snap_s=%d snapflag=%d)", c.snap_s, c.snapflag);
+    return QP3D_ERR_ARG;
+  }
   if (c.tile_update < 0 || c.tile_update > NTILES || c.tile_grad < 0 || c.tile_grad > NTILES || c.tile_pml < 0 ||
       c.tile_pml > NTILES || c.tile_pml_y < 0 || c.tile_pml_y > NTILES || c.streams < 0 || c.streams > 2) {
     qp3d_set_error("plan_create: tile shapes must be in [0,%d] and streams in [0,2]", NTILES);
@@ -409,6 +413,11 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
       if ((rc = dmalloc((void **)&d, sb, "snapshot"))) return fail(rc);
       p->snaps.push_back(d);
       p->snap_steps.push_back(k);
+      if (c.snap_s) {
+        float *ds = nullptr;
+        if ((rc = dmalloc((void **)&ds, sizeof(float) * (size_t)p->N, "S-field snapshot"))) return fail(rc);
+        p->snaps_s.push_back(ds);
+      }
     }
   }
   if ((rc = build_work(p))) return fail(rc);
@@ -438,6 +447,7 @@ extern "C" int qp3d_plan_destroy(qp3d_plan *p)
   (void)hipFree(p->wl_pml.d);
   (void)hipFree(p->wl_pmly.d);
   for (float *d : p->snaps) (void)hipFree(d);
+  for (float *d : p->snaps_s) (void)hipFree(d);
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
   if (p->ev_grad) (void)hipEventDestroy(p->ev_grad);
   if (p->ev_pml) (void)hipEventDestroy(p->ev_pml);
@@ -665,6 +675,14 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare)
       hipLaunchKernelGGL(k_snapshot<S>, dim3((p->hy + 255) / 256, p->hx, p->hz), dim3(256), 0, p->stream,
                          (const T *)p->u[prev], p->snaps[si], p->geo);
       HIP_TRY(hipGetLastError());
+      if (si < (int)p->snaps_s.size()) {
+        // S of this step: from u(k) (slot cur, which this step does not write) and G(k)
+        const int blocks = (int)std::min<long>((p->N + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_sfield<S>, dim3(blocks), dim3(256), 0, p->stream, (const T *)p->u[cur],
+                           (const float *)p->eps, (const float *)p->epsd, (const unsigned *)p->gmax, st, p->geo,
+                           p->coef, p->snaps_s[si]);
+        HIP_TRY(hipGetLastError());
+      }
       p->nsnap_taken = std::max(p->nsnap_taken, si + 1);
     }
   }
@@ -853,6 +871,20 @@ extern "C" int qp3d_plan_snapshot(const qp3d_plan *p, int i, int *step, float *o
   const size_t n = (size_t)p->hx * p->hy * p->hz;
   HIP_TRY(hipMemcpy(out, p->snaps[i], sizeof(float) * n, hipMemcpyDeviceToHost));
   unscale(p, out, n);
+  return QP3D_OK;
+}
+
+extern "C" int qp3d_plan_snapshot_s(const qp3d_plan *p, int i, int *step, float *out)
+{
+  if (!p || i < 0 || i >= p->nsnap_taken || i >= (int)p->snaps_s.size() || !out) {
+    qp3d_set_error("plan_snapshot_s: no S-field snapshot %d (snapshots taken %d, S fields kept: %s)", i,
+                   p ? p->nsnap_taken : 0, p && !p->snaps_s.empty() ? "yes" : "no");
+    return QP3D_ERR_ARG;
+  }
+  HIP_TRY(hipSetDevice(p->cfg.device));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  if (step) *step = p->snap_steps[i];
+  HIP_TRY(hipMemcpy(out, p->snaps_s[i], sizeof(float) * (size_t)p->N, hipMemcpyDeviceToHost));
   return QP3D_OK;
 }
 

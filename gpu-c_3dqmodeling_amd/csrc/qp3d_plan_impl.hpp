@@ -75,6 +75,7 @@ struct qp3d_plan {
   hipGraphExec_t gexec = nullptr;
 
   std::vector<float *> snaps;
+  std::vector<float *> snaps_s;  // S-field snapshots (cfg.snap_s), [nz][nx][ny] each
   std::vector<int> snap_steps;
   int nsnap_taken = 0;
 

@@ -28,8 +28,10 @@ static int write_floats(const char *name, const float *a, size_t n)
 }
 
 /* 3dPureqPModeling_single_shot.cu:340-377 / modeling3D.c:89-124: the interior cube and its
- * three centre slices, xz at y'=(hy-1)/2, yz at x'=(hx-1)/2, xy at z'=(hz-1)/2. */
-static int write_snapshot_files(const char *prefix, int k, const float *snap, int hx, int hy, int hz)
+ * three centre slices, xz at y'=(hy-1)/2, yz at x'=(hx-1)/2, xy at z'=(hz-1)/2. With tag "S"
+ * the same for the S field over the padded grid, named as the reference's commented-out writer
+ * names them (3dPureqPModeling_single_shot.cu:381-409): <prefix>_S<k>.dat, _Sxz, _Syz, _Sxy. */
+static int write_snapshot_files(const char *prefix, const char *tag, int k, const float *snap, int hx, int hy, int hz)
 {
   char name[4096];
   float *xz = (float *)malloc(sizeof(float) * (size_t)hz * hx);
@@ -47,13 +49,13 @@ static int write_snapshot_files(const char *prefix, int k, const float *snap, in
     for (int y = 0; y < hy; y++) yz[(size_t)z * hy + y] = snap[((size_t)z * hx + (hx - 1) / 2) * hy + y];
   for (int x = 0; x < hx; x++)
     for (int y = 0; y < hy; y++) xy[(size_t)x * hy + y] = snap[((size_t)((hz - 1) / 2) * hx + x) * hy + y];
-  snprintf(name, sizeof name, "%s%d.dat", prefix, k);
+  snprintf(name, sizeof name, "%s%s%s%d.dat", prefix, *tag ? "_" : "", tag, k);
   if ((rc = write_floats(name, snap, (size_t)hx * hy * hz))) goto done;
-  snprintf(name, sizeof name, "%s_xz%d.dat", prefix, k);
+  snprintf(name, sizeof name, "%s_%sxz%d.dat", prefix, tag, k);
   if ((rc = write_floats(name, xz, (size_t)hz * hx))) goto done;
-  snprintf(name, sizeof name, "%s_yz%d.dat", prefix, k);
+  snprintf(name, sizeof name, "%s_%syz%d.dat", prefix, tag, k);
   if ((rc = write_floats(name, yz, (size_t)hz * hy))) goto done;
-  snprintf(name, sizeof name, "%s_xy%d.dat", prefix, k);
+  snprintf(name, sizeof name, "%s_%sxy%d.dat", prefix, tag, k);
   rc = write_floats(name, xy, (size_t)hx * hy);
 done:
   free(xz);
@@ -86,6 +88,10 @@ int qp3d_modeling3d(int nx, int ny, int nz, int nt, int ntsnap, float dx, float 
   cfg.storage = QP3D_STORAGE_FP32;
   cfg.snapflag = snapflag == 1 ? 1 : 0;
   cfg.ntsnap = ntsnap;
+  /* QP3D_SNAP_S=1 also writes the S field at every snapshot step (the reference has that
+     writer commented out, so it is off by default) */
+  const char *ss = getenv("QP3D_SNAP_S");
+  cfg.snap_s = cfg.snapflag && ss && atoi(ss) == 1;
   qp3d_plan *p = NULL;
   int rc = qp3d_plan_create(&cfg, &p);
   if (rc) return rc;
@@ -104,9 +110,22 @@ int qp3d_modeling3d(int nx, int ny, int nz, int nt, int ntsnap, float dx, float 
     }
     for (int i = 0; i < qp3d_plan_snapshot_count(p) && rc == QP3D_OK; i++) {
       int k = 0;
-      if (!(rc = qp3d_plan_snapshot(p, i, &k, snap))) rc = write_snapshot_files(snap_file, k, snap, hx, hy, hz);
+      if (!(rc = qp3d_plan_snapshot(p, i, &k, snap))) rc = write_snapshot_files(snap_file, "", k, snap, hx, hy, hz);
     }
     free(snap);
+    if (rc == QP3D_OK && cfg.snap_s) {
+      float *sf = (float *)malloc(sizeof(float) * (size_t)nx * ny * nz);
+      if (!sf) {
+        qp3d_set_error("modeling3d: out of host memory for an S-field snapshot");
+        rc = QP3D_ERR_ALLOC;
+        goto out;
+      }
+      for (int i = 0; i < qp3d_plan_snapshot_count(p) && rc == QP3D_OK; i++) {
+        int k = 0;
+        if (!(rc = qp3d_plan_snapshot_s(p, i, &k, sf))) rc = write_snapshot_files(snap_file, "S", k, sf, nx, ny, nz);
+      }
+      free(sf);
+    }
   }
 out:
   qp3d_plan_destroy(p);

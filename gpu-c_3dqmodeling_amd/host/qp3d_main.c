@@ -15,8 +15,11 @@
  *   - missing/short files and model windows outside the model are reported and stop the run
  *     instead of reading out of bounds (Main.c:25-30,187);
  *   - GPU failures are reported with the library's status and message.
- * Optional extension: QP3D_SRC_DEPTH=n places the source and receivers n cells below the
- * top of the interior (the reference forces depth 0, Main.c:200).
+ * Optional extensions: QP3D_SRC_DEPTH=n places the source and receivers n cells below the
+ * top of the interior (the reference forces depth 0, Main.c:200); QP3D_SNAP_S=1 makes
+ * modeling3d() also write the S field of every snapshot step (<snap_file>_S<k>.dat, _Sxz,
+ * _Syz, _Sxy; the reference's writer for them is commented out,
+ * 3dPureqPModeling_single_shot.cu:381-409).
  */
 #include <stdio.h>
 #include <stdlib.h>

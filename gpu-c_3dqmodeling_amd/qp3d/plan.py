@@ -31,12 +31,13 @@ class Plan:
     """Device-resident forward solver for one grid (qp3d_plan_* in include/qp3d.h)."""
 
     def __init__(self, s, storage="fp32", use_graph=False, snapflag=0, ntsnap=0, zchunk=0, device=0,
-                 tile_update=0, tile_grad=0, tile_pml=0, streams=0, tile_pml_y=0):
+                 tile_update=0, tile_grad=0, tile_pml=0, streams=0, tile_pml_y=0, snap_s=0):
         self.lib = load()
         self.s = s
         cfg = Config(s.nx, s.ny, s.nz, s.pml, s.nt, s.dx, s.dy, s.dz, s.dt, s.sz, STORAGE[storage],
                      int(use_graph), int(snapflag), int(ntsnap), int(zchunk), int(device),
-                     int(tile_update), int(tile_grad), int(tile_pml), int(streams), int(tile_pml_y))
+                     int(tile_update), int(tile_grad), int(tile_pml), int(streams), int(tile_pml_y),
+                     int(snap_s))
         h = C.c_void_p()
         check(self.lib.qp3d_plan_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -86,6 +87,17 @@ class Plan:
             out = np.zeros((s.hz, s.hx, s.hy), np.float32)
             k = C.c_int()
             check(self.lib.qp3d_plan_snapshot(self.h, i, C.byref(k), fptr(out.reshape(-1))))
+            res.append((k.value, out))
+        return res
+
+    def snapshots_s(self):
+        """[(step, S)] of the snapshot steps, S over the padded grid [nz][nx][ny] (snap_s=1)."""
+        s = self.s
+        res = []
+        for i in range(self.lib.qp3d_plan_snapshot_count(self.h)):
+            out = np.zeros((s.nz, s.nx, s.ny), np.float32)
+            k = C.c_int()
+            check(self.lib.qp3d_plan_snapshot_s(self.h, i, C.byref(k), fptr(out.reshape(-1))))
             res.append((k.value, out))
         return res
 

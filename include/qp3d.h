@@ -102,6 +102,9 @@ typedef struct qp3d_config {
                       0 = default (1: on MI355X the two compete for HBM and gain nothing) */
   int tile_pml_y;  /* tile shape of the PML update on the y faces (a launch of its own),
                       1..QP3D_NTILES, 0 = default */
+  int snap_s;      /* 1 (with snapflag == 1): at every snapshot step k also keep the scalar-operator
+                      field S of that step over the whole padded grid (read it with
+                      qp3d_plan_snapshot_s); 0: not kept */
 } qp3d_config;
 
 /* Tile shapes (wavefronts along x x rows per lane; a tile is 64 cells along y):
@@ -131,6 +134,11 @@ int qp3d_plan_field(const qp3d_plan *p, int which, float *out);
 /* Number of snapshots held and the copy of snapshot i ([hz][hx][hy]) with its step index. */
 int qp3d_plan_snapshot_count(const qp3d_plan *p);
 int qp3d_plan_snapshot(const qp3d_plan *p, int i, int *step, float *out);
+/* The scalar-operator field S of snapshot i's step over the padded grid ([nz][nx][ny], 1 outside
+ * [5,n-5)^3), as that step's update used it; needs cfg.snap_s. Replaces the S-field snapshot
+ * the reference computes every step and has commented out of its writer
+ * (3dPureqPModeling_single_shot.cu:381-409). */
+int qp3d_plan_snapshot_s(const qp3d_plan *p, int i, int *step, float *out);
 /* Per-step gradient maxima G[k] (float) of steps [0, nt). */
 int qp3d_plan_step_maxima(const qp3d_plan *p, float *out);
 /* Work items the plan would launch for a configuration, built on the host without a device

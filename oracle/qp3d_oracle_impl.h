@@ -311,7 +311,8 @@ double SFX(oracle_last_G)(ST *s) { return (double)s->G; }
 
 /* modeling3D.c:17-153: nt steps, record after each step (:83-87), snapshots when
  * snapflag==1 and k%ntsnap==0 (:89-124). record is [nt][nx-2pml][ny-2pml]. Snapshot files are
- * written only by the float instance, byte-compatible with read_write.c:29-52. */
+ * written only by the float instance, byte-compatible with read_write.c:29-52; with
+ * QP3D_SNAP_S=1 in the environment the S field's files too. */
 int SFX(oracle_modeling3d)(int nx, int ny, int nz, int nt, int ntsnap, float dx, float dy, float dz, float dt, int pml,
                            int snapflag, int sz, const float *vp, const float *eps, const float *del,
                            const float *source, const float *wavelet, REAL *record, const float *dlr,
@@ -325,6 +326,9 @@ int SFX(oracle_modeling3d)(int nx, int ny, int nz, int nt, int ntsnap, float dx,
   const long pln = (long)nx * ny;
   REAL *snap = NULL;
   if (snapflag == 1 && sizeof(REAL) == sizeof(float)) snap = (REAL *)calloc((long)hx * hy * hz, sizeof(REAL));
+  /* QP3D_SNAP_S=1: also the S-field snapshot files, as the library's drop-in call writes them */
+  const char *ss = getenv("QP3D_SNAP_S");
+  const int snap_s = ss && atoi(ss) == 1;
   for (int k = 0; k < nt; k++) {
     SFX(oracle_step)(s, wavelet[k]);
     for (int x = pml; x < nx - pml; x++)
@@ -355,6 +359,28 @@ int SFX(oracle_modeling3d)(int nx, int ny, int nz, int nt, int ntsnap, float dx,
       snprintf(name, sizeof name, "%s_xy%d.dat", snap_file, k);
       if ((fp = fopen(name, "wb"))) { fwrite(xy, sizeof(REAL), (size_t)hx * hy, fp); fclose(fp); }
       free(xz); free(yz); free(xy);
+      if (snap_s) {
+        /* the S field of this step over the padded grid and its centre slices, the writer the
+           reference has commented out (3dPureqPModeling_single_shot.cu:381-409) */
+        snprintf(name, sizeof name, "%s_S%d.dat", snap_file, k);
+        if ((fp = fopen(name, "wb"))) { fwrite(s->S, sizeof(REAL), (size_t)nz * pln, fp); fclose(fp); }
+        REAL *sxz = (REAL *)malloc(sizeof(REAL) * (size_t)nz * nx);
+        REAL *syz = (REAL *)malloc(sizeof(REAL) * (size_t)nz * ny);
+        REAL *sxy = (REAL *)malloc(sizeof(REAL) * (size_t)nx * ny);
+        for (int z = 0; z < nz; z++)
+          for (int x = 0; x < nx; x++) sxz[(long)z * nx + x] = s->S[z * pln + (long)x * ny + (ny - 1) / 2];
+        for (int z = 0; z < nz; z++)
+          for (int y = 0; y < ny; y++) syz[(long)z * ny + y] = s->S[z * pln + (long)((nx - 1) / 2) * ny + y];
+        for (int x = 0; x < nx; x++)
+          for (int y = 0; y < ny; y++) sxy[(long)x * ny + y] = s->S[((nz - 1) / 2) * pln + (long)x * ny + y];
+        snprintf(name, sizeof name, "%s_Sxz%d.dat", snap_file, k);
+        if ((fp = fopen(name, "wb"))) { fwrite(sxz, sizeof(REAL), (size_t)nz * nx, fp); fclose(fp); }
+        snprintf(name, sizeof name, "%s_Syz%d.dat", snap_file, k);
+        if ((fp = fopen(name, "wb"))) { fwrite(syz, sizeof(REAL), (size_t)nz * ny, fp); fclose(fp); }
+        snprintf(name, sizeof name, "%s_Sxy%d.dat", snap_file, k);
+        if ((fp = fopen(name, "wb"))) { fwrite(sxy, sizeof(REAL), (size_t)nx * ny, fp); fclose(fp); }
+        free(sxz); free(syz); free(sxy);
+      }
     }
   }
   free(snap);

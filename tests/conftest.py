@@ -31,3 +31,11 @@ def rel_l2(a, b):
     nb = np.linalg.norm(b.ravel())
     return np.linalg.norm((a - b).ravel()) / (nb if nb > 0 else 1.0)
 
+
+def s_fields_agree(a, b, tol=5e-4, max_flip=2e-3):
+    """Two scalar-operator fields S agree: within tol everywhere except at cells whose |grad u|
+    lies within rounding of the 0.01*G threshold, where S jumps between 1 and S(n); those may
+    differ on at most a fraction max_flip of the cells."""
+    d = np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64))
+    far = d > tol
+    return far.mean() <= max_flip and (not (~far).any() or d[~far].max() <= tol)

@@ -9,7 +9,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import rel_l2
+from conftest import rel_l2, s_fields_agree
 
 import qp3d
 from oracle import oracle as O
@@ -112,3 +112,41 @@ def test_qp3d_main_matches_the_oracle_driver(tmp_path):
             assert not np.any(a), n
     R = p["nt"] * p["hx"] * p["hy"]
     assert os.path.getsize(g / "output/Record_fullwave.dat") == 4 * R * p["nxs"] * p["nys"]
+
+
+def test_qp3d_main_writes_s_field_snapshots(tmp_path, monkeypatch):
+    """QP3D_SNAP_S=1: the drop-in call also writes the S field of every snapshot step over the
+    padded grid and its three centre slices, under the names of the reference's commented-out
+    writer (3dPureqPModeling_single_shot.cu:381-409); compared with the oracle driver's files."""
+    monkeypatch.setenv("QP3D_SNAP_S", "1")
+    p = dict(nt=61, ntsnap=30, Nx0=30, Ny0=28, Nz0=24, sx0=14, sy0=13, nxs=1, nys=1, dxs=8, dys=6, hx=21, hy=19,
+             pml=10)
+    model = qp3d.genmodel(p["Nx0"], p["Ny0"], p["Nz0"], 6)
+    g, c = tmp_path / "gpu", tmp_path / "cpu"
+    os.makedirs(g / "model")
+    os.makedirs(g / "output")
+    for name, a in zip(("vp", "epsilon", "delta"), model):
+        a.tofile(g / "model" / f"{name}.dat")
+    (g / "parameter.txt").write_text(PAR.format(**p))
+    print("running qp3d_main with QP3D_SNAP_S=1", flush=True)
+    r = subprocess.run([MAIN, "parameter.txt"], cwd=g, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-500:], r.stderr[-500:], flush=True)
+    assert r.returncode == 0
+    oracle_run(str(c), p, model)
+    names = sorted(os.listdir(c / "output"))
+    assert sorted(os.listdir(g / "output")) == names
+    nx, ny, nz = p["hx"] + 2 * p["pml"], p["hy"] + 2 * p["pml"], p["Nz0"] + 2 * p["pml"]
+    sizes = {"S": nz * nx * ny, "Sxz": nz * nx, "Syz": nz * ny, "Sxy": nx * ny}
+    for k in (0, 30, 60):
+        for tag, n in sizes.items():
+            name = f"snapshot_fullwave_{tag}{k}.dat"
+            assert name in names
+            a = np.fromfile(g / "output" / name, np.float32)
+            b = np.fromfile(c / "output" / name, np.float32)
+            assert a.size == b.size == n, name
+            assert s_fields_agree(a, b), name
+    for n in names:
+        if "_S" not in n:
+            a = np.fromfile(g / "output" / n, np.float32)
+            b = np.fromfile(c / "output" / n, np.float32)
+            assert a.size == b.size and (rel_l2(a, b) <= TOL if np.any(b) else not np.any(a)), n

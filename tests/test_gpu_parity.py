@@ -7,7 +7,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import rel_l2
+from conftest import rel_l2, s_fields_agree
 
 import qp3d
 from oracle import oracle as O
@@ -261,3 +261,31 @@ def test_low_precision_storage_against_fp64(storage):
         assert np.array_equal(v.astype(np.float16).astype(np.float32), v)
     assert [k for k, _ in snaps] == [k for k, _ in s32] == [0, 100]
     assert rel_l2(snaps[1][1], s32[1][1]) <= 2 * LP_TOL[storage]
+
+
+def test_s_field_snapshots_against_the_oracle():
+    """S-field snapshots (snap_s): at each snapshot step k the scalar operator S the update of
+    step k used, over the padded grid, against the oracle's S of the same step; exactly 1
+    outside [5,n-5)^3 and 1 everywhere at step 0 (u = 0)."""
+    s = qp3d.config_setup(hx=37, hy=41, nz0=33, pml=10, nt=101, seed=3, src_depth=14)
+    say("S field snapshots")
+    with qp3d.Plan(s, snapflag=1, ntsnap=50, snap_s=1) as p:
+        p.forward()
+        got = dict(p.snapshots_s())
+        snaps = p.snapshots()
+    assert sorted(got) == [0, 50, 100] and [k for k, _ in snaps] == [0, 50, 100]
+    o = O.OracleState(s)
+    rim = np.ones((s.nz, s.nx, s.ny), bool)
+    rim[5:-5, 5:-5, 5:-5] = False
+    for k in range(s.nt):
+        o.step(s.wavelet[k])
+        if k in got:
+            Sg, So = got[k], o.field("S")
+            d = np.abs(Sg.astype(np.float64) - So)
+            say(f"step {k}: S < 1 at {(So < 1).sum()} cells, |dS| > 1e-4 at {(d > 1e-4).sum()} of {d.size}, "
+                f"max |dS| elsewhere {d[d <= 1e-4].max():.2e}")
+            assert np.all(Sg[rim] == 1.0)
+            assert s_fields_agree(Sg, So), k
+            if k == 0:
+                assert np.all(Sg == 1.0)
+    assert (got[100] < 0.999).sum() > 2000  # the anisotropic layers are reached
