@@ -177,7 +177,8 @@ struct GradArgs {
 // the two strips side by side, each with its own 5-cell halos (columns [0,37) and [37,74)).
 // Loads of u are exact up to 5 cells past the tile (the halo the stencil reads) and clamped
 // beyond; per-cell loads (model, previous wavefield, PML state) of lanes outside the tile are
-// clamped into the tile, so idle lanes add no memory traffic.
+// clamped into the tile, so idle lanes add no memory traffic. A non-folded tile may start its
+// lanes below y0 (at a 32-cell boundary); those lanes are outside the tile.
 //
 // Rows: a lane's rows r = 0..RX-1 sit at tile row tx + WX*r (interleaved across the wavefronts)
 // or, with RUN, at RX*tx + r (a run of consecutive rows per wavefront, so the x stencil of the
@@ -205,12 +206,16 @@ struct Lane {
     ty = threadIdx.x;
     tx = __builtin_amdgcn_readfirstlane(threadIdx.y);
     int yq;  // y for u loads
+    int ylo = y0;  // y of lane 0 (non-folded tiles)
     if (!fold) {
-      y = y0 + ty;
+      // Lane 0 sits on the 32-cell (128-byte) boundary at or below y0 when the tile still fits
+      // the wavefront from there, so each row a wavefront loads starts on a cache line.
+      if (y1 - (y0 & ~31) <= TY) ylo = y0 & ~31;
+      y = ylo + ty;
       Y = H + ty;
-      yon = y < y1;
+      yon = y >= y0 && y < y1;
       yq = min(y, y1 + H - 1);
-      yc = min(y, y1 - 1);
+      yc = min(max(y, y0), y1 - 1);
     } else {
       const int s = ty >> 5, j = ty & 31, w = y1 - y0;
       const int ys = s ? g.ny - g.pml : y0;
@@ -237,7 +242,7 @@ struct Lane {
     hy_on = t < T::TX * 2 * H;
     const int row = t / (2 * H), col = t - row * (2 * H);
     int gy;
-    if (!fold) gy = min(y0 - H + (col < H ? col : TY + col), y1 + H - 1);
+    if (!fold) gy = max(0, min(ylo - H + (col < H ? col : TY + col), y1 + H - 1));
     else gy = col < H ? y0 - H + col : g.ny - g.pml - H + (col - H);
     hyo = 4u * ((unsigned)min(x0 + row, x1 + H - 1) * g.ny + gy);
     hyL = hy_on ? (H + row) * T::LY + halo_col(col, fold) : T::LX * T::LY;

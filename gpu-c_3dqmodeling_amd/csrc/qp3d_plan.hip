@@ -68,6 +68,13 @@ void cut(std::vector<Seg> &out, int a, int b, int w)
   for (int s = a; s < b; s += w) out.push_back({s, std::min(s + w, b)});
 }
 
+// [a,b) cut at the multiples of w (pieces of at most w that start on a w-cell boundary, except
+// the first), so that the lanes of a 64-wide tile start on a cache line (see Lane).
+void cut_aligned(std::vector<Seg> &out, int a, int b, int w)
+{
+  for (int s = a; s < b; s = (s / w + 1) * w) out.push_back({s, std::min((s / w + 1) * w, b)});
+}
+
 int4 pack(int x0, int x1, int y0, int y1, int z0, int z1)
 {
   return make_int4(x0 | (x1 << 16), y0 | (y1 << 16), z0, z1);
@@ -146,7 +153,7 @@ void qp3d_items_uniform(const qp3d_plan *p, int tile, int capacity, std::vector<
   const int TX = tile_tx(tile);
   std::vector<Seg> xs, ys;
   cut(xs, H, c.nx - H, TX);
-  cut(ys, H, c.ny - H, TY);
+  cut_aligned(ys, H, c.ny - H, TY);
   const int len = c.nz - 2 * H;
   const int zc = choose_zchunk((int)(xs.size() * ys.size()), len, capacity, c.zchunk);
   for (int z = H; z < c.nz - H; z += zc)
@@ -178,12 +185,15 @@ void qp3d_items_update(const qp3d_plan *p, int tile_i, int tile_p, int cap_i, in
   };
   // interior kernel: its own x tiling of the interior; PML kernel: x tiling with its tile width
   // (the y faces with their own when they go to a launch of their own)
-  std::vector<Seg> xu, xs, xsy, ys;
+  std::vector<Seg> xu, xs, xsy, ys, ysa;
   std::vector<bool> xui, xi, xiy, yi;
   segs(xu, xui, c.nx, pml, tile_tx(tile_i));
   segs(xs, xi, c.nx, pml, tile_tx(tile_p));
   segs(xsy, xiy, c.nx, pml, tile_tx(ipy ? tile_py : tile_p));
   segs(ys, yi, c.ny, pml, TY);
+  // x-face tiles own every y (edges and corners included), so their y cuts need not follow the
+  // y faces: 64-cell pieces on 64-cell boundaries
+  cut_aligned(ysa, H, c.ny - H, TY);
   int nint_tiles = 0;
   for (size_t a = 0; a < xu.size(); a++)
     for (size_t b = 0; b < ys.size(); b++) nint_tiles += xui[a] && yi[b];
@@ -193,8 +203,8 @@ void qp3d_items_update(const qp3d_plan *p, int tile_i, int tile_p, int cap_i, in
   std::vector<std::pair<int4, bool>> ptiles, ytiles;  // PML xy tiles over the whole z range
   for (size_t a = 0; a < xs.size(); a++)
     if (!xi[a])
-      for (size_t b = 0; b < ys.size(); b++)
-        ptiles.push_back({pack(xs[a].a, xs[a].b, ys[b].a, ys[b].b, 0, 0), false});
+      for (size_t b = 0; b < ysa.size(); b++)
+        ptiles.push_back({pack(xs[a].a, xs[a].b, ysa[b].a, ysa[b].b, 0, 0), false});
   for (size_t a = 0; a < xsy.size(); a++) {
     if (!xiy[a]) continue;
     if (fold) {
