@@ -386,10 +386,11 @@ def main():
         plan.forward_async()
         plan.sync()
         log(f"warmup shot {i}: {time.perf_counter() - t:.3f} s")
-    # Kernel times come from HIP events around every launch of the timed shots. Replayed
-    # graphs carry no events, so with --graph the events come from one extra shot of direct
-    # launches after the timed region.
-    plan.set_instrument(not args.graph)
+    # The roofline kernel's time comes from HIP events around every interior-update launch of
+    # the timed shots (two events per step). The other launches' times come from one extra shot
+    # with events around every launch, after the timed region. Replayed graphs carry no events,
+    # so with --graph the interior time comes from that extra shot too.
+    plan.set_instrument(0 if args.graph else 2)
     plan.sync()
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -397,13 +398,15 @@ def main():
         plan.sync()  # also harvests this shot's kernel events
     t1 = time.perf_counter()
     wall = t1 - t0
-    if args.graph:
-        plan.set_instrument(True)
-        plan.forward_async()
-        plan.sync()
+    kt_timed = plan.kernel_times()
+    plan.set_instrument(1)
+    plan.forward_async()
+    plan.sync()
     kt = plan.kernel_times()
+    if not args.graph:
+        kt["update_interior"] = kt_timed["update_interior"]
     kb = plan.step_bytes()
-    plan.set_instrument(False)
+    plan.set_instrument(0)
     log(f"timed {args.steps} shots in {wall:.3f} s; per step: {kt}")
     gbs = {k: kb[k] / (kt[k] * 1e-3) / 1e9 for k in ("gradmax", "update_interior", "update_pml") if kt[k] > 0}
     log(f"bytes per step {kb}; achieved GB/s {gbs}")

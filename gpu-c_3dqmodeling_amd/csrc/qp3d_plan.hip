@@ -606,13 +606,15 @@ static int harvest_events(qp3d_plan *p)
     // e0..e1 gradient max; e1..e2 interior update; PML update ends at e3 and starts at e1
     // (concurrent, two streams) or e2 (one stream)
     float a = 0, b = 0, c = 0;
-    HIP_TRY(hipEventElapsedTime(&a, e[0], e[1]));
     HIP_TRY(hipEventElapsedTime(&b, e[1], e[2]));
-    HIP_TRY(hipEventElapsedTime(&c, p->two_streams ? e[1] : e[2], e[3]));
+    if (p->instrument == 1) {
+      HIP_TRY(hipEventElapsedTime(&a, e[0], e[1]));
+      HIP_TRY(hipEventElapsedTime(&c, p->two_streams ? e[1] : e[2], e[3]));
+    }
     p->t_grad += a;
     p->t_int += b;
     p->t_pml += c;
-    p->t_upd += p->two_streams ? std::max(b, c) : b + c;
+    if (p->instrument == 1) p->t_upd += p->two_streams ? std::max(b, c) : b + c;
     p->n_launch += 1;
   }
   p->ev_pending.clear();
@@ -640,7 +642,8 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare)
   }
   // u(k) is complete only when the previous step's PML update has finished too
   if (p->two_streams) HIP_TRY(hipStreamWaitEvent(p->stream, p->ev_pml, 0));
-  if (e) HIP_TRY(hipEventRecord(e[0], p->stream));
+  const bool all = p->instrument == 1;  // 2: events around the interior update only
+  if (e && all) HIP_TRY(hipEventRecord(e[0], p->stream));
   GradArgs<S> ga;
   UpdArgs<S> ua;
   make_args<S>(p, k & 1, st, ga, ua);
@@ -670,7 +673,7 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare)
     HIP_TRY(hipGetLastError());
   }
   if (e) {
-    HIP_TRY(hipEventRecord(e[3], ps));
+    if (all) HIP_TRY(hipEventRecord(e[3], ps));
     p->ev_pending.push_back(k);
   }
   // The rim source and the snapshot touch cells no PML tile writes (the outer 5 cells; the
@@ -961,7 +964,11 @@ extern "C" int qp3d_plan_step_maxima(const qp3d_plan *p, float *out)
 extern "C" int qp3d_plan_set_instrument(qp3d_plan *p, int on)
 {
   if (!p) return QP3D_ERR_ARG;
-  p->instrument = on != 0;
+  if (on < 0 || on > 2) {
+    qp3d_set_error("plan_set_instrument: mode %d outside [0,2]", on);
+    return QP3D_ERR_ARG;
+  }
+  p->instrument = on;
   p->t_grad = p->t_int = p->t_pml = p->t_upd = 0;
   p->n_launch = 0;
   p->ev_pending.clear();

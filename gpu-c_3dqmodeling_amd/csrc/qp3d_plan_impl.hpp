@@ -83,7 +83,7 @@ struct qp3d_plan {
   unsigned upload_gen = 0;  // counts uploads (objects built on the plan check it)
   int steps_done = 0;  // one past the last step enqueued since the last reset
 
-  bool instrument = false;
+  int instrument = 0;  // 1: events around every stencil launch; 2: around the interior update only
   std::vector<hipEvent_t> ev;   // 4 per step: before gradmax, after gradmax, after interior, after PML
   std::vector<int> ev_pending;  // steps with recorded events not yet harvested
   double t_grad = 0, t_int = 0, t_pml = 0, t_upd = 0;

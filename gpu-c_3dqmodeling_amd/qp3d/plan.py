@@ -107,6 +107,8 @@ class Plan:
         return out
 
     def set_instrument(self, on=True):
+        """True / 1: HIP events around every stencil launch; 2: around the interior update
+        only; False / 0: none. Resets the accumulated times."""
         check(self.lib.qp3d_plan_set_instrument(self.h, int(on)))
 
     def kernel_times(self):

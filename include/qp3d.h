@@ -149,12 +149,13 @@ int qp3d_plan_step_maxima(const qp3d_plan *p, float *out);
 int qp3d_work_items(const qp3d_config *cfg, int which, int capacity, int *out, int cap);
 /* Interior extents hx, hy, hz of the plan's grid (any output may be NULL). */
 int qp3d_plan_dims(const qp3d_plan *p, int *hx, int *hy, int *hz);
-/* 1: bracket every stencil launch with HIP events on the plan's stream (and reset the
- * accumulated times). */
+/* 1: bracket every stencil launch with HIP events on the plan's stream; 2: only the interior
+ * update (two events per step instead of four); 0: none. Resets the accumulated times. */
 int qp3d_plan_set_instrument(qp3d_plan *p, int on);
 /* Mean device time (ms) per time step over the instrumented steps: ms[0] gradient-max launch,
  * ms[1] interior update, ms[2] PML update, ms[3] the whole update stage (end of the gradient
- * maximum to the end of both updates; they overlap when streams == 2); ms[4] = steps counted. */
+ * maximum to the end of both updates; they overlap when streams == 2); ms[4] = steps counted.
+ * With instrumentation mode 2 only ms[1] and ms[4] are measured (the others read 0). */
 int qp3d_plan_kernel_times(qp3d_plan *p, double *ms);
 /* Algorithmic HBM bytes of one time step (interior cells 28 B, PML cells 68 B in FP32):
  * bytes[0] gradient-max launch (4 B per updated cell), bytes[1] interior update launch (24 B per
