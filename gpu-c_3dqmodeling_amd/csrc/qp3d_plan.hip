@@ -604,7 +604,7 @@ static int harvest_events(qp3d_plan *p)
   for (int k : p->ev_pending) {
     hipEvent_t *e = &p->ev[4 * (size_t)k];
     // e0..e1 gradient max; e1..e2 interior update; PML update ends at e3 and starts at e1
-    // (concurrent, two streams) or e2 (one stream)
+    // (concurrent, two streams) or e2 (one stream). Mode 2 records only e1 and e2.
     float a = 0, b = 0, c = 0;
     HIP_TRY(hipEventElapsedTime(&b, e[1], e[2]));
     if (p->instrument == 1) {
