@@ -19,6 +19,9 @@ struct qp3d_batch {
   hipStream_t stream = nullptr;
   // gradient max, interior update, PML update on x and z faces, on y faces (QP3D_TILES index)
   int tile[4] = {6, 0, 5, 1};
+  // each shot's own launches (its plan's work lists and tiles) one shot after another within a
+  // step, instead of one launch per stage for all shots; the autotuner sets it when faster
+  bool per_shot = false;
   WorkList wl[4];
   void *args_g[2] = {}, *args_u[2] = {};  // per-parity argument blocks of every shot
   int *kdev = nullptr;
@@ -122,6 +125,13 @@ static int batch_step_s(qp3d_batch *b, int k, Step st)
 {
   using T = typename S::T;
   const int par = k & 1;
+  if (b->per_shot) {
+    for (qp3d_plan *p : b->plans) {
+      const int rc = qp3d_forward_step_on(p, k, st, b->stream);
+      if (rc) return rc;
+    }
+    return QP3D_OK;
+  }
   for (int s = 0; s < 4; s++) {
     const bool tm = b->timing && s == b->timed_stage;
     if (tm) HIP_TRY(hipEventRecord(b->ev[0], b->stream));
@@ -329,9 +339,10 @@ extern "C" int qp3d_batch_sync(qp3d_batch *b)
 }
 
 // Times every tile shape for each stage over `steps` time steps from rest, keeps the fastest,
-// and resets the batch. tiles_out (4 ints, may be NULL) receives the choice (1-based, as in
-// qp3d_config.tile_*: gradient max, interior, PML x/z faces, PML y faces); ms_out (4 doubles,
-// may be NULL) the chosen stages' mean ms per step.
+// then times whole steps with one launch per stage against each shot's own launches and keeps
+// the faster, and resets the batch. tiles_out (4 ints, may be NULL) receives the tile choice
+// (1-based, as in qp3d_config.tile_*: gradient max, interior, PML x/z faces, PML y faces);
+// ms_out (4 doubles, may be NULL) the chosen stages' mean ms per step.
 extern "C" int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, double *ms_out)
 {
   if (!b || steps < 1 || steps + 2 > b->nt) {
@@ -340,6 +351,7 @@ extern "C" int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, dou
   }
   HIP_TRY(hipSetDevice(b->device));
   int rc;
+  b->per_shot = false;
   for (int s = 0; s < 4; s++) {
     double best = DBL_MAX;
     int bt = b->tile[s];
@@ -365,7 +377,54 @@ extern "C" int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, dou
     if (ms_out) ms_out[s] = best;
     if (tiles_out) tiles_out[s] = bt + 1;
   }
+  // whole steps, direct launches: batched stages against each shot's own launches
+  const bool graph = b->use_graph;
+  b->use_graph = false;
+  double t[2] = {0, 0};
+  for (int mode = 0; mode < 2 && rc == QP3D_OK; mode++) {
+    b->per_shot = mode == 1;
+    if ((rc = qp3d_batch_reset(b)) || (rc = qp3d_batch_run(b, 0, 2))) break;
+    float ms = 0;
+    if (hipEventRecord(b->ev[0], b->stream) != hipSuccess) rc = QP3D_ERR_HIP;
+    if (!rc) rc = qp3d_batch_run(b, 2, 2 + steps);
+    if (!rc && (hipEventRecord(b->ev[1], b->stream) != hipSuccess || hipEventSynchronize(b->ev[1]) != hipSuccess ||
+                hipEventElapsedTime(&ms, b->ev[0], b->ev[1]) != hipSuccess))
+      rc = QP3D_ERR_HIP;
+    t[mode] = ms;
+  }
+  b->use_graph = graph;
+  if (rc) {
+    b->per_shot = false;
+    if (rc == QP3D_ERR_HIP) qp3d_set_error("batch_autotune: timing the step modes failed");
+    return rc;
+  }
+  b->per_shot = t[1] < t[0];
+  // a graph captured before holds the other mode's launches
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
   return qp3d_batch_reset(b);
+}
+
+extern "C" int qp3d_batch_per_shot(const qp3d_batch *b) { return b ? (b->per_shot ? 1 : 0) : QP3D_ERR_ARG; }
+
+extern "C" int qp3d_batch_set_per_shot(qp3d_batch *b, int on)
+{
+  if (!b || on < 0 || on > 1) {
+    qp3d_set_error("batch_set_per_shot: bad arguments");
+    return QP3D_ERR_ARG;
+  }
+  if (b->per_shot != (on == 1)) {
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->per_shot = on == 1;
+  }
+  return QP3D_OK;
 }
 
 // Algorithmic HBM bytes of one batched time step (every shot), as qp3d_plan_step_bytes sums.

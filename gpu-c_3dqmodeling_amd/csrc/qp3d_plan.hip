@@ -626,8 +626,12 @@ static int harvest_events(qp3d_plan *p)
 // A bare step (graph capture, recomputation) leaves out the snapshot and the instrumentation
 // events.
 template <class S>
-static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare)
+static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on)
 {
+  // on: launch everything on this stream (and without the plan's second stream) instead of on
+  // the plan's own streams
+  hipStream_t const s0 = on ? on : p->stream;
+  const bool two = p->two_streams && !on;
   using T = typename S::T;
   const qp3d_config &cf = p->cfg;
   const int prev = k & 1, cur = prev ^ 1;  // u[prev] holds u(k-1) and receives u(k+1)
@@ -641,27 +645,27 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare)
     e = &p->ev[4 * (size_t)k];
   }
   // u(k) is complete only when the previous step's PML update has finished too
-  if (p->two_streams) HIP_TRY(hipStreamWaitEvent(p->stream, p->ev_pml, 0));
+  if (two) HIP_TRY(hipStreamWaitEvent(s0, p->ev_pml, 0));
   const bool all = p->instrument == 1;  // 2: events around the interior update only
-  if (e && all) HIP_TRY(hipEventRecord(e[0], p->stream));
+  if (e && all) HIP_TRY(hipEventRecord(e[0], s0));
   GradArgs<S> ga;
   UpdArgs<S> ua;
   make_args<S>(p, k & 1, st, ga, ua);
-  launch_gradmax(p->tile_g, p->wl_grad, p->stream, ga);
+  launch_gradmax(p->tile_g, p->wl_grad, s0, ga);
   HIP_TRY(hipGetLastError());
-  if (e) HIP_TRY(hipEventRecord(e[1], p->stream));
-  if (p->two_streams) {
-    HIP_TRY(hipEventRecord(p->ev_grad, p->stream));
+  if (e) HIP_TRY(hipEventRecord(e[1], s0));
+  if (two) {
+    HIP_TRY(hipEventRecord(p->ev_grad, s0));
     HIP_TRY(hipStreamWaitEvent(p->stream2, p->ev_grad, 0));
   }
 
   if (p->wl_int.n) {
     ua.work = p->wl_int.d;
-    launch_update<S, false>(p->tile_u, p->wl_int, p->stream, ua);
+    launch_update<S, false>(p->tile_u, p->wl_int, s0, ua);
     HIP_TRY(hipGetLastError());
   }
-  if (e) HIP_TRY(hipEventRecord(e[2], p->stream));
-  hipStream_t ps = p->two_streams ? p->stream2 : p->stream;
+  if (e) HIP_TRY(hipEventRecord(e[2], s0));
+  hipStream_t ps = two ? p->stream2 : s0;
   if (p->wl_pml.n) {
     ua.work = p->wl_pml.d;
     launch_update<S, true>(p->tile_p, p->wl_pml, ps, ua);
@@ -678,11 +682,11 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare)
   }
   // The rim source and the snapshot touch cells no PML tile writes (the outer 5 cells; the
   // interior cube), so they follow the interior update on the main stream.
-  if (p->two_streams) HIP_TRY(hipEventRecord(p->ev_pml, p->stream2));
+  if (two) HIP_TRY(hipEventRecord(p->ev_pml, p->stream2));
   if (p->src_rim) {
     const long nb = (long)(p->sb[1] - p->sb[0] + 1) * (p->sb[3] - p->sb[2] + 1) * (p->sb[5] - p->sb[4] + 1);
     const int blocks = (int)std::min<long>((nb + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_rim_source<S>, dim3(blocks), dim3(256), 0, p->stream, (const T *)p->u[cur],
+    hipLaunchKernelGGL(k_rim_source<S>, dim3(blocks), dim3(256), 0, s0, (const T *)p->u[cur],
                        (T *)p->u[prev], p->geo, p->sb[0], p->sb[1], p->sb[2], p->sb[3], p->sb[4], p->sb[5],
                        (const float *)p->sbox, (const float *)p->wav, st);
     HIP_TRY(hipGetLastError());
@@ -690,13 +694,13 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare)
   if (!bare && cf.snapflag == 1 && k % cf.ntsnap == 0) {
     const int si = k / cf.ntsnap;
     if (si < (int)p->snaps.size()) {
-      hipLaunchKernelGGL(k_snapshot<S>, dim3((p->hy + 255) / 256, p->hx, p->hz), dim3(256), 0, p->stream,
+      hipLaunchKernelGGL(k_snapshot<S>, dim3((p->hy + 255) / 256, p->hx, p->hz), dim3(256), 0, s0,
                          (const T *)p->u[prev], p->snaps[si], p->geo);
       HIP_TRY(hipGetLastError());
       if (si < (int)p->snaps_s.size()) {
         // S of this step: from u(k) (slot cur, which this step does not write) and G(k)
         const int blocks = (int)std::min<long>((p->N + 255) / 256, 8192);
-        hipLaunchKernelGGL(k_sfield<S>, dim3(blocks), dim3(256), 0, p->stream, (const T *)p->u[cur],
+        hipLaunchKernelGGL(k_sfield<S>, dim3(blocks), dim3(256), 0, s0, (const T *)p->u[cur],
                            (const float *)p->eps, (const float *)p->epsd, (const unsigned *)p->gmax, st, p->geo,
                            p->coef, p->snaps_s[si]);
         HIP_TRY(hipGetLastError());
@@ -707,14 +711,16 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare)
   return QP3D_OK;
 }
 
-static int launch_step(qp3d_plan *p, int k, Step st, bool bare)
+static int launch_step(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on = nullptr)
 {
   int rc = QP3D_OK;
-  with_storage(p->cfg.storage, [&](auto sp) { rc = launch_step_s<decltype(sp)>(p, k, st, bare); });
+  with_storage(p->cfg.storage, [&](auto sp) { rc = launch_step_s<decltype(sp)>(p, k, st, bare, on); });
   return rc;
 }
 
 int qp3d_forward_step(qp3d_plan *p, int k) { return launch_step(p, k, Step{nullptr, k}, true); }
+
+int qp3d_forward_step_on(qp3d_plan *p, int k, Step st, hipStream_t on) { return launch_step(p, k, st, true, on); }
 
 static bool snapshot_in(const qp3d_plan *p, int k0, int k1)
 {

@@ -170,3 +170,5 @@ int qp3d_upload_items(WorkList &w, const std::vector<int4> &logical, bool batch)
 int qp3d_cu_count(const qp3d_plan *p);
 // Enqueues forward time step k with direct launches (no capture, no snapshot, no events).
 int qp3d_forward_step(qp3d_plan *p, int k);
+// The same on stream `on` (one stream), with the step index the kernels read given by `st`.
+int qp3d_forward_step_on(qp3d_plan *p, int k, Step st, hipStream_t on);

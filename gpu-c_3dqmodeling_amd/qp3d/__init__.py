@@ -6,9 +6,9 @@
 
 qp3d.autograd (imported separately, it needs torch) holds the torch.autograd shim.
 """
-from ._lib import EXPORTED, LIB_PATH, Qp3dError, load
+from ._lib import EXPORTED, LIB_PATH, NTILES, Qp3dError, load
 from .plan import Batch, Fwi, Plan, modeling3d
 from .setup import Setup, build_setup, config_setup, fd_tables, file_to_grid, genmodel, window_setup
 
-__all__ = ["EXPORTED", "LIB_PATH", "Qp3dError", "load", "Batch", "Fwi", "Plan", "modeling3d", "Setup", "build_setup",
+__all__ = ["EXPORTED", "LIB_PATH", "NTILES", "Qp3dError", "load", "Batch", "Fwi", "Plan", "modeling3d", "Setup", "build_setup",
            "config_setup", "fd_tables", "file_to_grid", "genmodel", "window_setup"]

@@ -21,6 +21,7 @@ QP3D_ERR_NONFINITE = -5
 QP3D_ERR_STATE = -6
 
 STORAGE = {"fp32": 0, "bf16": 1, "fp16": 2}
+NTILES = 7  # launchable tile shapes (QP3D_NTILES in include/qp3d.h)
 
 _F = C.POINTER(C.c_float)
 
@@ -85,6 +86,8 @@ _SIGS = [
     ("qp3d_batch_sync", C.c_int, [C.c_void_p]),
     ("qp3d_batch_autotune", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     ("qp3d_batch_step_bytes", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    ("qp3d_batch_per_shot", C.c_int, [C.c_void_p]),
+    ("qp3d_batch_set_per_shot", C.c_int, [C.c_void_p, C.c_int]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]

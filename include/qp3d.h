@@ -196,9 +196,15 @@ int qp3d_batch_destroy(qp3d_batch *b);
 int qp3d_batch_reset(qp3d_batch *b);
 int qp3d_batch_run(qp3d_batch *b, int k0, int k1);
 int qp3d_batch_sync(qp3d_batch *b);
-/* Picks each stage's tile shape by timing every candidate over `steps` steps, then resets;
+/* Picks each stage's tile shape by timing every candidate over `steps` steps, then whether a
+ * step goes out as one launch per stage for all shots or as each shot's own launches one after
+ * another (faster when every shot fills the GPU by itself), then resets;
  * tiles_out[4] / ms_out[4] (may be NULL): gradient max, interior, PML x/z faces, PML y faces. */
 int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, double *ms_out);
+/* 1 when the batch runs each shot's own launches (chosen by qp3d_batch_autotune), 0 when it
+ * runs one launch per stage for all shots (the default); the setter forces either. */
+int qp3d_batch_per_shot(const qp3d_batch *b);
+int qp3d_batch_set_per_shot(qp3d_batch *b, int on);
 /* bytes[0] gradient-max bytes, bytes[1] update bytes per step (all shots), bytes[2] updated
  * cells, bytes[3] interior cells. */
 int qp3d_batch_step_bytes(const qp3d_batch *b, double *bytes);

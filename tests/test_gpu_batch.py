@@ -1,7 +1,8 @@
 """Batched shots: one launch per stage for several shots with uneven windows must give the
 same records, state and per-step maxima as running each shot's plan alone (bit for bit: the
 per-cell arithmetic does not depend on how cells are grouped into work items), with direct
-launches, with the captured graph, after the autotuner, and with 16-bit storage."""
+launches, with the captured graph, with each shot's own launches, after the autotuner, and
+with 16-bit storage."""
 import sys
 
 import numpy as np
@@ -58,10 +59,13 @@ def test_batch_equals_shots_alone(storage):
             say(f"batch {storage} graph={graph}")
             with qp3d.Batch(plans, use_graph=graph) as b:
                 same(together(b, plans), ref)
+                b.set_per_shot(True)
+                same(together(b, plans), ref)
+                b.set_per_shot(False)
                 if not graph:
                     t = b.autotune(steps=3)
                     say(f"autotuned {t}")
-                    assert all(1 <= x <= 6 for x in t["tiles"])
+                    assert all(1 <= x <= qp3d.NTILES for x in t["tiles"])
                     same(together(b, plans), ref)
     finally:
         for p in plans:

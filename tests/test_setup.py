@@ -163,6 +163,11 @@ def header_symbols():
     return sorted(set(re.findall(r"\b(modeling3d|qp3d_[a-z0-9_]+)\s*\(", text)))
 
 
+def test_tile_count_matches_the_header():
+    text = open(os.path.join(REPO, "include", "qp3d.h")).read()
+    assert int(re.search(r"#define QP3D_NTILES (\d+)", text).group(1)) == qp3d.NTILES
+
+
 def test_library_exports_every_declared_symbol():
     lib = C.CDLL(qp3d.LIB_PATH)
     syms = header_symbols()
