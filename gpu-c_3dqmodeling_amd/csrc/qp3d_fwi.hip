@@ -273,7 +273,8 @@ extern "C" int qp3d_fwi_gradient(qp3d_fwi *f, const float *resid, double *g_vp, 
   const qp3d_config &c = p->cfg;
   const size_t N = (size_t)p->N;
   HIP_TRY(hipSetDevice(c.device));
-  HIP_TRY(hipMemcpyAsync(f->resid, resid, sizeof(float) * (size_t)c.nt * p->hx * p->hy, hipMemcpyHostToDevice,
+  // resid and the gradient arrays may be host or device memory
+  HIP_TRY(hipMemcpyAsync(f->resid, resid, sizeof(float) * (size_t)c.nt * p->hx * p->hy, hipMemcpyDefault,
                          p->stream));
   HIP_TRY(hipMemsetAsync(f->adj, 0, sizeof(float) * 9 * N, p->stream));
   HIP_TRY(hipMemsetAsync(f->gsum, 0, sizeof(double) * 3 * N, p->stream));
@@ -310,9 +311,9 @@ extern "C" int qp3d_fwi_gradient(qp3d_fwi *f, const float *resid, double *g_vp, 
   hipLaunchKernelGGL(k_grad_finish, dim3(grid_of((long)N)), dim3(256), 0, p->stream, f->gsum,
                      (const float *)p->vdt2, c.dt, (long)N);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(g_vp, f->gsum, sizeof(double) * N, hipMemcpyDeviceToHost, p->stream));
-  HIP_TRY(hipMemcpyAsync(g_eps, f->gsum + N, sizeof(double) * N, hipMemcpyDeviceToHost, p->stream));
-  HIP_TRY(hipMemcpyAsync(g_del, f->gsum + 2 * N, sizeof(double) * N, hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipMemcpyAsync(g_vp, f->gsum, sizeof(double) * N, hipMemcpyDefault, p->stream));
+  HIP_TRY(hipMemcpyAsync(g_eps, f->gsum + N, sizeof(double) * N, hipMemcpyDefault, p->stream));
+  HIP_TRY(hipMemcpyAsync(g_del, f->gsum + 2 * N, sizeof(double) * N, hipMemcpyDefault, p->stream));
   HIP_TRY(hipStreamSynchronize(p->stream));
   f->ms_recompute = ms_re;
   f->ms_adjoint = ms_adj;

@@ -472,6 +472,61 @@ extern "C" int qp3d_plan_destroy(qp3d_plan *p)
   return QP3D_OK;
 }
 
+// Whether ptr is device memory (arrays handed to the plan may live on the host or the device).
+static bool on_device(const void *ptr)
+{
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, ptr) != hipSuccess) {
+    (void)hipGetLastError();  // memory the runtime does not know: plain host memory
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice;
+}
+
+// In place over the three model arrays, holding vp, eps and del on entry:
+// (vp dt)^2 in double rounded to float, eps, eps - del (the arithmetic of the host setup).
+static __global__ void k_model(float *v2, const float *eps, float *ed, float dt, long n)
+{
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double v = (double)v2[i] * dt;
+    v2[i] = (float)(v * v);
+    ed[i] = eps[i] - ed[i];
+  }
+}
+
+// Bounding box {z0, z1, x0, x1, y0, y1} of the non-zero cells of a source array in device
+// memory; box starts at {nz, -1, nx, -1, ny, -1}.
+static __global__ void k_source_box(const float *src, int nx, int ny, int nz, int *box)
+{
+  const long pln = (long)nx * ny, n = pln * nz;
+  int b[6] = {nz, -1, nx, -1, ny, -1};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    if (src[i] != 0.0f) {
+      const int z = (int)(i / pln), x = (int)((i - z * pln) / ny), y = (int)(i - z * pln - (long)x * ny);
+      b[0] = min(b[0], z); b[1] = max(b[1], z);
+      b[2] = min(b[2], x); b[3] = max(b[3], x);
+      b[4] = min(b[4], y); b[5] = max(b[5], y);
+    }
+  if (b[1] >= 0) {
+    atomicMin(&box[0], b[0]); atomicMax(&box[1], b[1]);
+    atomicMin(&box[2], b[2]); atomicMax(&box[3], b[3]);
+    atomicMin(&box[4], b[4]); atomicMax(&box[5], b[5]);
+  }
+}
+
+static __global__ void k_scale(float *a, long n, float f)
+{
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) a[i] *= f;
+}
+
+// n floats from host or device memory into a host vector.
+static int to_host(const float *src, size_t n, std::vector<float> &v)
+{
+  v.resize(n);
+  if (n) HIP_TRY(hipMemcpy(v.data(), src, sizeof(float) * n, hipMemcpyDefault));  // lands before return
+  return QP3D_OK;
+}
+
 extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps, const float *del,
                                 const float *source, const float *wavelet, const float *dlr, const float *ddlr,
                                 const float *dtb, const float *ddtb, const float *dfb, const float *ddfb,
@@ -486,17 +541,25 @@ extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps,
   HIP_TRY(hipSetDevice(cf.device));
   HIP_TRY(hipStreamSynchronize(p->stream));
   const long N = p->N;
-  // model: (vp*dt)^2, epsilon, epsilon-delta
-  std::vector<float> h(3 * (size_t)N);
-  for (long i = 0; i < N; i++) {
-    const double v = (double)vp[i] * cf.dt;
-    h[i] = (float)(v * v);
-    h[N + i] = eps[i];
-    h[2 * N + i] = eps[i] - del[i];
-  }
-  HIP_TRY(hipMemcpy(p->vdt2, h.data(), sizeof(float) * (size_t)N, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(p->eps, h.data() + N, sizeof(float) * (size_t)N, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(p->epsd, h.data() + 2 * N, sizeof(float) * (size_t)N, hipMemcpyHostToDevice));
+  // Every array may be host or device memory. Model: vp, eps, del copied into the plan's model
+  // arrays, then (vp*dt)^2, epsilon, epsilon-delta formed in place.
+  // (copies on the plan's stream: a device-to-device hipMemcpy may return before it lands)
+  HIP_TRY(hipMemcpyAsync(p->vdt2, vp, sizeof(float) * (size_t)N, hipMemcpyDefault, p->stream));
+  HIP_TRY(hipMemcpyAsync(p->eps, eps, sizeof(float) * (size_t)N, hipMemcpyDefault, p->stream));
+  HIP_TRY(hipMemcpyAsync(p->epsd, del, sizeof(float) * (size_t)N, hipMemcpyDefault, p->stream));
+  hipLaunchKernelGGL(k_model, dim3((unsigned)std::min<long>((N + 255) / 256, 8192)), dim3(256), 0, p->stream, p->vdt2,
+                     (const float *)p->eps, p->epsd, cf.dt, N);
+  HIP_TRY(hipGetLastError());
+  std::vector<float> hw, hdlr, hddlr, hdtb, hddtb, hdfb, hddfb, hc, hc2;
+  int rc;
+  if ((rc = to_host(wavelet, (size_t)cf.nt, hw)) || (rc = to_host(dlr, (size_t)cf.pml, hdlr)) ||
+      (rc = to_host(ddlr, (size_t)cf.pml, hddlr)) || (rc = to_host(dtb, (size_t)cf.pml, hdtb)) ||
+      (rc = to_host(ddtb, (size_t)cf.pml, hddtb)) || (rc = to_host(dfb, (size_t)cf.pml, hdfb)) ||
+      (rc = to_host(ddfb, (size_t)cf.pml, hddfb)) || (rc = to_host(c, 36, hc)) || (rc = to_host(c2, 36, hc2)))
+    return rc;
+  wavelet = hw.data();
+  dlr = hdlr.data(); ddlr = hddlr.data(); dtb = hdtb.data(); ddtb = hddtb.data(); dfb = hdfb.data(); ddfb = hddfb.data();
+  c = hc.data(); c2 = hc2.data();
   // PML coefficient table per face family (x: dlr, y: dfb, z: dtb) and layer j:
   //   K1 = (1+d dt)^2 - 3, K2 = 2 d dt - 1, K3 = 1 - dt d, E = d'   (GPU_kernel.cu:568-576)
   std::vector<float4> tab(3 * (size_t)cf.pml);
@@ -511,7 +574,7 @@ extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps,
       t.w = de[f][j];
       tab[(size_t)f * cf.pml + j] = t;
     }
-  HIP_TRY(hipMemcpy(p->pmltab, tab.data(), sizeof(float4) * tab.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpyAsync(p->pmltab, tab.data(), sizeof(float4) * tab.size(), hipMemcpyHostToDevice, p->stream));
   // FD coefficients: the 10th-order rows of the 6x6 tables (CPU_function.c:123-127,143-147)
   Coef &k = p->coef;
   for (int m = 1; m <= H; m++) {
@@ -531,16 +594,36 @@ extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps,
   k.dt = cf.dt;
   // source: bounding box of the non-zero cells of the full-grid spatial source
   int z0 = cf.nz, z1 = -1, x0 = cf.nx, x1 = -1, y0 = cf.ny, y1 = -1;
-  for (int z = 0; z < cf.nz; z++)
-    for (int x = 0; x < cf.nx; x++) {
-      const float *row = source + (long)z * cf.nx * cf.ny + (long)x * cf.ny;
-      for (int y = 0; y < cf.ny; y++)
-        if (row[y] != 0.0f) {
-          z0 = std::min(z0, z); z1 = std::max(z1, z);
-          x0 = std::min(x0, x); x1 = std::max(x1, x);
-          y0 = std::min(y0, y); y1 = std::max(y1, y);
-        }
+  const bool src_dev = on_device(source);
+  if (src_dev) {
+    int box[6] = {cf.nz, -1, cf.nx, -1, cf.ny, -1}, *dbox = nullptr;
+    HIP_TRY(hipMalloc((void **)&dbox, sizeof box));
+    hipError_t e = hipMemcpyAsync(dbox, box, sizeof box, hipMemcpyHostToDevice, p->stream);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_source_box, dim3((unsigned)std::min<long>((N + 255) / 256, 8192)), dim3(256), 0, p->stream,
+                         source, cf.nx, cf.ny, cf.nz, dbox);
+      e = hipGetLastError();
     }
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e == hipSuccess) e = hipMemcpy(box, dbox, sizeof box, hipMemcpyDeviceToHost);
+    (void)hipFree(dbox);
+    if (e != hipSuccess) {
+      qp3d_set_error("plan_upload: source box on the device failed: %s", hipGetErrorString(e));
+      return QP3D_ERR_HIP;
+    }
+    z0 = box[0]; z1 = box[1]; x0 = box[2]; x1 = box[3]; y0 = box[4]; y1 = box[5];
+  } else {
+    for (int z = 0; z < cf.nz; z++)
+      for (int x = 0; x < cf.nx; x++) {
+        const float *row = source + (long)z * cf.nx * cf.ny + (long)x * cf.ny;
+        for (int y = 0; y < cf.ny; y++)
+          if (row[y] != 0.0f) {
+            z0 = std::min(z0, z); z1 = std::max(z1, z);
+            x0 = std::min(x0, x); x1 = std::max(x1, x);
+            y0 = std::min(y0, y); y1 = std::max(y1, y);
+          }
+      }
+  }
   (void)hipFree(p->sbox);
   p->sbox = nullptr;
   p->src_on = z1 >= 0;
@@ -549,10 +632,14 @@ extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps,
   if (p->src_on) {
     const int bz = z1 - z0 + 1, bx = x1 - x0 + 1, by = y1 - y0 + 1;
     std::vector<float> b((size_t)bz * bx * by);
-    for (int z = 0; z < bz; z++)
-      for (int x = 0; x < bx; x++)
-        memcpy(&b[((size_t)z * bx + x) * by], source + (long)(z + z0) * cf.nx * cf.ny + (long)(x + x0) * cf.ny + y0,
-               sizeof(float) * by);
+    for (int z = 0; z < bz; z++) {
+      const float *s0 = source + (long)(z + z0) * cf.nx * cf.ny + (long)x0 * cf.ny + y0;
+      if (src_dev)
+        HIP_TRY(hipMemcpy2D(&b[(size_t)z * bx * by], sizeof(float) * by, s0, sizeof(float) * cf.ny, sizeof(float) * by, bx,
+                            hipMemcpyDeviceToHost));
+      else
+        for (int x = 0; x < bx; x++) memcpy(&b[((size_t)z * bx + x) * by], s0 + (long)x * cf.ny, sizeof(float) * by);
+    }
     if (cf.storage == QP3D_STORAGE_FP16) {
       // Half storage: scale the fields (through the source) so that the peak injected value
       // sits at 2^7. The wavefield peaks near 8x the peak of source*wavelet and the PML state
@@ -565,11 +652,14 @@ extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps,
       for (float &v : b) v = std::ldexp(v, p->scale_log2);
     }
     HIP_TRY(hipMalloc((void **)&p->sbox, sizeof(float) * b.size()));
-    HIP_TRY(hipMemcpy(p->sbox, b.data(), sizeof(float) * b.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(p->sbox, b.data(), sizeof(float) * b.size(), hipMemcpyHostToDevice, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));  // b is about to go out of scope
     p->sb[0] = z0; p->sb[1] = z1; p->sb[2] = x0; p->sb[3] = x1; p->sb[4] = y0; p->sb[5] = y1;
     p->src_rim = z0 < H || x0 < H || y0 < H || z1 >= cf.nz - H || x1 >= cf.nx - H || y1 >= cf.ny - H;
   }
-  if (cf.nt > 0) HIP_TRY(hipMemcpy(p->wav, wavelet, sizeof(float) * cf.nt, hipMemcpyHostToDevice));
+  if (cf.nt > 0) HIP_TRY(hipMemcpyAsync(p->wav, wavelet, sizeof(float) * cf.nt, hipMemcpyHostToDevice, p->stream));
+  // the copies and the model kernel are on the plan's stream; done before the host vectors go
+  HIP_TRY(hipStreamSynchronize(p->stream));
   // a captured loop holds the source box pointer and bounds: capture again on the next run
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
@@ -824,6 +914,26 @@ static void unscale(const qp3d_plan *p, float *a, size_t n)
   for (size_t i = 0; i < n; i++) a[i] *= f;
 }
 
+// n floats of plan data (device) into out, host or device memory, with the FP16 storage scale
+// taken out.
+static int copy_out(const qp3d_plan *p, float *out, const float *src, size_t n)
+{
+  if (!on_device(out)) {
+    HIP_TRY(hipMemcpy(out, src, sizeof(float) * n, hipMemcpyDeviceToHost));
+    unscale(p, out, n);
+    return QP3D_OK;
+  }
+  // on the plan's stream, so that the scaling follows the copy, and complete on return
+  HIP_TRY(hipMemcpyAsync(out, src, sizeof(float) * n, hipMemcpyDeviceToDevice, p->stream));
+  if (p->scale_log2) {
+    hipLaunchKernelGGL(k_scale, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)), dim3(256), 0, p->stream, out,
+                       (long)n, std::ldexp(1.0f, -p->scale_log2));
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return QP3D_OK;
+}
+
 extern "C" int qp3d_plan_record(const qp3d_plan *p, float *record)
 {
   if (!p || !record) {
@@ -833,9 +943,7 @@ extern "C" int qp3d_plan_record(const qp3d_plan *p, float *record)
   HIP_TRY(hipSetDevice(p->cfg.device));
   HIP_TRY(hipStreamSynchronize(p->stream));
   const size_t n = (size_t)p->cfg.nt * p->hx * p->hy;
-  if (p->rec) HIP_TRY(hipMemcpy(record, p->rec, sizeof(float) * n, hipMemcpyDeviceToHost));
-  unscale(p, record, n);
-  return QP3D_OK;
+  return p->rec ? copy_out(p, record, p->rec, n) : QP3D_OK;
 }
 
 // After the step k the newest wavefield sits in slot k&1 (launch_step writes u(k+1) over u(k-1)).
@@ -860,9 +968,12 @@ extern "C" int qp3d_plan_field(const qp3d_plan *p, int which, float *out)
   case 7: src = p->pp[older]; break;
   case 8: src = p->w21; break;
   }
-  if (p->esz == 4) {
-    HIP_TRY(hipMemcpy(out, src, sizeof(float) * (size_t)p->N, hipMemcpyDeviceToHost));
-  } else {
+  if (p->esz == 4) return copy_out(p, out, (const float *)src, (size_t)p->N);
+  const bool dev = on_device(out);
+  std::vector<float> hout(dev ? (size_t)p->N : 0);
+  float *const dst = out;
+  if (dev) out = hout.data();
+  {
     std::vector<unsigned short> h((size_t)p->N);
     HIP_TRY(hipMemcpy(h.data(), src, sizeof(unsigned short) * (size_t)p->N, hipMemcpyDeviceToHost));
     if (p->cfg.storage == QP3D_STORAGE_BF16)
@@ -878,6 +989,7 @@ extern "C" int qp3d_plan_field(const qp3d_plan *p, int which, float *out)
       }
   }
   unscale(p, out, (size_t)p->N);
+  if (dev) HIP_TRY(hipMemcpy(dst, out, sizeof(float) * (size_t)p->N, hipMemcpyHostToDevice));
   return QP3D_OK;
 }
 
@@ -892,10 +1004,7 @@ extern "C" int qp3d_plan_snapshot(const qp3d_plan *p, int i, int *step, float *o
   HIP_TRY(hipSetDevice(p->cfg.device));
   HIP_TRY(hipStreamSynchronize(p->stream));
   if (step) *step = p->snap_steps[i];
-  const size_t n = (size_t)p->hx * p->hy * p->hz;
-  HIP_TRY(hipMemcpy(out, p->snaps[i], sizeof(float) * n, hipMemcpyDeviceToHost));
-  unscale(p, out, n);
-  return QP3D_OK;
+  return copy_out(p, out, p->snaps[i], (size_t)p->hx * p->hy * p->hz);
 }
 
 extern "C" int qp3d_plan_snapshot_s(const qp3d_plan *p, int i, int *step, float *out)
@@ -908,7 +1017,8 @@ extern "C" int qp3d_plan_snapshot_s(const qp3d_plan *p, int i, int *step, float 
   HIP_TRY(hipSetDevice(p->cfg.device));
   HIP_TRY(hipStreamSynchronize(p->stream));
   if (step) *step = p->snap_steps[i];
-  HIP_TRY(hipMemcpy(out, p->snaps_s[i], sizeof(float) * (size_t)p->N, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpyAsync(out, p->snaps_s[i], sizeof(float) * (size_t)p->N, hipMemcpyDefault, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
   return QP3D_OK;
 }
 
@@ -962,9 +1072,7 @@ extern "C" int qp3d_plan_step_maxima(const qp3d_plan *p, float *out)
   }
   HIP_TRY(hipSetDevice(p->cfg.device));
   HIP_TRY(hipStreamSynchronize(p->stream));
-  HIP_TRY(hipMemcpy(out, p->gmax, sizeof(float) * (size_t)p->cfg.nt, hipMemcpyDeviceToHost));
-  unscale(p, out, (size_t)p->cfg.nt);
-  return QP3D_OK;
+  return copy_out(p, out, (const float *)p->gmax, (size_t)p->cfg.nt);
 }
 
 extern "C" int qp3d_plan_set_instrument(qp3d_plan *p, int on)

@@ -113,6 +113,9 @@ typedef struct qp3d_config {
 
 typedef struct qp3d_plan qp3d_plan;
 
+/* Arrays handed to the plan, FWI and batch calls below may be host memory or device memory of
+ * the plan's device (the kind is read from each pointer); outputs are written where they point.
+ * The modeling3d() entry point above takes host memory, as the reference's does. */
 int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out);
 int qp3d_plan_destroy(qp3d_plan *p);
 /* Uploads model, source and tables (same meaning as the modeling3d() arguments). */
@@ -125,7 +128,7 @@ int qp3d_plan_reset(qp3d_plan *p);
 int qp3d_plan_run(qp3d_plan *p, int k0, int k1);
 /* Waits for the plan's stream; returns QP3D_ERR_NONFINITE if the field blew up. */
 int qp3d_plan_sync(qp3d_plan *p);
-/* Copies the device record [nt][hx][hy] to host memory. */
+/* Copies the device record [nt][hx][hy] to `record` (host or device memory). */
 int qp3d_plan_record(const qp3d_plan *p, float *record);
 /* State after the last step run, padded [nz][nx][ny] floats. which: 0 = u after the last
  * step (u2), 1 = the step before (u1); 2/3 = w1 newest/older, 4/5 = w3 newest/older,
@@ -178,7 +181,7 @@ int qp3d_fwi_create(qp3d_plan *p, int interval, int ckpt_storage, qp3d_fwi **out
 int qp3d_fwi_destroy(qp3d_fwi *f);
 /* Forward run from rest with checkpoints; record, maxima and state as after qp3d_plan_run. */
 int qp3d_fwi_forward(qp3d_fwi *f);
-/* resid = record - observed, host [nt][hx][hy]; outputs host [nz][nx][ny] each. */
+/* resid = record - observed, [nt][hx][hy]; outputs [nz][nx][ny] each (host or device memory). */
 int qp3d_fwi_gradient(qp3d_fwi *f, const float *resid, double *g_vp, double *g_eps, double *g_del);
 /* ms[0] checkpointed forward run, ms[1] recomputation, ms[2] adjoint sweep (device time). */
 int qp3d_fwi_times(const qp3d_fwi *f, double *ms);

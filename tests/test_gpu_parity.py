@@ -289,3 +289,80 @@ def test_s_field_snapshots_against_the_oracle():
             if k == 0:
                 assert np.all(Sg == 1.0)
     assert (got[100] < 0.999).sum() > 2000  # the anisotropic layers are reached
+
+
+def _hip():
+    import ctypes as C
+    qp3d.load()  # the library's HIP runtime, whose device pointers the plan recognises
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipMalloc.restype = hip.hipMemcpy.restype = hip.hipFree.restype = C.c_int
+    return hip
+
+
+class _DevBuf:
+    """A device buffer through the HIP runtime (1 = host to device, 2 = device to host)."""
+
+    def __init__(self, hip, nbytes, host=None):
+        import ctypes as C
+        self.hip, self.nbytes, self.ptr = hip, nbytes, C.c_void_p()
+        assert hip.hipMalloc(C.byref(self.ptr), nbytes) == 0
+        if host is not None:
+            assert hip.hipMemcpy(self.ptr, host.ctypes.data, nbytes, 1) == 0
+
+    def f32(self):
+        import ctypes as C
+        return C.cast(self.ptr, C.POINTER(C.c_float))
+
+    def get(self, shape, dtype=np.float32):
+        out = np.empty(shape, dtype)
+        assert out.nbytes == self.nbytes
+        assert self.hip.hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2) == 0
+        return out
+
+    def free(self):
+        self.hip.hipFree(self.ptr)
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_device_pointers_match_host_pointers(storage):
+    """The plan API takes arrays in device memory as well as host memory: upload from device
+    buffers and read the record, a wavefield, the snapshots and the maxima into device buffers,
+    bit-identical to the host-memory path (FP16 storage also takes its scale out on the device)."""
+    import ctypes as C
+    from qp3d.plan import _tables
+    hip = _hip()
+    s = qp3d.config_setup(hx=31, hy=29, nz0=27, pml=9, nt=60, seed=4, src_depth=3)
+    say(f"device pointers {storage}")
+    bufs = []
+    try:
+        with qp3d.Plan(s, storage=storage, snapflag=1, ntsnap=30) as p:
+            rec_h, u_h, g_h, sn_h = p.forward(), p.field(0), p.step_maxima(), p.snapshots()
+            arrs = [np.ascontiguousarray(a, np.float32).reshape(-1)
+                    for a in [s.vp, s.eps, s.dele, s.source, s.wavelet] + _tables(s)]
+            dev = [_DevBuf(hip, a.nbytes, a) for a in arrs]
+            bufs += dev
+            lib = p.lib
+            assert lib.qp3d_plan_upload(p.h, *[d.f32() for d in dev]) == 0
+            p.reset()
+            p.run()
+            p.sync()
+            out = {"rec": (_DevBuf(hip, rec_h.nbytes), rec_h.shape), "u": (_DevBuf(hip, u_h.nbytes), u_h.shape),
+                   "g": (_DevBuf(hip, g_h.nbytes), g_h.shape), "sn": (_DevBuf(hip, sn_h[1][1].nbytes), sn_h[1][1].shape)}
+            bufs += [b for b, _ in out.values()]
+            assert lib.qp3d_plan_record(p.h, out["rec"][0].f32()) == 0
+            assert lib.qp3d_plan_field(p.h, 0, out["u"][0].f32()) == 0
+            assert lib.qp3d_plan_step_maxima(p.h, out["g"][0].f32()) == 0
+            k = C.c_int()
+            assert lib.qp3d_plan_snapshot(p.h, 1, C.byref(k), out["sn"][0].f32()) == 0
+            assert k.value == sn_h[1][0]
+            got = {name: b.get(shape) for name, (b, shape) in out.items()}
+        assert np.array_equal(got["rec"], rec_h)
+        assert np.array_equal(got["u"], u_h)
+        assert np.array_equal(got["g"], g_h)
+        assert np.array_equal(got["sn"], sn_h[1][1])
+    finally:
+        for b in bufs:
+            b.free()
