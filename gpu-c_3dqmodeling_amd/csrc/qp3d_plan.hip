@@ -465,6 +465,8 @@ extern "C" int qp3d_plan_destroy(qp3d_plan *p)
   for (float *d : p->snaps_s) (void)hipFree(d);
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
   if (p->ev_grad) (void)hipEventDestroy(p->ev_grad);
+  if (p->ev_in) (void)hipEventDestroy(p->ev_in);
+  if (p->ev_out) (void)hipEventDestroy(p->ev_out);
   if (p->ev_pml) (void)hipEventDestroy(p->ev_pml);
   if (p->stream2) (void)hipStreamDestroy(p->stream2);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -881,6 +883,29 @@ extern "C" int qp3d_plan_run(qp3d_plan *p, int k0, int k1)
   }
   // everything enqueued so far is ordered before later work on the plan's stream
   if (p->two_streams) HIP_TRY(hipStreamWaitEvent(p->stream, p->ev_pml, 0));
+  return QP3D_OK;
+}
+
+// qp3d_plan_run ordered with a caller's stream: the steps start after the work queued on
+// `stream` so far, and work queued on `stream` afterwards waits for them (events, no host wait).
+extern "C" int qp3d_plan_run_on(qp3d_plan *p, int k0, int k1, void *stream)
+{
+  if (!p) {
+    qp3d_set_error("plan_run_on: null plan");
+    return QP3D_ERR_ARG;
+  }
+  HIP_TRY(hipSetDevice(p->cfg.device));
+  if (!p->ev_in) {
+    HIP_TRY(hipEventCreateWithFlags(&p->ev_in, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&p->ev_out, hipEventDisableTiming));
+  }
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipEventRecord(p->ev_in, s));
+  HIP_TRY(hipStreamWaitEvent(p->stream, p->ev_in, 0));
+  const int rc = qp3d_plan_run(p, k0, k1);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(p->ev_out, p->stream));
+  HIP_TRY(hipStreamWaitEvent(s, p->ev_out, 0));
   return QP3D_OK;
 }
 

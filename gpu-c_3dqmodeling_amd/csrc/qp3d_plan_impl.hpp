@@ -45,6 +45,7 @@ struct qp3d_plan {
   bool two_streams = false;
   hipStream_t stream2 = nullptr;                          // PML updates when two_streams
   hipEvent_t ev_grad = nullptr, ev_pml = nullptr;         // cross-stream ordering, no timing
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;           // ordering with a caller's stream
   WorkList wl_grad, wl_int, wl_pml, wl_pmly;  // wl_pml: x and z faces; wl_pmly: y faces
   int tile_py = 1;
   long n_int_cells = 0, n_pml_cells = 0;

@@ -63,6 +63,7 @@ _SIGS = [
     ("qp3d_plan_reset", C.c_int, [C.c_void_p]),
     ("qp3d_plan_run", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("qp3d_plan_sync", C.c_int, [C.c_void_p]),
+    ("qp3d_plan_run_on", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     ("qp3d_plan_record", C.c_int, [C.c_void_p, _F]),
     ("qp3d_plan_field", C.c_int, [C.c_void_p, C.c_int, _F]),
     ("qp3d_plan_snapshot_count", C.c_int, [C.c_void_p]),

@@ -126,6 +126,10 @@ int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps, const floa
 int qp3d_plan_reset(qp3d_plan *p);
 /* Enqueues time steps [k0, k1) on the plan's stream (asynchronous). */
 int qp3d_plan_run(qp3d_plan *p, int k0, int k1);
+/* The same, ordered with the caller's HIP stream (a hipStream_t; NULL = the null stream): the
+ * steps start after the work already queued on it, and work queued on it afterwards waits for
+ * them. No host synchronisation; qp3d_plan_sync still reports a non-finite wavefield. */
+int qp3d_plan_run_on(qp3d_plan *p, int k0, int k1, void *stream);
 /* Waits for the plan's stream; returns QP3D_ERR_NONFINITE if the field blew up. */
 int qp3d_plan_sync(qp3d_plan *p);
 /* Copies the device record [nt][hx][hy] to `record` (host or device memory). */

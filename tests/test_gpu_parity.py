@@ -366,3 +366,26 @@ def test_device_pointers_match_host_pointers(storage):
     finally:
         for b in bufs:
             b.free()
+
+
+def test_run_on_a_caller_stream():
+    """qp3d_plan_run_on orders the steps with a caller's stream: after the caller's stream has
+    drained, the record equals a plain run's."""
+    import ctypes as C
+    hip = _hip()
+    hip.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+    hip.hipStreamSynchronize.argtypes = hip.hipStreamDestroy.argtypes = [C.c_void_p]
+    hip.hipStreamCreate.restype = hip.hipStreamSynchronize.restype = hip.hipStreamDestroy.restype = C.c_int
+    s = qp3d.config_setup(hx=25, hy=27, nz0=23, pml=8, nt=40, seed=6, src_depth=2)
+    st = C.c_void_p()
+    assert hip.hipStreamCreate(C.byref(st)) == 0
+    try:
+        with qp3d.Plan(s) as p:
+            ref = p.forward()
+            p.reset()
+            assert p.lib.qp3d_plan_run_on(p.h, 0, s.nt, st) == 0
+            assert hip.hipStreamSynchronize(st) == 0
+            p.sync()
+            assert np.array_equal(p.record(), ref)
+    finally:
+        hip.hipStreamDestroy(st)
