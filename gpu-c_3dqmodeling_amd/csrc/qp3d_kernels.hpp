@@ -13,6 +13,8 @@
 //                    receiver-plane write (3dPureqPModeling_single_shot.cu:333-337);
 //   k_update<PML=1>  the same for the PML tiles, with the split-field recursions
 //                    (GPU_kernel.cu:566-637).
+// k_sfield forms the scalar-operator field of a snapshot step on request (the reference's
+// writer for it is commented out, 3dPureqPModeling_single_shot.cu:381-409).
 // Tiles are aligned to the PML boundary, so an interior tile holds only interior cells and a
 // PML tile only cells of one face family (x faces own edges and corners, then y, then z),
 // which makes the family uniform over a workgroup and keeps the PML state out of the interior
@@ -30,7 +32,8 @@
 // the plane before. Loads of lanes outside the tile read a clamped cell and are never used.
 // A lane's offsets within a plane are fixed for the whole march and the plane
 // base is a scalar, so each load is one instruction with no vector address arithmetic
-// (grids stay below 2^30 cells).
+// (grids stay below 2^30 cells). Lanes start on a 128-byte line where the tile allows it, and
+// in the gradient pass each lane owns a run of consecutive rows (see Lane).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
