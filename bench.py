@@ -344,6 +344,8 @@ def main():
     ap.add_argument("--tile-pml", type=int, default=0)
     ap.add_argument("--tile-pml-y", type=int, default=0)
     ap.add_argument("--streams", type=int, default=0, help="2: interior and PML updates concurrent; 1: serial")
+    ap.add_argument("--gmax-mode", type=int, default=0,
+                    help="1: gradient pass before the update; 2: predicted threshold with fix-up; 0: default")
     ap.add_argument("--storage", default="fp32", choices=["fp32", "bf16", "fp16"],
                     help="storage of wavefields and PML state (arithmetic is always FP32)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -381,7 +383,7 @@ def main():
     ntsnap = 500
     plan = qp3d.Plan(s, snapflag=1, ntsnap=ntsnap, use_graph=bool(args.graph), tile_update=args.tile_update,
                      tile_grad=args.tile_grad, tile_pml=args.tile_pml, tile_pml_y=args.tile_pml_y,
-                     streams=args.streams, storage=args.storage)
+                     streams=args.streams, storage=args.storage, gmax_mode=args.gmax_mode)
     log("model uploaded")
     for i in range(args.warmup):
         t = time.perf_counter()
@@ -409,6 +411,9 @@ def main():
         kt["update_interior"] = kt_timed["update_interior"]
     kb = plan.step_bytes()
     plan.set_instrument(0)
+    from qp3d.threshold import spec_valid_steps
+    gmode = plan.gmax_mode()
+    in_band = int(spec_valid_steps(plan.step_maxima()).sum()) if gmode == 2 else None
     log(f"timed {args.steps} shots in {wall:.3f} s; per step: {kt}")
     gbs = {k: kb[k] / (kt[k] * 1e-3) / 1e9 for k in ("gradmax", "update_interior", "update_pml") if kt[k] > 0}
     log(f"bytes per step {kb}; achieved GB/s {gbs}")
@@ -455,7 +460,7 @@ def main():
                    "ms_gradmax": round(kt["gradmax"], 5), "ms_update_interior": round(kt["update_interior"], 5),
                    "ms_update_pml": round(kt["update_pml"], 5), "ms_update_stage": round(kt["update_stage"], 5),
                    "gbs_gradmax": round(gbs.get("gradmax", 0), 1), "gbs_update_pml": round(gbs.get("update_pml", 0), 1),
-                   "storage": args.storage,
+                   "storage": args.storage, "gmax_mode": gmode, "steps_in_predicted_band": in_band,
                    "roofline_kernel": f"k_update<interior> ({kb['update_interior'] / kb['cells_interior']:.0f} B per interior cell)",
                    "hbm_frac_step": round(step_bytes * s.nt * args.steps / wall / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

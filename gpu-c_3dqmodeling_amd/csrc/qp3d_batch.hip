@@ -22,6 +22,9 @@ struct qp3d_batch {
   // each shot's own launches (its plan's work lists and tiles) one shot after another within a
   // step, instead of one launch per stage for all shots; the autotuner sets it when faster
   bool per_shot = false;
+  // the batch kernels take G from the gradient pass: plans with gmax_mode 2 run with it off
+  // while they belong to the batch, and get it back when the batch is destroyed
+  std::vector<char> spec_saved;
   WorkList wl[4];
   void *args_g[2] = {}, *args_u[2] = {};  // per-parity argument blocks of every shot
   int *kdev = nullptr;
@@ -176,6 +179,7 @@ static int batch_step(qp3d_batch *b, int k, Step st)
 static void batch_free(qp3d_batch *b)
 {
   if (!b) return;
+  for (size_t i = 0; i < b->spec_saved.size(); i++) b->plans[i]->spec = b->spec_saved[i] != 0;
   for (WorkList &w : b->wl) (void)hipFree(w.d);
   for (int i = 0; i < 2; i++) {
     (void)hipFree(b->args_g[i]);
@@ -213,6 +217,10 @@ extern "C" int qp3d_batch_create(qp3d_plan *const *plans, int nshots, int use_gr
   qp3d_batch *b = new qp3d_batch();
   b->plans.assign(plans, plans + nshots);
   for (int i = 0; i < nshots; i++) b->gen.push_back(plans[i]->upload_gen);
+  for (int i = 0; i < nshots; i++) {
+    b->spec_saved.push_back(plans[i]->spec ? 1 : 0);
+    plans[i]->spec = false;
+  }
   b->nt = c0.nt;
   b->storage = c0.storage;
   b->device = c0.device;

@@ -151,7 +151,9 @@ struct UpdArgs {
   const typename S::T *w1c, *w3c, *pc;      // PML slots k
   typename S::T *w21;                       // PML accumulator (updated in place)
   const float4 *pmltab;                     // [3][pml]: K1, K2, K3, E per face family and layer
-  const unsigned *gmax;                     // [nt] G per step (float bits)
+  unsigned *gmax;                           // [nt] G per step (float bits); SPEC: G_k reduced here
+  int *flag;                                // SPEC: set to 1 when a non-finite gradient is seen
+  unsigned *fixmask;                        // SPEC: one bit per cell, cells to check after the step
   const float *wav;                         // [nt] wavelet
   Step step;
   int src_on;
@@ -171,6 +173,7 @@ struct GradArgs {
   int *flag;        // out: set to 1 when a non-finite gradient is seen
   const int4 *work;
   Step step;
+  unsigned *fixmask;  // marking pass: one bit per cell whose threshold decision changes
 };
 
 // A lane's fixed position in its tile and its in-plane offsets.
@@ -353,10 +356,13 @@ struct Unroll<N, N> {
 // threshold |grad u| <= thr; above it n = grad u / (|grad u| + 2.22e-17) and
 // S = (1 + sqrt(1 - 8 (eps-del) (nx^2+ny^2) nz^2 / (1 + 2 eps (nx^2+ny^2))^2)) / 2.
 // e = eps, ed = eps - del. The update kernel and the S-field snapshot share it.
-__device__ __forceinline__ float scalar_op(float ux, float uy, float uz, float e, float ed, float thr)
+__device__ __forceinline__ float cell_mode(float ux, float uy, float uz)
+{
+  return __builtin_amdgcn_sqrtf(fmaf(ux, ux, fmaf(uy, uy, uz * uz)));
+}
+__device__ __forceinline__ float scalar_op_m(float mode, float ux, float uy, float uz, float e, float ed, float thr)
 {
   float Sv = 1.f;
-  const float mode = __builtin_amdgcn_sqrtf(fmaf(ux, ux, fmaf(uy, uy, uz * uz)));
   if (mode > thr) {
     const float inv = __builtin_amdgcn_rcpf(mode + 2.22e-17f);
     const float nxv = ux * inv, nyv = uy * inv, nzv = uz * inv;
@@ -367,26 +373,101 @@ __device__ __forceinline__ float scalar_op(float ux, float uy, float uz, float e
   }
   return Sv;
 }
+__device__ __forceinline__ float scalar_op(float ux, float uy, float uz, float e, float ed, float thr)
+{
+  return scalar_op_m(cell_mode(ux, uy, uz), ux, uy, uz, e, ed, thr);
+}
 
 // The threshold 0.01*G of the scalar operator: the reference compares |grad u| > 0.01*G in
 // double (GPU_kernel.cu:99); for a float magnitude this equals mode > rd(0.01*G).
 __device__ __forceinline__ float s_threshold(float G) { return __double2float_rd(1.0e-02 * (double)G); }
 
+// Predicted threshold (gmax_mode 2). The update of step k runs before G_k is known: it takes
+// ts = rd(0.01 Gp) with Gp extrapolated from G_{k-1} and G_{k-2} (ratio clamped to [1/2, 2]),
+// marks every cell whose |grad u| lies in the band (lo, hi] around ts, and reduces G_k itself.
+// The true threshold tt = rd(0.01 G_k) then changes the decision |grad u| > thr only for cells
+// between ts and tt: all of them are marked when lo <= tt <= hi (valid), and a marking pass
+// over the grid finds them otherwise.
+constexpr float SPEC_BAND = 1.f / 32.f;
+struct SpecBand {
+  float ts, lo, hi;
+};
+__device__ __forceinline__ SpecBand spec_band(const unsigned *gmax, int k)
+{
+  float gp = 0.f;
+  if (k >= 1) {
+    const float g1 = __uint_as_float(gmax[k - 1]);
+    gp = g1;
+    if (k >= 2) {
+      const float g2 = __uint_as_float(gmax[k - 2]);
+      if (g2 > 0.f && g1 > 0.f && g1 <= 3.4e38f) gp = g1 * fminf(fmaxf(g1 / g2, 0.5f), 2.f);
+    }
+  }
+  SpecBand b;
+  b.ts = s_threshold(gp);
+  b.lo = b.ts * (1.f - SPEC_BAND);
+  b.hi = b.ts * (1.f + SPEC_BAND);
+  return b;
+}
+__device__ __forceinline__ bool spec_valid(const SpecBand &b, float tt) { return tt >= b.lo && tt <= b.hi; }
+__device__ __forceinline__ void mark_cell(unsigned *mask, unsigned cell) { atomicOr(mask + (cell >> 5), 1u << (cell & 31)); }
+
+// Workgroup maximum of m into *gmax (float bits, one atomic per workgroup); *flag |= 1 when a
+// lane saw a non-finite gradient (nonfinite, or m above the float range).
+template <class T>
+__device__ __forceinline__ void reduce_gmax(float m, bool nonfinite, int tx, int ty, unsigned *gmax, int *flag)
+{
+  __shared__ float red[T::WX];
+  __shared__ int redbad[T::WX];
+  int bad = nonfinite || !(m <= 3.4e38f);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    m = fmaxf(m, __shfl_xor(m, off, 64));
+    bad |= __shfl_xor(bad, off, 64);
+  }
+  if (ty == 0) {
+    red[tx] = m;
+    redbad[tx] = bad;
+  }
+  __syncthreads();
+  if (tx == 0 && ty == 0) {
+    float v = red[0];
+    int bb = redbad[0];
+#pragma unroll
+    for (int w = 1; w < T::WX; w++) {
+      v = fmaxf(v, red[w]);
+      bb |= redbad[w];
+    }
+    atomicMax(gmax, __float_as_uint(v));
+    if (bb) atomicOr(flag, 1);
+  }
+}
+
 // One work item of the gradient-max pass (w: packed tile, kstep: time step). Each lane owns a
 // run of RX consecutive rows: the x stencil of the run reads 10 rows from LDS and takes the
 // run's own rows from the lane's queue, which cuts the pass's LDS reads (its bottleneck) from
 // 10 + 10 to 10/RX + 10 per cell.
-template <class S, class T>
+//
+// MARK (gmax_mode 2, a step whose predicted threshold ts missed: tt outside the marked band):
+// the same march computes |grad u| with the update kernel's arithmetic and marks the cells
+// whose decision |grad u| > thr differs between ts and tt; the workgroup exits at once when
+// the band was valid.
+template <class S, class T, bool MARK = false>
 __device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int kstep)
 {
   __shared__ float sm[2][T::LX + 1][T::LY];
-  __shared__ float red[T::WX];
-  __shared__ int redbad[T::WX];
   const Geo &g = a.g;
   int x0, x1, y0, y1, z0, z1;
   bool fold;
   unpack(w, x0, x1, y0, y1, z0, z1, fold);
   if (z0 >= z1) return;
+  float ts = 0.f, tt = 0.f;
+  if constexpr (MARK) {
+    const SpecBand sb = spec_band(a.gmax, kstep);
+    tt = s_threshold(__uint_as_float(a.gmax[kstep]));
+    if (spec_valid(sb, tt)) return;
+    ts = sb.ts;
+  }
   Lane<T, true> L(g, x0, x1, y0, y1, fold);
   const bool ycmp = L.yon;
   bool rows_on[T::RX];
@@ -426,15 +507,28 @@ __device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int k
       for (int r = 0; r < T::RX; r++) {
         const int X = XB + r, Y = L.Y;
         float ux = 0.f, uy = 0.f, uz = 0.f;
+        if constexpr (MARK) {
+          // the update kernel's first derivatives, bit for bit: sum_m c1_m (u+m - u-m), then 1/d
 #pragma unroll
-        for (int k = 1; k <= H; k++) {
-          ux = fmaf(a.c.g1x[k - 1], xv(r + k) - xv(r - k), ux);
-          uy = fmaf(a.c.g1y[k - 1], s[X][Y + k] - s[X][Y - k], uy);
-          uz = fmaf(a.c.g1z[k - 1], q[r][(H + k + it) % QN] - q[r][(H - k + it) % QN], uz);
-        }
-        if (rows_on[r]) {
-          m = fmaxf(m, fmaxf(fabsf(ux), fmaxf(fabsf(uy), fabsf(uz))));
-          nanacc = fmaf(ux, 0.f, nanacc);
+          for (int k = 1; k <= H; k++) {
+            ux = fmaf(a.c.c1[k - 1], xv(r + k) - xv(r - k), ux);
+            uy = fmaf(a.c.c1[k - 1], s[X][Y + k] - s[X][Y - k], uy);
+            uz = fmaf(a.c.c1[k - 1], q[r][(H + k + it) % QN] - q[r][(H - k + it) % QN], uz);
+          }
+          ux *= a.c.ix; uy *= a.c.iy; uz *= a.c.iz;
+          const float mode = cell_mode(ux, uy, uz);
+          if (rows_on[r] && ((mode > ts) != (mode > tt))) mark_cell(a.fixmask, (unsigned)z * g.pln + (L.off[r] >> 2));
+        } else {
+#pragma unroll
+          for (int k = 1; k <= H; k++) {
+            ux = fmaf(a.c.g1x[k - 1], xv(r + k) - xv(r - k), ux);
+            uy = fmaf(a.c.g1y[k - 1], s[X][Y + k] - s[X][Y - k], uy);
+            uz = fmaf(a.c.g1z[k - 1], q[r][(H + k + it) % QN] - q[r][(H - k + it) % QN], uz);
+          }
+          if (rows_on[r]) {
+            m = fmaxf(m, fmaxf(fabsf(ux), fmaxf(fabsf(uy), fabsf(uz))));
+            nanacc = fmaf(ux, 0.f, nanacc);
+          }
         }
       }
       if (more) {
@@ -449,29 +543,7 @@ __device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int k
     }
   };
   for (; zb < z1; zb += 2 * QN) Unroll<0, 2 * QN>::run(plane, z1 - zb);
-  int bad = !(nanacc == 0.f) || !(m <= 3.4e38f);
-  // workgroup max -> one atomic per workgroup
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    m = fmaxf(m, __shfl_xor(m, off, 64));
-    bad |= __shfl_xor(bad, off, 64);
-  }
-  if (L.ty == 0) {
-    red[L.tx] = m;
-    redbad[L.tx] = bad;
-  }
-  __syncthreads();
-  if (L.tx == 0 && L.ty == 0) {
-    float v = red[0];
-    int bb = redbad[0];
-#pragma unroll
-    for (int w = 1; w < T::WX; w++) {
-      v = fmaxf(v, red[w]);
-      bb |= redbad[w];
-    }
-    atomicMax(a.gmax + kstep, __float_as_uint(v));
-    if (bb) atomicOr(a.flag, 1);
-  }
+  if constexpr (!MARK) reduce_gmax<T>(m, !(nanacc == 0.f), L.tx, L.ty, a.gmax + kstep, a.flag);
 }
 
 // Batched launches (several grids in one launch): item .z packs z0 | z1 << 16 and .w is the
@@ -487,6 +559,13 @@ template <class S, class T>
 __global__ __launch_bounds__(T::NT) void k_gradmax(GradArgs<S> a)
 {
   gradmax_tile<S, T>(a, a.work[blockIdx.x], a.step.get());
+}
+
+// Marking pass of a step whose predicted threshold missed (gmax_mode 2); see gradmax_tile.
+template <class S, class T>
+__global__ __launch_bounds__(T::NT) void k_spec_mark(GradArgs<S> a)
+{
+  gradmax_tile<S, T, true>(a, a.work[blockIdx.x], a.step.get());
 }
 
 template <class S, class T>
@@ -532,7 +611,9 @@ struct CellLoad {
 };
 
 // One work item of the update pass (w: packed tile, kstep: time step).
-template <class S, class T, bool PML>
+// SPEC (gmax_mode 2): the threshold is predicted (spec_band), cells in the band are marked in
+// a.fixmask, and the tile's maximum gradient component goes into G_k.
+template <class S, class T, bool PML, bool SPEC = false>
 __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kstep)
 {
   __shared__ float sm[2][T::LX + 1][T::LY];
@@ -549,10 +630,21 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
 #pragma unroll
   for (int r = 0; r < T::RX; r++) rows_on[r] = ycmp && x0 + L.row(r) < x1;
 
-  const float G = __uint_as_float(a.gmax[kstep]);
   const float wk = a.wav[kstep];
   float *const rec = a.rec + (size_t)kstep * (g.nx - 2 * g.pml) * (g.ny - 2 * g.pml);
-  const float thr = s_threshold(G);
+  // threshold of the scalar operator: from G_k, or predicted (SPEC) with the band (blo, bhi]
+  // of cells to mark
+  float thr, blo = 0.f, bhi = 0.f;
+  if constexpr (SPEC) {
+    const SpecBand sb = spec_band(a.gmax, kstep);
+    thr = sb.ts;
+    blo = sb.lo;
+    bhi = sb.hi;
+  } else {
+    thr = s_threshold(__uint_as_float(a.gmax[kstep]));
+  }
+  float gm = 0.f;          // SPEC: G_k over this tile's cells
+  bool nonfinite = false;  // SPEC: a non-finite gradient component seen
   // source box and receiver plane tests, hoisted to the workgroup and the plane (uniform)
   const bool src_tile = a.src_on && a.sb_x0 < x1 && a.sb_x1 >= x0 &&
                         (fold || (a.sb_y0 < y1 && a.sb_y1 >= y0)) && a.sb_z0 < z1 && a.sb_z1 >= z0;
@@ -648,9 +740,15 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
         const float Uzz = fmaf(-a.c.s2, c0, Lzz) * a.c.izz;
         ux *= a.c.ix; uy *= a.c.iy; uz *= a.c.iz;
         const float e = C.e[r], ed = C.ed[r], v2 = C.v2[r];
-        const float Sv = scalar_op(ux, uy, uz, e, ed, thr);
+        const float mode = cell_mode(ux, uy, uz);
+        const float Sv = scalar_op_m(mode, ux, uy, uz, e, ed, thr);
         const float e1 = fmaf(2.f, e, 1.f);
         const unsigned o = L.off[r];
+        if constexpr (SPEC) {
+          gm = fmaxf(gm, fmaxf(fabsf(ux), fmaxf(fabsf(uy), fabsf(uz))));
+          nonfinite |= !(fabsf(ux) + fabsf(uy) + fabsf(uz) <= 3.4e38f);
+          if (mode > blo && mode <= bhi) mark_cell(a.fixmask, (unsigned)z * g.pln + (o >> 2));
+        }
         float un;
         if constexpr (!PML) {
           // GPU_kernel.cu:558-565
@@ -714,12 +812,13 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
       }
     }
   }
+  if constexpr (SPEC) reduce_gmax<T>(gm, nonfinite, L.tx, L.ty, a.gmax + kstep, a.flag);
 }
 
-template <class S, class T, bool PML>
+template <class S, class T, bool PML, bool SPEC = false>
 __global__ __launch_bounds__(T::NT) void k_update(UpdArgs<S> a)
 {
-  update_tile<S, T, PML>(a, a.work[blockIdx.x], a.step.get());
+  update_tile<S, T, PML, SPEC>(a, a.work[blockIdx.x], a.step.get());
 }
 
 template <class S, class T, bool PML>
@@ -794,6 +893,102 @@ __global__ void k_sfield(const typename S::T *u, const float *eps, const float *
       Sv = scalar_op(ux, uy, uz, eps[i], epsd[i], thr);
     }
     out[i] = Sv;
+  }
+}
+
+// Fix-up of a step taken with the predicted threshold ts (gmax_mode 2), after every update of
+// the step has run and G_k is complete. Each thread takes words of the cell mask, clears them,
+// and for each marked cell recomputes the update kernel's derivatives from u_k (bit for bit:
+// same terms in the same order); where |grad u| > ts and |grad u| > tt = rd(0.01 G_k) disagree,
+// the scalar operator that step used (S at ts) is replaced by S at tt: the new wavefield, the
+// PML state and the record move by the change of S times the terms it multiplies (every
+// update is linear in S). u_k, which the step does not write, is still in place.
+template <class S>
+__device__ __forceinline__ void fix_cell(const UpdArgs<S> &a, unsigned i, float ts, float tt, int k)
+{
+  const Geo &g = a.g;
+  const int z = (int)(i / g.pln);
+  const unsigned rem = i - (unsigned)z * g.pln;
+  const int x = (int)(rem / (unsigned)g.ny), y = (int)(rem - (unsigned)x * g.ny);
+  if (z < H || z >= g.nz - H || x < H || x >= g.nx - H || y < H || y >= g.ny - H) return;
+  const unsigned b = 4u * i, bx = 4u * (unsigned)g.ny, bz = 4u * g.pln;
+  const float c0 = S::ld(a.ucur, b);
+  float Lxx = 0.f, Lyy = 0.f, Lzz = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
+#pragma unroll
+  for (int m = 1; m <= H; m++) {
+    const float ax = S::ld(a.ucur, b + m * bx), bx_ = S::ld(a.ucur, b - m * bx);
+    const float ay = S::ld(a.ucur, b + 4u * m), by = S::ld(a.ucur, b - 4u * m);
+    const float az = S::ld(a.ucur, b + m * bz), bz_ = S::ld(a.ucur, b - m * bz);
+    Lxx = fmaf(a.c.c2[m - 1], ax + bx_, Lxx);
+    Lyy = fmaf(a.c.c2[m - 1], ay + by, Lyy);
+    Lzz = fmaf(a.c.c2[m - 1], az + bz_, Lzz);
+    ux = fmaf(a.c.c1[m - 1], ax - bx_, ux);
+    uy = fmaf(a.c.c1[m - 1], ay - by, uy);
+    uz = fmaf(a.c.c1[m - 1], az - bz_, uz);
+  }
+  ux *= a.c.ix; uy *= a.c.iy; uz *= a.c.iz;
+  const float mode = cell_mode(ux, uy, uz);
+  if ((mode > ts) == (mode > tt)) return;
+  const float Uxx = fmaf(-a.c.s2, c0, Lxx) * a.c.ixx;
+  const float Uyy = fmaf(-a.c.s2, c0, Lyy) * a.c.iyy;
+  const float Uzz = fmaf(-a.c.s2, c0, Lzz) * a.c.izz;
+  const float e = a.eps[i], ed = a.epsd[i], v2 = a.vdt2[i];
+  const float dS = scalar_op_m(mode, ux, uy, uz, e, ed, tt) - scalar_op_m(mode, ux, uy, uz, e, ed, ts);
+  const float e1 = fmaf(2.f, e, 1.f);
+  const int pml = g.pml;
+  const bool xf = x < pml || x >= g.nx - pml, yf = y < pml || y >= g.ny - pml, zf = z < pml || z >= g.nz - pml;
+  const float un = S::ld(a.unext, b);
+  float unn;
+  if (!xf && !yf && !zf) {
+    unn = S::rnd(fmaf((e1 * (Uxx + Uyy) + Uzz) * dS, v2, un));
+  } else {
+    // face priority x > y > z, as the PML tiles
+    float4 t;
+    float sgn, Unn, Tg, gn, dC;
+    const float dB = dS * v2;
+    if (xf) {
+      t = a.pmltab[x < pml ? pml - x : x - g.nx + pml];
+      sgn = x < pml ? 1.f : -1.f;
+      dC = e1 * dB; Unn = Uxx; Tg = Uzz + e1 * Uyy; gn = ux;
+    } else if (yf) {
+      t = a.pmltab[pml + (y < pml ? pml - y : y - g.ny + pml)];
+      sgn = y < pml ? 1.f : -1.f;
+      dC = e1 * dB; Unn = Uyy; Tg = Uzz + e1 * Uxx; gn = uy;
+    } else {
+      t = a.pmltab[2 * pml + (z < pml ? pml - z : z - g.nz + pml)];
+      sgn = z < pml ? 1.f : -1.f;
+      dC = dB; Unn = Uzz; Tg = e1 * (Uxx + Uyy); gn = uz;
+    }
+    const float w1 = S::ld(a.w1p, b), p = S::ld(a.pp, b), w21 = S::ld(a.w21, b), w3 = S::ld(a.w3p, b);
+    const float w1n = S::rnd(fmaf(dC, Unn, w1));
+    const float pn = S::rnd(fmaf(sgn, dC * t.w * gn, p));
+    const float w21n = S::rnd(fmaf(a.c.dt, pn - p, w21));
+    const float w3n = S::rnd(fmaf(dB, Tg, w3));
+    S::st(a.w1p, b, w1n);
+    S::st(a.pp, b, pn);
+    S::st(a.w21, b, w21n);
+    S::st(a.w3p, b, w3n);
+    unn = S::rnd(un + ((w1n - w1) + (w21n - w21) + (w3n - w3)));
+  }
+  S::st(a.unext, b, unn);
+  if (z == a.sz && x >= pml && x < g.nx - pml && y >= pml && y < g.ny - pml)
+    a.rec[(size_t)k * (g.nx - 2 * pml) * (g.ny - 2 * pml) + (size_t)(x - pml) * (g.ny - 2 * pml) + (y - pml)] = unn;
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void k_spec_fix(UpdArgs<S> a, long nwords)
+{
+  const int k = a.step.get();
+  const float ts = spec_band(a.gmax, k).ts, tt = s_threshold(__uint_as_float(a.gmax[k]));
+  for (long w = blockIdx.x * (long)blockDim.x + threadIdx.x; w < nwords; w += (long)gridDim.x * blockDim.x) {
+    unsigned bits = a.fixmask[w];
+    if (!bits) continue;
+    a.fixmask[w] = 0u;
+    while (bits) {
+      const int j = __builtin_ctz(bits);
+      bits &= bits - 1u;
+      fix_cell<S>(a, (unsigned)w * 32u + (unsigned)j, ts, tt, k);
+    }
   }
 }
 

@@ -27,13 +27,29 @@ static void launch_gradmax(int tile, const WorkList &w, hipStream_t st, const Gr
 #undef QP3D_L
 }
 
-template <class S, bool PML>
-static void launch_update(int tile, const WorkList &w, hipStream_t st, const UpdArgs<S> &a)
+template <class S>
+static void launch_mark(int tile, const WorkList &w, hipStream_t st, const GradArgs<S> &a)
 {
   int i = 0;
 #define QP3D_L(wx, rx)                                                                                 \
   if (tile == i++) {                                                                                   \
-    hipLaunchKernelGGL((k_update<S, Tile<wx, rx>, PML>), dim3(w.n), dim3(TY, wx), 0, st, a);           \
+    hipLaunchKernelGGL((k_spec_mark<S, Tile<wx, rx>>), dim3(w.n), dim3(TY, wx), 0, st, a);             \
+    return;                                                                                            \
+  }
+  QP3D_TILES(QP3D_L)
+#undef QP3D_L
+}
+
+template <class S, bool PML>
+static void launch_update(int tile, const WorkList &w, hipStream_t st, const UpdArgs<S> &a, bool spec)
+{
+  int i = 0;
+#define QP3D_L(wx, rx)                                                                                 \
+  if (tile == i++) {                                                                                   \
+    if (spec)                                                                                          \
+      hipLaunchKernelGGL((k_update<S, Tile<wx, rx>, PML, true>), dim3(w.n), dim3(TY, wx), 0, st, a);   \
+    else                                                                                               \
+      hipLaunchKernelGGL((k_update<S, Tile<wx, rx>, PML>), dim3(w.n), dim3(TY, wx), 0, st, a);         \
     return;                                                                                            \
   }
   QP3D_TILES(QP3D_L)
@@ -277,12 +293,19 @@ static int build_work(qp3d_plan *p)
   with_storage(p->cfg.storage, [&](auto sp) {
     using S = decltype(sp);
     occ_g = occupancy_of(p->tile_g, [](auto t) { return reinterpret_cast<const void *>(&k_gradmax<S, decltype(t)>); });
-    occ_i = occupancy_of(p->tile_u,
-                         [](auto t) { return reinterpret_cast<const void *>(&k_update<S, decltype(t), false>); });
-    occ_p = occupancy_of(p->tile_p,
-                         [](auto t) { return reinterpret_cast<const void *>(&k_update<S, decltype(t), true>); });
-    occ_py = occupancy_of(p->tile_py,
-                          [](auto t) { return reinterpret_cast<const void *>(&k_update<S, decltype(t), true>); });
+    const bool spec = p->spec;
+    occ_i = occupancy_of(p->tile_u, [spec](auto t) {
+      return spec ? reinterpret_cast<const void *>(&k_update<S, decltype(t), false, true>)
+                : reinterpret_cast<const void *>(&k_update<S, decltype(t), false>);
+    });
+    occ_p = occupancy_of(p->tile_p, [spec](auto t) {
+      return spec ? reinterpret_cast<const void *>(&k_update<S, decltype(t), true, true>)
+                : reinterpret_cast<const void *>(&k_update<S, decltype(t), true>);
+    });
+    occ_py = occupancy_of(p->tile_py, [spec](auto t) {
+      return spec ? reinterpret_cast<const void *>(&k_update<S, decltype(t), true, true>)
+                : reinterpret_cast<const void *>(&k_update<S, decltype(t), true>);
+    });
   });
   int rc = qp3d_build_uniform_list(p, p->tile_g, occ_g, p->wl_grad);
   if (rc) return rc;
@@ -336,6 +359,10 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
     qp3d_set_error("plan_create: tile shapes must be in [0,%d] and streams in [0,2]", NTILES);
     return QP3D_ERR_ARG;
   }
+  if (c.gmax_mode < 0 || c.gmax_mode > 2) {
+    qp3d_set_error("plan_create: gmax_mode %d outside [0,2]", c.gmax_mode);
+    return QP3D_ERR_ARG;
+  }
   if (c.storage != QP3D_STORAGE_FP32 && c.storage != QP3D_STORAGE_BF16 && c.storage != QP3D_STORAGE_FP16) {
     qp3d_set_error("plan_create: unknown storage mode %d", c.storage);
     return QP3D_ERR_ARG;
@@ -367,7 +394,9 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
   p->tile_g = c.tile_grad > 0 ? c.tile_grad - 1 : default_tile_grad(c.storage);
   p->tile_p = c.tile_pml > 0 ? c.tile_pml - 1 : 5;
   p->tile_py = c.tile_pml_y > 0 ? c.tile_pml_y - 1 : 1;  // 16x1
-  p->two_streams = c.streams == 2 && !c.use_graph;  // a captured loop runs on one stream
+  p->spec = c.gmax_mode == 2;
+  // a captured loop runs on one stream, and so does a step whose fix-up follows all updates
+  p->two_streams = c.streams == 2 && !c.use_graph && !p->spec;
 
   auto fail = [&](int rc) {
     qp3d_plan_destroy(p);
@@ -418,6 +447,14 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
     return fail(rc);
   if ((rc = dmalloc((void **)&p->flag, sizeof(int), "flag"))) return fail(rc);
   if ((rc = dmalloc((void **)&p->kdev, sizeof(int), "step index"))) return fail(rc);
+  if (p->spec) {
+    p->fixwords = (p->N + 31) / 32;
+    if ((rc = dmalloc((void **)&p->fixmask, sizeof(unsigned) * (size_t)p->fixwords, "cell mask"))) return fail(rc);
+    if (hipMemset(p->fixmask, 0, sizeof(unsigned) * (size_t)p->fixwords) != hipSuccess) {
+      qp3d_set_error("plan_create: clearing the cell mask failed");
+      return fail(QP3D_ERR_HIP);
+    }
+  }
   if ((rc = dmalloc((void **)&p->wav, sizeof(float) * (size_t)(c.nt > 0 ? c.nt : 1), "wavelet"))) return fail(rc);
   const size_t recb = sizeof(float) * (size_t)c.nt * p->hx * p->hy;
   if (recb && (rc = dmalloc((void **)&p->rec, recb, "record"))) return fail(rc);
@@ -451,6 +488,7 @@ extern "C" int qp3d_plan_destroy(qp3d_plan *p)
   (void)hipFree(p->pmltab);
   (void)hipFree(p->gmax);
   (void)hipFree(p->flag);
+  (void)hipFree(p->fixmask);
   (void)hipFree(p->kdev);
   (void)hipFree(p->wav);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
@@ -682,6 +720,7 @@ extern "C" int qp3d_plan_reset(qp3d_plan *p)
   HIP_TRY(hipMemsetAsync(p->state, 0, p->state_bytes, p->stream));
   HIP_TRY(hipMemsetAsync(p->gmax, 0, sizeof(unsigned) * (size_t)(p->cfg.nt > 0 ? p->cfg.nt : 1), p->stream));
   HIP_TRY(hipMemsetAsync(p->flag, 0, sizeof(int), p->stream));
+  if (p->fixmask) HIP_TRY(hipMemsetAsync(p->fixmask, 0, sizeof(unsigned) * (size_t)p->fixwords, p->stream));
   if (p->rec) HIP_TRY(hipMemsetAsync(p->rec, 0, sizeof(float) * (size_t)p->cfg.nt * p->hx * p->hy, p->stream));
   p->nsnap_taken = 0;
   p->steps_done = 0;
@@ -694,13 +733,14 @@ extern "C" int qp3d_plan_reset(qp3d_plan *p)
 static int harvest_events(qp3d_plan *p)
 {
   for (int k : p->ev_pending) {
-    hipEvent_t *e = &p->ev[4 * (size_t)k];
+    hipEvent_t *e = &p->ev[EV_PER_STEP * (size_t)k];
     // e0..e1 gradient max; e1..e2 interior update; PML update ends at e3 and starts at e1
-    // (concurrent, two streams) or e2 (one stream). Mode 2 records only e1 and e2.
+    // (concurrent, two streams) or e2 (one stream); with gmax_mode 2 there is no gradient pass
+    // (e0 = e1) and e3..e4 is the marking pass and the fix-up. Mode 2 records only e1 and e2.
     float a = 0, b = 0, c = 0;
     HIP_TRY(hipEventElapsedTime(&b, e[1], e[2]));
     if (p->instrument == 1) {
-      HIP_TRY(hipEventElapsedTime(&a, e[0], e[1]));
+      HIP_TRY(hipEventElapsedTime(&a, p->spec ? e[3] : e[0], p->spec ? e[4] : e[1]));
       HIP_TRY(hipEventElapsedTime(&c, p->two_streams ? e[1] : e[2], e[3]));
     }
     p->t_grad += a;
@@ -723,18 +763,18 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on
   // on: launch everything on this stream (and without the plan's second stream) instead of on
   // the plan's own streams
   hipStream_t const s0 = on ? on : p->stream;
-  const bool two = p->two_streams && !on;
+  const bool two = p->two_streams && !on;  // never with gmax_mode 2
   using T = typename S::T;
   const qp3d_config &cf = p->cfg;
   const int prev = k & 1, cur = prev ^ 1;  // u[prev] holds u(k-1) and receives u(k+1)
   hipEvent_t *e = nullptr;
   if (p->instrument && !bare) {
-    if (p->ev.size() < 4 * (size_t)cf.nt) {
+    if (p->ev.size() < EV_PER_STEP * (size_t)cf.nt) {
       size_t old = p->ev.size();
-      p->ev.resize(4 * (size_t)cf.nt);
+      p->ev.resize(EV_PER_STEP * (size_t)cf.nt);
       for (size_t i = old; i < p->ev.size(); i++) HIP_TRY(hipEventCreate(&p->ev[i]));
     }
-    e = &p->ev[4 * (size_t)k];
+    e = &p->ev[EV_PER_STEP * (size_t)k];
   }
   // u(k) is complete only when the previous step's PML update has finished too
   if (two) HIP_TRY(hipStreamWaitEvent(s0, p->ev_pml, 0));
@@ -743,8 +783,10 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on
   GradArgs<S> ga;
   UpdArgs<S> ua;
   make_args<S>(p, k & 1, st, ga, ua);
-  launch_gradmax(p->tile_g, p->wl_grad, s0, ga);
-  HIP_TRY(hipGetLastError());
+  if (!p->spec) {
+    launch_gradmax(p->tile_g, p->wl_grad, s0, ga);
+    HIP_TRY(hipGetLastError());
+  }
   if (e) HIP_TRY(hipEventRecord(e[1], s0));
   if (two) {
     HIP_TRY(hipEventRecord(p->ev_grad, s0));
@@ -753,25 +795,33 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on
 
   if (p->wl_int.n) {
     ua.work = p->wl_int.d;
-    launch_update<S, false>(p->tile_u, p->wl_int, s0, ua);
+    launch_update<S, false>(p->tile_u, p->wl_int, s0, ua, p->spec);
     HIP_TRY(hipGetLastError());
   }
   if (e) HIP_TRY(hipEventRecord(e[2], s0));
   hipStream_t ps = two ? p->stream2 : s0;
   if (p->wl_pml.n) {
     ua.work = p->wl_pml.d;
-    launch_update<S, true>(p->tile_p, p->wl_pml, ps, ua);
+    launch_update<S, true>(p->tile_p, p->wl_pml, ps, ua, p->spec);
     HIP_TRY(hipGetLastError());
   }
   if (p->wl_pmly.n) {
     ua.work = p->wl_pmly.d;
-    launch_update<S, true>(p->tile_py, p->wl_pmly, ps, ua);
+    launch_update<S, true>(p->tile_py, p->wl_pmly, ps, ua, p->spec);
     HIP_TRY(hipGetLastError());
   }
-  if (e) {
-    if (all) HIP_TRY(hipEventRecord(e[3], ps));
-    p->ev_pending.push_back(k);
+  if (e && all) HIP_TRY(hipEventRecord(e[3], ps));
+  if (p->spec) {
+    // after every update of the step (G_k complete): the marking pass (its workgroups exit at
+    // once when the prediction held), then the fix-up of the marked cells
+    launch_mark(p->tile_g, p->wl_grad, s0, ga);
+    HIP_TRY(hipGetLastError());
+    const int blocks = (int)std::min<long>((p->fixwords + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_spec_fix<S>, dim3(blocks), dim3(256), 0, s0, ua, p->fixwords);
+    HIP_TRY(hipGetLastError());
+    if (e && all) HIP_TRY(hipEventRecord(e[4], s0));
   }
+  if (e) p->ev_pending.push_back(k);
   // The rim source and the snapshot touch cells no PML tile writes (the outer 5 cells; the
   // interior cube), so they follow the interior update on the main stream.
   if (two) HIP_TRY(hipEventRecord(p->ev_pml, p->stream2));
@@ -1100,6 +1150,15 @@ extern "C" int qp3d_plan_step_maxima(const qp3d_plan *p, float *out)
   return copy_out(p, out, (const float *)p->gmax, (size_t)p->cfg.nt);
 }
 
+extern "C" int qp3d_plan_gmax_mode(const qp3d_plan *p)
+{
+  if (!p) {
+    qp3d_set_error("plan_gmax_mode: null plan");
+    return QP3D_ERR_ARG;
+  }
+  return p->spec ? 2 : 1;
+}
+
 extern "C" int qp3d_plan_set_instrument(qp3d_plan *p, int on)
 {
   if (!p) return QP3D_ERR_ARG;
@@ -1135,13 +1194,15 @@ extern "C" int qp3d_plan_step_bytes(const qp3d_plan *p, double *bytes)
   const double nupd = (double)(c.nx - 10) * (c.ny - 10) * (c.nz - 10);
   const double nint = (double)p->hx * p->hy * p->hz;
   const double npml = nupd - nint;
-  // gradient-max: one read of u per updated cell. Interior update: u_cur, u_prev, 3 model
+  // gradient pass: one read of u per updated cell. Interior update: u_cur, u_prev, 3 model
   // fields, u_next (24 B). PML update: u_cur, 3 model fields, u_next, 7 state reads and 4 state
   // writes (64 B); the PML launch also carries the interior cells of mixed tiles (24 B).
   // Stored fields take esz bytes (4 float, 2 bfloat16/half); the model stays float.
   const double pml_launch_int = (double)p->n_pml_cells - npml;
   const double e = (double)p->esz, bi = 12.0 + 3.0 * e, bp = 12.0 + 13.0 * e;
-  bytes[0] = e * nupd;
+  // gmax_mode 2 has no gradient pass: its fix-up reads the cell mask (one bit per cell) and
+  // touches only the marked cells
+  bytes[0] = p->spec ? 4.0 * (double)p->fixwords : e * nupd;
   bytes[1] = bi * (double)p->n_int_cells;
   bytes[2] = bp * npml + bi * pml_launch_int;
   bytes[3] = (double)p->n_int_cells;

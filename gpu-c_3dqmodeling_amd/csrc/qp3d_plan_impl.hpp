@@ -22,7 +22,8 @@ using namespace qp3d;
 
 namespace qp3d_impl {
 
-constexpr int GRAPH_LEN = 10;  // time steps per captured graph
+constexpr int GRAPH_LEN = 10;   // time steps per captured graph
+constexpr int EV_PER_STEP = 5;  // instrumentation events per step
 
 struct WorkList {
   int4 *d = nullptr;
@@ -34,6 +35,7 @@ struct WorkList {
 
 using qp3d_impl::WorkList;
 using qp3d_impl::GRAPH_LEN;
+using qp3d_impl::EV_PER_STEP;
 
 struct qp3d_plan {
   qp3d_config cfg{};
@@ -63,6 +65,9 @@ struct qp3d_plan {
   float4 *pmltab = nullptr;
   unsigned *gmax = nullptr;  // [nt]
   int *flag = nullptr;
+  bool spec = false;             // gmax_mode 2 (a batch turns it off for its plans)
+  unsigned *fixmask = nullptr;   // gmax_mode 2: one bit per cell, all clear between steps
+  long fixwords = 0;
   float *rec = nullptr;      // [nt][hx][hy]
   float *sbox = nullptr;
   int sb[6]{};               // z0 z1 x0 x1 y0 y1
@@ -85,7 +90,8 @@ struct qp3d_plan {
   int steps_done = 0;  // one past the last step enqueued since the last reset
 
   int instrument = 0;  // 1: events around every stencil launch; 2: around the interior update only
-  std::vector<hipEvent_t> ev;   // 4 per step: before gradmax, after gradmax, after interior, after PML
+  std::vector<hipEvent_t> ev;   // EV_PER_STEP per step: before gradmax, after gradmax, after interior,
+                                // after PML, after the fix-up (gmax_mode 2)
   std::vector<int> ev_pending;  // steps with recorded events not yet harvested
   double t_grad = 0, t_int = 0, t_pml = 0, t_upd = 0;
   long n_launch = 0;
@@ -133,6 +139,7 @@ inline void make_args(const qp3d_plan *p, int prev, Step st, GradArgs<S> &ga, Up
   ga.flag = p->flag;
   ga.work = p->wl_grad.d;
   ga.step = st;
+  ga.fixmask = p->fixmask;
   ua.g = p->geo;
   ua.c = p->coef;
   ua.ucur = (const T *)p->u[cur];
@@ -144,6 +151,8 @@ inline void make_args(const qp3d_plan *p, int prev, Step st, GradArgs<S> &ga, Up
   ua.w21 = (T *)p->w21;
   ua.pmltab = p->pmltab;
   ua.gmax = p->gmax;
+  ua.flag = p->flag;
+  ua.fixmask = p->fixmask;
   ua.wav = p->wav;
   ua.step = st;
   ua.src_on = p->src_on ? 1 : 0;

@@ -39,7 +39,7 @@ class Config(C.Structure):
         ("sz", C.c_int), ("storage", C.c_int), ("use_graph", C.c_int), ("snapflag", C.c_int),
         ("ntsnap", C.c_int), ("zchunk", C.c_int), ("device", C.c_int),
         ("tile_update", C.c_int), ("tile_grad", C.c_int), ("tile_pml", C.c_int), ("streams", C.c_int),
-        ("tile_pml_y", C.c_int), ("snap_s", C.c_int),
+        ("tile_pml_y", C.c_int), ("snap_s", C.c_int), ("gmax_mode", C.c_int),
     ]
 
 
@@ -71,6 +71,7 @@ _SIGS = [
     ("qp3d_plan_snapshot_s", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _F]),
     ("qp3d_plan_step_maxima", C.c_int, [C.c_void_p, _F]),
     ("qp3d_plan_set_instrument", C.c_int, [C.c_void_p, C.c_int]),
+    ("qp3d_plan_gmax_mode", C.c_int, [C.c_void_p]),
     ("qp3d_work_items", C.c_int, [C.POINTER(Config), C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int]),
     ("qp3d_plan_dims", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("qp3d_plan_kernel_times", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),

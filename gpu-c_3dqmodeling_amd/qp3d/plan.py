@@ -31,13 +31,13 @@ class Plan:
     """Device-resident forward solver for one grid (qp3d_plan_* in include/qp3d.h)."""
 
     def __init__(self, s, storage="fp32", use_graph=False, snapflag=0, ntsnap=0, zchunk=0, device=0,
-                 tile_update=0, tile_grad=0, tile_pml=0, streams=0, tile_pml_y=0, snap_s=0):
+                 tile_update=0, tile_grad=0, tile_pml=0, streams=0, tile_pml_y=0, snap_s=0, gmax_mode=0):
         self.lib = load()
         self.s = s
         cfg = Config(s.nx, s.ny, s.nz, s.pml, s.nt, s.dx, s.dy, s.dz, s.dt, s.sz, STORAGE[storage],
                      int(use_graph), int(snapflag), int(ntsnap), int(zchunk), int(device),
                      int(tile_update), int(tile_grad), int(tile_pml), int(streams), int(tile_pml_y),
-                     int(snap_s))
+                     int(snap_s), int(gmax_mode))
         h = C.c_void_p()
         check(self.lib.qp3d_plan_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -105,6 +105,12 @@ class Plan:
         out = np.zeros(self.s.nt, np.float32)
         check(self.lib.qp3d_plan_step_maxima(self.h, fptr(out)))
         return out
+
+    def gmax_mode(self):
+        """1: a gradient pass gives G before each update; 2: predicted threshold with fix-up."""
+        rc = self.lib.qp3d_plan_gmax_mode(self.h)
+        check(min(rc, 0))
+        return rc
 
     def set_instrument(self, on=True):
         """True / 1: HIP events around every stencil launch; 2: around the interior update

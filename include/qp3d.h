@@ -105,6 +105,14 @@ typedef struct qp3d_config {
   int snap_s;      /* 1 (with snapflag == 1): at every snapshot step k also keep the scalar-operator
                       field S of that step over the whole padded grid (read it with
                       qp3d_plan_snapshot_s); 0: not kept */
+  int gmax_mode;   /* how a step obtains G, the gradient maximum that sets the scalar operator's
+                      threshold 0.01*G: 1: a gradient pass over the grid before the update (the
+                      reference's order); 2: the update kernels use the threshold predicted from
+                      the two previous steps' G, reduce this step's G themselves, and a fix-up
+                      kernel recomputes the cells whose decision |grad u| > 0.01*G the prediction
+                      got wrong (a marking pass over the grid finds them when the prediction
+                      missed by more than 1/32); the same results up to the rounding of those
+                      cells; implies streams = 1. 0 = default (1) */
 } qp3d_config;
 
 /* Tile shapes (wavefronts along x x rows per lane; a tile is 64 cells along y):
@@ -156,10 +164,14 @@ int qp3d_plan_step_maxima(const qp3d_plan *p, float *out);
 int qp3d_work_items(const qp3d_config *cfg, int which, int capacity, int *out, int cap);
 /* Interior extents hx, hy, hz of the plan's grid (any output may be NULL). */
 int qp3d_plan_dims(const qp3d_plan *p, int *hx, int *hy, int *hz);
+/* How the plan's steps obtain G (qp3d_config.gmax_mode after defaults: 1 or 2), or a
+ * QP3D_* error. */
+int qp3d_plan_gmax_mode(const qp3d_plan *p);
 /* 1: bracket every stencil launch with HIP events on the plan's stream; 2: only the interior
  * update (two events per step instead of four); 0: none. Resets the accumulated times. */
 int qp3d_plan_set_instrument(qp3d_plan *p, int on);
-/* Mean device time (ms) per time step over the instrumented steps: ms[0] gradient-max launch,
+/* Mean device time (ms) per time step over the instrumented steps: ms[0] gradient-max launch
+ * (gmax_mode 2: the marking pass and the fix-up),
  * ms[1] interior update, ms[2] PML update, ms[3] the whole update stage (end of the gradient
  * maximum to the end of both updates; they overlap when streams == 2); ms[4] = steps counted.
  * With instrumentation mode 2 only ms[1] and ms[4] are measured (the others read 0). */

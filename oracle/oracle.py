@@ -46,6 +46,9 @@ def _load_oracle():
             f = getattr(lib, f"oracle_step_{sfx}")
             f.restype = None
             f.argtypes = [C.c_void_p, C.c_float]
+            f = getattr(lib, f"oracle_step_planted_G_{sfx}")
+            f.restype = None
+            f.argtypes = [C.c_void_p, C.c_float, C.c_double]
             f = getattr(lib, f"oracle_field_{sfx}")
             f.restype = RP
             f.argtypes = [C.c_void_p, C.c_int]
@@ -205,6 +208,11 @@ class OracleState(_StateBase):
 
     def step(self, w):
         getattr(self.lib, f"oracle_step_{self.sfx}")(self.h, float(w))
+
+    def step_planted_G(self, w, G_threshold):
+        """A planted mistake for the tests: one step whose scalar operator takes its
+        threshold 0.01*G from G_threshold instead of this step's G."""
+        getattr(self.lib, f"oracle_step_planted_G_{self.sfx}")(self.h, float(w), float(G_threshold))
 
     @property
     def G(self):

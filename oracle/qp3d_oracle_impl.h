@@ -273,8 +273,13 @@ void SFX(oracle_destroy)(ST *s)
   free(s);
 }
 
-/* One time step in the order of modeling3D.c:53-81: grad, max, S, update, source, exchange. */
-void SFX(oracle_step)(ST *s, float w)
+/* One time step in the order of modeling3D.c:53-81: grad, max, S, update, source, exchange.
+ * Gthr < 0: the threshold of S is 0.01*G of this step (the reference). Gthr >= 0 plants a
+ * mistake for the tests: S takes its threshold from Gthr (s->G is still this step's G). */
+static void SFX(oracle_step_impl)(ST *s, float w, double Gthr);
+void SFX(oracle_step)(ST *s, float w) { SFX(oracle_step_impl)(s, w, -1.0); }
+void SFX(oracle_step_planted_G)(ST *s, float w, double Gthr) { SFX(oracle_step_impl)(s, w, Gthr); }
+static void SFX(oracle_step_impl)(ST *s, float w, double Gthr)
 {
   long n = (long)s->nx * s->ny * s->nz;
   SFX(oracle_grad)(s->u2, s->ux, s->uy, s->uz, s->c, s->nx, s->ny, s->nz, s->dx, s->dy, s->dz, s->nthreads);
@@ -284,6 +289,7 @@ void SFX(oracle_step)(ST *s, float w)
   REAL G = gx > gy ? gx : gy;
   G = G > gz ? G : gz;
   s->G = G;
+  if (Gthr >= 0) G = (REAL)Gthr;
   SFX(oracle_scalar_operator)(G, s->ux, s->uy, s->uz, s->eps, s->del, s->S, s->nx, s->ny, s->nz, s->nthreads);
   SFX(oracle_wavefield_update)(s);
   SFX(oracle_addsource)(s->src, w, s->u3, n, s->nthreads);

@@ -39,3 +39,7 @@ def s_fields_agree(a, b, tol=5e-4, max_flip=2e-3):
     d = np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64))
     far = d > tol
     return far.mean() <= max_flip and (not (~far).any() or d[~far].max() <= tol)
+
+
+# host restatement of the predicted-threshold band (gmax_mode 2), shared with bench.py
+from qp3d.threshold import s_threshold, spec_band, spec_valid_steps  # noqa: E402,F401

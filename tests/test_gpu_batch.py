@@ -18,12 +18,12 @@ def say(msg):
     sys.stdout.flush()
 
 
-def survey(nt=50, storage="fp32"):
+def survey(nt=50, storage="fp32", gmax_mode=1):
     model = qp3d.genmodel(70, 64, 30, 3)
     wins = [(0, 0, 31, 27), (20, 9, 44, 40), (35, 30, 26, 33), (5, 40, 60, 21)]
     setups = [qp3d.window_setup(model, x0, y0, hx, hy, pml=10, nt=nt, fm=20.0, src_depth=2)
               for x0, y0, hx, hy in wins]
-    return setups, [qp3d.Plan(s, storage=storage) for s in setups]
+    return setups, [qp3d.Plan(s, storage=storage, gmax_mode=gmax_mode) for s in setups]
 
 
 def alone(plans):
@@ -81,4 +81,23 @@ def test_batch_rejects_a_plan_uploaded_again():
                 b.run()
     finally:
         for p in plans:
+            p.close()
+
+
+def test_batch_runs_predicted_threshold_plans_with_the_gradient_pass():
+    """Plans created with gmax_mode 2 run inside a batch with the gradient pass (the batch
+    kernels take G from it), bit-identical to gmax_mode 1 plans alone, and run with the
+    predicted threshold again, bit-identical to before, once the batch is destroyed."""
+    setups, plans1 = survey(nt=30)
+    plans2 = [qp3d.Plan(s, gmax_mode=2) for s in setups]
+    try:
+        ref1, ref2 = alone(plans1), alone(plans2)
+        for per_shot in (False, True):
+            say(f"batch of gmax_mode 2 plans, per_shot={per_shot}")
+            with qp3d.Batch(plans2) as b:
+                b.set_per_shot(per_shot)
+                same(together(b, plans2), ref1)
+        same(alone(plans2), ref2)
+    finally:
+        for p in plans1 + plans2:
             p.close()

@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import rel_l2
+from conftest import rel_l2, spec_band, spec_valid_steps
 
 import qp3d
 from oracle import oracle as O
@@ -302,3 +302,23 @@ def test_import_state_round_trip():
         b.step(s.wavelet[k])
     for f in ("u1", "u2"):
         assert np.array_equal(bits(a.field(f)), bits(b.field(f)))
+
+
+def test_uncorrected_threshold_prediction_misses_the_parity_bar():
+    """Sensitivity of the GPU parity test of gmax_mode 2 (predicted threshold plus fix-up):
+    the oracle with a planted mistake, S thresholded at the predicted G instead of G_k (the
+    prediction without its fix-up), moves the gather of this anisotropic case far beyond the
+    1e-5 bar, and the case holds steps on both sides of the band test (fix-up from the update's
+    marks, and the marking pass)."""
+    s = qp3d.config_setup(hx=25, hy=27, nz0=23, pml=8, nt=100, seed=1, src_depth=8)
+    a, b = O.OracleState(s), O.OracleState(s)
+    ga, ra, rb = [], [], []
+    for k in range(s.nt):
+        a.step(s.wavelet[k])
+        ga.append(np.float32(a.G))
+        b.step_planted_G(s.wavelet[k], float(spec_band(ga, k)[0]))
+        ra.append(a.field("u2")[s.sz].copy())
+        rb.append(b.field("u2")[s.sz].copy())
+    valid = spec_valid_steps(np.array(ga, np.float32))
+    assert 20 <= valid.sum() <= s.nt - 20
+    assert rel_l2(np.array(rb), np.array(ra)) > 10 * 1e-5  # measured 1.4e-4

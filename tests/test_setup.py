@@ -187,7 +187,8 @@ def test_plan_create_rejects_bad_config_without_touching_the_device():
            Config(40, 40, 40, 10, 10, 10, 10, 10, 0.001, 40, 0, 0, 0, 0, 0, 0),   # sz outside
            Config(40, 40, 40, 10, 10, 10, 10, 10, 0.001, 10, 0, 0, 1, 0, 0, 0),   # snapflag, ntsnap 0
            Config(40, 40, 40, 10, 10, 10, 10, 10, 0.001, 10, 7, 0, 0, 0, 0, 0),   # unknown storage
-           Config(40, 40, 40, 10, 10, 10, 10, 10, 0.001, 10, 0, 0, 0, 0, 0, 0, snap_s=1)]  # S field, no snapshots
+           Config(40, 40, 40, 10, 10, 10, 10, 10, 0.001, 10, 0, 0, 0, 0, 0, 0, snap_s=1),  # S field, no snapshots
+           Config(40, 40, 40, 10, 10, 10, 10, 10, 0.001, 10, 0, 0, 0, 0, 0, 0, gmax_mode=3)]  # unknown G mode
     for cfg in bad:
         assert lib.qp3d_plan_create(C.byref(cfg), C.byref(h)) == QP3D_ERR_ARG
         assert lib.qp3d_last_error().decode()
