@@ -164,8 +164,8 @@ int qp3d_plan_step_maxima(const qp3d_plan *p, float *out);
 int qp3d_work_items(const qp3d_config *cfg, int which, int capacity, int *out, int cap);
 /* Interior extents hx, hy, hz of the plan's grid (any output may be NULL). */
 int qp3d_plan_dims(const qp3d_plan *p, int *hx, int *hy, int *hz);
-/* How the plan's steps obtain G (qp3d_config.gmax_mode after defaults: 1 or 2), or a
- * QP3D_* error. */
+/* How the plan's steps obtain G (qp3d_config.gmax_mode after defaults: 1 or 2; 1 while the
+ * plan belongs to a batch), or a QP3D_* error. */
 int qp3d_plan_gmax_mode(const qp3d_plan *p);
 /* 1: bracket every stencil launch with HIP events on the plan's stream; 2: only the interior
  * update (two events per step instead of four); 0: none. Resets the accumulated times. */

@@ -120,6 +120,24 @@ def test_checkpointed_equals_full_storage_and_bf16_checkpoints():
             assert err <= (1e-5 if key != "bf16" else 2e-2)
 
 
+def test_predicted_threshold_gradient():
+    """With gmax_mode 2 (predicted threshold and fix-up) the checkpoint recomputation
+    reproduces the forward run's steps, so the checkpointed gradient matches full storage to
+    the bar above, and the gradient matches gmax_mode 1's to rounding."""
+    s = small_setup(elliptic=False, nt=120)
+    obs = observed(s)
+    grads = {}
+    for key, mode, interval in (("g1", 1, 7), ("full", 2, 120), ("ck7", 2, 7)):
+        with qp3d.Plan(s, gmax_mode=mode) as p, qp3d.Fwi(p, interval=interval) as f:
+            rec = f.forward().astype(np.float64)
+            grads[key] = f.gradient((rec - obs).astype(np.float32))
+    for key, tol in (("ck7", 1e-5), ("g1", 1e-4)):
+        for a, b in zip(grads[key], grads["full"]):
+            err = np.linalg.norm(a - b) / np.linalg.norm(b)
+            say(f"gmax_mode 2: {key} vs full storage {err:.3e}")
+            assert err <= tol
+
+
 def test_autograd_shim_matches_direct_gradient():
     import torch
 
