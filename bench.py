@@ -411,9 +411,9 @@ def main():
         kt["update_interior"] = kt_timed["update_interior"]
     kb = plan.step_bytes()
     plan.set_instrument(0)
-    from qp3d.threshold import spec_valid_steps
+    from qp3d.threshold import steps_in_band
     gmode = plan.gmax_mode()
-    in_band = int(spec_valid_steps(plan.step_maxima()).sum()) if gmode == 2 else None
+    in_band = int(steps_in_band(plan.step_maxima(), plan.step_predictions()).sum()) if gmode == 2 else None
     log(f"timed {args.steps} shots in {wall:.3f} s; per step: {kt}")
     gbs = {k: kb[k] / (kt[k] * 1e-3) / 1e9 for k in ("gradmax", "update_interior", "update_pml") if kt[k] > 0}
     log(f"bytes per step {kb}; achieved GB/s {gbs}")

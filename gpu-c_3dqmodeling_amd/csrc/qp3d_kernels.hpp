@@ -152,6 +152,7 @@ struct UpdArgs {
   typename S::T *w21;                       // PML accumulator (updated in place)
   const float4 *pmltab;                     // [3][pml]: K1, K2, K3, E per face family and layer
   unsigned *gmax;                           // [nt] G per step (float bits); SPEC: G_k reduced here
+  float *gpred;                             // SPEC: [nt] predicted G per step (fix-up writes k+1)
   int *flag;                                // SPEC: set to 1 when a non-finite gradient is seen
   unsigned *fixmask;                        // SPEC: one bit per cell, cells to check after the step
   const float *wav;                         // [nt] wavelet
@@ -174,6 +175,7 @@ struct GradArgs {
   const int4 *work;
   Step step;
   unsigned *fixmask;  // marking pass: one bit per cell whose threshold decision changes
+  const float *gpred; // marking pass: predicted G per step
 };
 
 // A lane's fixed position in its tile and its in-plane offsets.
@@ -383,28 +385,19 @@ __device__ __forceinline__ float scalar_op(float ux, float uy, float uz, float e
 __device__ __forceinline__ float s_threshold(float G) { return __double2float_rd(1.0e-02 * (double)G); }
 
 // Predicted threshold (gmax_mode 2). The update of step k runs before G_k is known: it takes
-// ts = rd(0.01 Gp) with Gp extrapolated from G_{k-1} and G_{k-2} (ratio clamped to [1/2, 2]),
-// marks every cell whose |grad u| lies in the band (lo, hi] around ts, and reduces G_k itself.
-// The true threshold tt = rd(0.01 G_k) then changes the decision |grad u| > thr only for cells
-// between ts and tt: all of them are marked when lo <= tt <= hi (valid), and a marking pass
-// over the grid finds them otherwise.
-constexpr float SPEC_BAND = 1.f / 32.f;
+// ts = rd(0.01 Gp) with Gp = gpred[k], the prediction stored by the fix-up of step k-1
+// (predict_next), marks every cell whose |grad u| lies in the band (lo, hi] around ts, and
+// reduces G_k itself. The true threshold tt = rd(0.01 G_k) then changes the decision
+// |grad u| > thr only for cells between ts and tt: all of them are marked when lo <= tt <= hi
+// (valid), and a marking pass over the grid finds them otherwise.
+constexpr float SPEC_BAND = 1.f / 128.f;
 struct SpecBand {
   float ts, lo, hi;
 };
-__device__ __forceinline__ SpecBand spec_band(const unsigned *gmax, int k)
+__device__ __forceinline__ SpecBand spec_band(const float *gpred, int k)
 {
-  float gp = 0.f;
-  if (k >= 1) {
-    const float g1 = __uint_as_float(gmax[k - 1]);
-    gp = g1;
-    if (k >= 2) {
-      const float g2 = __uint_as_float(gmax[k - 2]);
-      if (g2 > 0.f && g1 > 0.f && g1 <= 3.4e38f) gp = g1 * fminf(fmaxf(g1 / g2, 0.5f), 2.f);
-    }
-  }
   SpecBand b;
-  b.ts = s_threshold(gp);
+  b.ts = s_threshold(k >= 1 ? gpred[k] : 0.f);
   b.lo = b.ts * (1.f - SPEC_BAND);
   b.hi = b.ts * (1.f + SPEC_BAND);
   return b;
@@ -463,7 +456,7 @@ __device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int k
   if (z0 >= z1) return;
   float ts = 0.f, tt = 0.f;
   if constexpr (MARK) {
-    const SpecBand sb = spec_band(a.gmax, kstep);
+    const SpecBand sb = spec_band(a.gpred, kstep);
     tt = s_threshold(__uint_as_float(a.gmax[kstep]));
     if (spec_valid(sb, tt)) return;
     ts = sb.ts;
@@ -636,7 +629,7 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
   // of cells to mark
   float thr, blo = 0.f, bhi = 0.f;
   if constexpr (SPEC) {
-    const SpecBand sb = spec_band(a.gmax, kstep);
+    const SpecBand sb = spec_band(a.gpred, kstep);
     thr = sb.ts;
     blo = sb.lo;
     bhi = sb.hi;
@@ -975,19 +968,133 @@ __device__ __forceinline__ void fix_cell(const UpdArgs<S> &a, unsigned i, float 
     a.rec[(size_t)k * (g.nx - 2 * pml) * (g.ny - 2 * pml) + (size_t)(x - pml) * (g.ny - 2 * pml) + (y - pml)] = unn;
 }
 
+// Prediction of G_{k+1} from G_0..G_k (double precision, deterministic): an AR(6) fit of
+// log G over the last 32 steps (least squares through the normal equations, centred at
+// log G_k, tiny ridge), which follows the few-step periodic jitter of G as wave crests cross
+// grid points; with fewer steps, or a non-positive G in the window, G_k (G_k / G_{k-1})
+// clamped to [1/2, 2]. The result is clamped to [G_k / 2, 2 G_k]. All threads of the calling
+// workgroup (256) take part.
+constexpr int AR_P = 6, AR_W = 32;
+__device__ __forceinline__ void predict_next(const unsigned *gmax, float *gpred, int k, int nt)
+{
+  __shared__ double lg[AR_P + AR_W];
+  __shared__ double sums[35];
+  __shared__ int okw;
+  if (k + 1 >= nt) return;  // uniform
+  const int t = threadIdx.x;
+  const int n0 = k + 1 - AR_W - AR_P;
+  const float gk = __uint_as_float(gmax[k]);
+  if (t == 0) okw = n0 >= 0 && gk > 0.f && gk <= 3.4e38f;
+  __syncthreads();
+  if (okw && t < AR_P + AR_W) {
+    const float gv = __uint_as_float(gmax[n0 + t]);
+    if (!(gv > 0.f && gv <= 3.4e38f)) okw = 0;
+    else lg[t] = log((double)gv) - log((double)gk);
+  }
+  __syncthreads();
+  const bool ar = okw;
+  if (ar && t < 35) {
+    // t < 28: entry (a, b), a <= b, of the 7x7 normal matrix; t >= 28: right-hand side a
+    int a = 0, b = 0;
+    if (t < 28) {
+      int r = t;
+      while (r >= 7 - a) { r -= 7 - a; a++; }
+      b = a + r;
+    } else {
+      a = t - 28;
+      b = -1;
+    }
+    double acc = 0.0;
+    for (int j = 0; j < AR_W; j++) {
+      const int n = AR_P + j;  // target index in the window
+      const double xa = a < AR_P ? lg[n - 1 - a] : 1.0;
+      const double xb = b < 0 ? lg[n] : (b < AR_P ? lg[n - 1 - b] : 1.0);
+      acc = fma(xa, xb, acc);
+    }
+    sums[t] = acc;
+  }
+  __syncthreads();
+  if (t != 0) return;
+  double gp = gk;
+  bool done = false;
+  if (ar) {
+    double M[7][8];
+    int idx = 0;
+    for (int a = 0; a < 7; a++)
+      for (int b = a; b < 7; b++, idx++) M[a][b] = M[b][a] = sums[idx];
+    double tr = 0.0;
+    for (int a = 0; a < 7; a++) {
+      M[a][7] = sums[28 + a];
+      tr += M[a][a];
+    }
+    for (int a = 0; a < 7; a++) M[a][a] += 1e-9 * tr / 7.0;
+    bool ok = tr > 0.0;
+    for (int c = 0; c < 7 && ok; c++) {  // Gaussian elimination, partial pivoting
+      int piv = c;
+      for (int r = c + 1; r < 7; r++)
+        if (fabs(M[r][c]) > fabs(M[piv][c])) piv = r;
+      if (!(fabs(M[piv][c]) > 1e-300)) { ok = false; break; }
+      if (piv != c)
+        for (int j = c; j < 8; j++) { const double x = M[c][j]; M[c][j] = M[piv][j]; M[piv][j] = x; }
+      for (int r = c + 1; r < 7; r++) {
+        const double f = M[r][c] / M[c][c];
+        for (int j = c; j < 8; j++) M[r][j] -= f * M[c][j];
+      }
+    }
+    if (ok) {
+      double cf[7];
+      for (int r = 6; r >= 0; r--) {
+        double v = M[r][7];
+        for (int j = r + 1; j < 7; j++) v -= M[r][j] * cf[j];
+        cf[r] = v / M[r][r];
+      }
+      double L = cf[AR_P];
+      for (int m = 0; m < AR_P; m++) L += cf[m] * lg[AR_P + AR_W - 1 - m];
+      if (L == L && fabs(L) < 50.0) {
+        gp = (double)gk * exp(L);
+        done = true;
+      }
+    }
+  }
+  if (!done && k >= 1) {
+    const float g1 = __uint_as_float(gmax[k - 1]);
+    if (g1 > 0.f && gk > 0.f && gk <= 3.4e38f) gp = (double)gk * fmin(fmax((double)gk / (double)g1, 0.5), 2.0);
+  }
+  if (gk > 0.f && gk <= 3.4e38f) gp = fmin(fmax(gp, 0.5 * (double)gk), 2.0 * (double)gk);
+  gpred[k + 1] = (float)gp;
+}
+
 template <class S>
-__global__ __launch_bounds__(256) void k_spec_fix(UpdArgs<S> a, long nwords)
+__global__ __launch_bounds__(256) void k_spec_fix(UpdArgs<S> a, long nwords, int nt)
 {
   const int k = a.step.get();
-  const float ts = spec_band(a.gmax, k).ts, tt = s_threshold(__uint_as_float(a.gmax[k]));
-  for (long w = blockIdx.x * (long)blockDim.x + threadIdx.x; w < nwords; w += (long)gridDim.x * blockDim.x) {
-    unsigned bits = a.fixmask[w];
-    if (!bits) continue;
-    a.fixmask[w] = 0u;
-    while (bits) {
-      const int j = __builtin_ctz(bits);
-      bits &= bits - 1u;
-      fix_cell<S>(a, (unsigned)w * 32u + (unsigned)j, ts, tt, k);
+  if (blockIdx.x == 0) predict_next(a.gmax, a.gpred, k, nt);  // for the next step's updates
+  const float ts = spec_band(a.gpred, k).ts, tt = s_threshold(__uint_as_float(a.gmax[k]));
+  // four mask words per thread and iteration (one 16-byte load)
+  const long nq = (nwords + 3) / 4;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < nq; q += (long)gridDim.x * blockDim.x) {
+    const long w0 = 4 * q;
+    uint4 v;
+    if (w0 + 4 <= nwords) {
+      v = *reinterpret_cast<const uint4 *>(a.fixmask + w0);
+    } else {
+      v.x = a.fixmask[w0];
+      v.y = w0 + 1 < nwords ? a.fixmask[w0 + 1] : 0u;
+      v.z = w0 + 2 < nwords ? a.fixmask[w0 + 2] : 0u;
+      v.w = 0u;
+    }
+    if (!(v.x | v.y | v.z | v.w)) continue;
+    const unsigned wv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      unsigned bits = wv[i];
+      if (!bits) continue;
+      a.fixmask[w0 + i] = 0u;
+      while (bits) {
+        const int j = __builtin_ctz(bits);
+        bits &= bits - 1u;
+        fix_cell<S>(a, (unsigned)(w0 + i) * 32u + (unsigned)j, ts, tt, k);
+      }
     }
   }
 }

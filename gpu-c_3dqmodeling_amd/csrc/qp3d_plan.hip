@@ -450,7 +450,10 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
   if (p->spec) {
     p->fixwords = (p->N + 31) / 32;
     if ((rc = dmalloc((void **)&p->fixmask, sizeof(unsigned) * (size_t)p->fixwords, "cell mask"))) return fail(rc);
-    if (hipMemset(p->fixmask, 0, sizeof(unsigned) * (size_t)p->fixwords) != hipSuccess) {
+    if ((rc = dmalloc((void **)&p->gpred, sizeof(float) * (size_t)(c.nt > 0 ? c.nt : 1), "predicted maxima")))
+      return fail(rc);
+    if (hipMemset(p->fixmask, 0, sizeof(unsigned) * (size_t)p->fixwords) != hipSuccess ||
+        hipMemset(p->gpred, 0, sizeof(float) * (size_t)(c.nt > 0 ? c.nt : 1)) != hipSuccess) {
       qp3d_set_error("plan_create: clearing the cell mask failed");
       return fail(QP3D_ERR_HIP);
     }
@@ -489,6 +492,7 @@ extern "C" int qp3d_plan_destroy(qp3d_plan *p)
   (void)hipFree(p->gmax);
   (void)hipFree(p->flag);
   (void)hipFree(p->fixmask);
+  (void)hipFree(p->gpred);
   (void)hipFree(p->kdev);
   (void)hipFree(p->wav);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
@@ -721,6 +725,8 @@ extern "C" int qp3d_plan_reset(qp3d_plan *p)
   HIP_TRY(hipMemsetAsync(p->gmax, 0, sizeof(unsigned) * (size_t)(p->cfg.nt > 0 ? p->cfg.nt : 1), p->stream));
   HIP_TRY(hipMemsetAsync(p->flag, 0, sizeof(int), p->stream));
   if (p->fixmask) HIP_TRY(hipMemsetAsync(p->fixmask, 0, sizeof(unsigned) * (size_t)p->fixwords, p->stream));
+  if (p->gpred)
+    HIP_TRY(hipMemsetAsync(p->gpred, 0, sizeof(float) * (size_t)(p->cfg.nt > 0 ? p->cfg.nt : 1), p->stream));
   if (p->rec) HIP_TRY(hipMemsetAsync(p->rec, 0, sizeof(float) * (size_t)p->cfg.nt * p->hx * p->hy, p->stream));
   p->nsnap_taken = 0;
   p->steps_done = 0;
@@ -816,8 +822,8 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on
     // once when the prediction held), then the fix-up of the marked cells
     launch_mark(p->tile_g, p->wl_grad, s0, ga);
     HIP_TRY(hipGetLastError());
-    const int blocks = (int)std::min<long>((p->fixwords + 255) / 256, 2048);
-    hipLaunchKernelGGL(k_spec_fix<S>, dim3(blocks), dim3(256), 0, s0, ua, p->fixwords);
+    const int blocks = (int)std::min<long>((p->fixwords / 4 + 256) / 256, 8192);
+    hipLaunchKernelGGL(k_spec_fix<S>, dim3(blocks), dim3(256), 0, s0, ua, p->fixwords, cf.nt);
     HIP_TRY(hipGetLastError());
     if (e && all) HIP_TRY(hipEventRecord(e[4], s0));
   }
@@ -1148,6 +1154,20 @@ extern "C" int qp3d_plan_step_maxima(const qp3d_plan *p, float *out)
   HIP_TRY(hipSetDevice(p->cfg.device));
   HIP_TRY(hipStreamSynchronize(p->stream));
   return copy_out(p, out, (const float *)p->gmax, (size_t)p->cfg.nt);
+}
+
+extern "C" int qp3d_plan_step_predictions(const qp3d_plan *p, float *out)
+{
+  if (!p || !out) {
+    qp3d_set_error("plan_step_predictions: null argument");
+    return QP3D_ERR_ARG;
+  }
+  if (!p->gpred) {
+    qp3d_set_error("plan_step_predictions: the plan was created with gmax_mode != 2");
+    return QP3D_ERR_STATE;
+  }
+  HIP_TRY(hipSetDevice(p->cfg.device));
+  return copy_out(p, out, (const float *)p->gpred, (size_t)p->cfg.nt);
 }
 
 extern "C" int qp3d_plan_gmax_mode(const qp3d_plan *p)

@@ -67,6 +67,7 @@ struct qp3d_plan {
   int *flag = nullptr;
   bool spec = false;             // gmax_mode 2 (a batch turns it off for its plans)
   unsigned *fixmask = nullptr;   // gmax_mode 2: one bit per cell, all clear between steps
+  float *gpred = nullptr;        // gmax_mode 2: [nt] predicted G per step
   long fixwords = 0;
   float *rec = nullptr;      // [nt][hx][hy]
   float *sbox = nullptr;
@@ -140,6 +141,7 @@ inline void make_args(const qp3d_plan *p, int prev, Step st, GradArgs<S> &ga, Up
   ga.work = p->wl_grad.d;
   ga.step = st;
   ga.fixmask = p->fixmask;
+  ga.gpred = p->gpred;
   ua.g = p->geo;
   ua.c = p->coef;
   ua.ucur = (const T *)p->u[cur];
@@ -153,6 +155,7 @@ inline void make_args(const qp3d_plan *p, int prev, Step st, GradArgs<S> &ga, Up
   ua.gmax = p->gmax;
   ua.flag = p->flag;
   ua.fixmask = p->fixmask;
+  ua.gpred = p->gpred;
   ua.wav = p->wav;
   ua.step = st;
   ua.src_on = p->src_on ? 1 : 0;

@@ -106,6 +106,12 @@ class Plan:
         check(self.lib.qp3d_plan_step_maxima(self.h, fptr(out)))
         return out
 
+    def step_predictions(self):
+        """gmax_mode 2: the predicted G of every step (the threshold and band it was run with)."""
+        out = np.zeros(self.s.nt, np.float32)
+        check(self.lib.qp3d_plan_step_predictions(self.h, fptr(out)))
+        return out
+
     def gmax_mode(self):
         """1: a gradient pass gives G before each update; 2: predicted threshold with fix-up."""
         rc = self.lib.qp3d_plan_gmax_mode(self.h)

@@ -108,10 +108,10 @@ typedef struct qp3d_config {
   int gmax_mode;   /* how a step obtains G, the gradient maximum that sets the scalar operator's
                       threshold 0.01*G: 1: a gradient pass over the grid before the update (the
                       reference's order); 2: the update kernels use the threshold predicted from
-                      the two previous steps' G, reduce this step's G themselves, and a fix-up
+                      the previous steps' G, reduce this step's G themselves, and a fix-up
                       kernel recomputes the cells whose decision |grad u| > 0.01*G the prediction
                       got wrong (a marking pass over the grid finds them when the prediction
-                      missed by more than 1/32); the same results up to the rounding of those
+                      missed by more than 1/128); the same results up to the rounding of those
                       cells; implies streams = 1. 0 = default (1) */
 } qp3d_config;
 
@@ -164,6 +164,10 @@ int qp3d_plan_step_maxima(const qp3d_plan *p, float *out);
 int qp3d_work_items(const qp3d_config *cfg, int which, int capacity, int *out, int cap);
 /* Interior extents hx, hy, hz of the plan's grid (any output may be NULL). */
 int qp3d_plan_dims(const qp3d_plan *p, int *hx, int *hy, int *hz);
+/* gmax_mode 2: the predicted G of every step, out[nt] (0 for step 0), from which each step's
+ * threshold and marked band were taken (the prediction of step k is fitted to G_0..G_{k-1});
+ * QP3D_ERR_STATE for a plan created with another mode. */
+int qp3d_plan_step_predictions(const qp3d_plan *p, float *out);
 /* How the plan's steps obtain G (qp3d_config.gmax_mode after defaults: 1 or 2; 1 while the
  * plan belongs to a batch), or a QP3D_* error. */
 int qp3d_plan_gmax_mode(const qp3d_plan *p);

@@ -42,4 +42,4 @@ def s_fields_agree(a, b, tol=5e-4, max_flip=2e-3):
 
 
 # host restatement of the predicted-threshold band (gmax_mode 2), shared with bench.py
-from qp3d.threshold import s_threshold, spec_band, spec_valid_steps  # noqa: E402,F401
+from qp3d.threshold import band, predict_G, s_threshold, steps_in_band  # noqa: E402,F401

