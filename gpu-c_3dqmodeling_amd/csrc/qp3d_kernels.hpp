@@ -979,20 +979,17 @@ __device__ __forceinline__ void predict_next(const unsigned *gmax, float *gpred,
 {
   __shared__ double lg[AR_P + AR_W];
   __shared__ double sums[35];
-  __shared__ int okw;
   if (k + 1 >= nt) return;  // uniform
   const int t = threadIdx.x;
   const int n0 = k + 1 - AR_W - AR_P;
   const float gk = __uint_as_float(gmax[k]);
-  if (t == 0) okw = n0 >= 0 && gk > 0.f && gk <= 3.4e38f;
-  __syncthreads();
-  if (okw && t < AR_P + AR_W) {
+  int ok = n0 >= 0 && gk > 0.f && gk <= 3.4e38f;  // uniform
+  if (ok && t < AR_P + AR_W) {
     const float gv = __uint_as_float(gmax[n0 + t]);
-    if (!(gv > 0.f && gv <= 3.4e38f)) okw = 0;
-    else lg[t] = log((double)gv) - log((double)gk);
+    if (gv > 0.f && gv <= 3.4e38f) lg[t] = log((double)gv) - log((double)gk);
+    else ok = 0;
   }
-  __syncthreads();
-  const bool ar = okw;
+  const bool ar = __syncthreads_and(ok);  // every G of the window usable; lg complete
   if (ar && t < 35) {
     // t < 28: entry (a, b), a <= b, of the 7x7 normal matrix; t >= 28: right-hand side a
     int a = 0, b = 0;
