@@ -7,7 +7,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import rel_l2, s_fields_agree, steps_in_band
+from conftest import predict_G, rel_l2, s_fields_agree, steps_in_band
 
 import qp3d
 from oracle import oracle as O
@@ -414,6 +414,10 @@ def test_predicted_threshold_against_the_oracle():
     valid = steps_in_band(gm, gpred[0])
     say(f"predicted threshold: {valid.sum()} of {s.nt} steps inside the marked band")
     assert 5 <= valid.sum() <= s.nt - 20
+    # the device's predictions (AR fit after 38 steps, ratio rule before) against the host
+    # restatement on the same maxima (not bit for bit: log and exp round differently)
+    host = np.array([predict_G(gm, k) for k in range(s.nt)], np.float32)
+    np.testing.assert_allclose(gpred[0], host, rtol=1e-5, atol=0)
     assert rel_l2(rec, orec) <= TOL
     assert rel_l2(st[0], o.field("u2")) <= TOL
     assert rel_l2(st[1], o.field("u1")) <= TOL
@@ -423,6 +427,9 @@ def test_predicted_threshold_against_the_oracle():
         p.run()
         p.sync()
         rec1, st1 = p.record(), [p.field(w) for w in range(9)]
+        with pytest.raises(qp3d.Qp3dError) as ei:
+            p.step_predictions()
+        assert ei.value.code == -6
     for which, arr in PML_PAIRS:
         ref = _unified(o, s, arr)
         e1, e2, d = rel_l2(st1[which], ref), rel_l2(st[which], ref), rel_l2(st[which], st1[which])
