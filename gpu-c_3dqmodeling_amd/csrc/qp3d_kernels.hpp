@@ -15,6 +15,10 @@
 //                    (GPU_kernel.cu:566-637).
 // k_sfield forms the scalar-operator field of a snapshot step on request (the reference's
 // writer for it is commented out, 3dPureqPModeling_single_shot.cu:381-409).
+// Optionally (SPEC, gmax_mode 2) there is no gradient pass: k_update<..., SPEC> takes a
+// threshold predicted from the previous steps' G, reduces G itself and marks the cells near the
+// predicted threshold; then k_spec_mark (only when the true threshold left the marked band)
+// and k_spec_fix correct the cells whose decision the prediction got wrong.
 // Tiles are aligned to the PML boundary, so an interior tile holds only interior cells and a
 // PML tile only cells of one face family (x faces own edges and corners, then y, then z),
 // which makes the family uniform over a workgroup and keeps the PML state out of the interior
