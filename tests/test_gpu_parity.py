@@ -440,11 +440,11 @@ def test_predicted_threshold_against_the_oracle():
     assert max(d) <= 5e-6
 
 
-@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
 def test_predicted_threshold_graph_replay_and_storage(storage):
     """gmax_mode 2 replayed from the captured graph (runs split at odd steps, snapshot
-    steps) is bit-identical to its direct launches, in FP32 and in FP16 storage; in FP16 it
-    stays within the storage's rounding of gmax_mode 1."""
+    steps) is bit-identical to its direct launches, in FP32 and in 16-bit storage; in 16 bits
+    it stays within the storage's rounding of gmax_mode 1."""
     s = qp3d.config_setup(hx=29, hy=23, nz0=21, pml=8, nt=64, seed=6, src_depth=6)
     say(f"predicted threshold, graph, {storage}")
     res = {}
@@ -462,4 +462,4 @@ def test_predicted_threshold_graph_replay_and_storage(storage):
         assert ka == kb and np.array_equal(sa, sb)
     d = rel_l2(a[0], res[1, False][0])
     say(f"gmax_mode 2 vs 1 ({storage}): gather {d:.2e}")
-    assert d <= (1e-6 if storage == "fp32" else 2e-3)
+    assert d <= {"fp32": 1e-6, "fp16": 2e-3, "bf16": 2e-2}[storage]
