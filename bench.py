@@ -105,7 +105,7 @@ def bench_fwi(args):
     s_true = qp3d.config_setup(hx=n, hy=n, nz0=n, pml=args.pml, nt=nt, seed=2)
     s0 = smoothed_setup(qp3d, n, n, n, args.pml, nt, seed=2, sigma=15.0)
     log(f"fwi setup {s0.nx}^3, nt {nt}, {time.perf_counter() - t:.1f} s")
-    with qp3d.Plan(s_true) as p:
+    with qp3d.Plan(s_true, gmax_mode=args.gmax_mode) as p:
         obs = p.forward()
         t = time.perf_counter()
         p.forward()
@@ -118,7 +118,7 @@ def bench_fwi(args):
         cases.append(("full_storage", nt, "fp32"))
     for key, interval, ck in cases:
         try:
-            with qp3d.Plan(s0) as p, qp3d.Fwi(p, interval=interval, ckpt_storage=ck) as f:
+            with qp3d.Plan(s0, gmax_mode=args.gmax_mode) as p, qp3d.Fwi(p, interval=interval, ckpt_storage=ck) as f:
                 f.forward()
                 p.sync()
                 t = time.perf_counter()
