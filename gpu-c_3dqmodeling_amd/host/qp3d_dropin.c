@@ -92,6 +92,10 @@ int qp3d_modeling3d(int nx, int ny, int nz, int nt, int ntsnap, float dx, float 
      writer commented out, so it is off by default) */
   const char *ss = getenv("QP3D_SNAP_S");
   cfg.snap_s = cfg.snapflag && ss && atoi(ss) == 1;
+  /* QP3D_GMAX_MODE=2 takes G from the update kernels under a predicted threshold (the same
+     results up to rounding; see qp3d_config.gmax_mode); unset or 0: the default */
+  const char *gm = getenv("QP3D_GMAX_MODE");
+  cfg.gmax_mode = gm ? atoi(gm) : 0;
   qp3d_plan *p = NULL;
   int rc = qp3d_plan_create(&cfg, &p);
   if (rc) return rc;

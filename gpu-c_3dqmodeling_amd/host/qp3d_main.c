@@ -19,7 +19,8 @@
  * top of the interior (the reference forces depth 0, Main.c:200); QP3D_SNAP_S=1 makes
  * modeling3d() also write the S field of every snapshot step (<snap_file>_S<k>.dat, _Sxz,
  * _Syz, _Sxy; the reference's writer for them is commented out,
- * 3dPureqPModeling_single_shot.cu:381-409).
+ * 3dPureqPModeling_single_shot.cu:381-409); QP3D_GMAX_MODE=2 makes modeling3d() take the
+ * gradient maximum from the update kernels under a predicted threshold (qp3d_config.gmax_mode).
  */
 #include <stdio.h>
 #include <stdlib.h>

@@ -82,8 +82,13 @@ def oracle_run(d, p, model):
                 np.ascontiguousarray(full[:, sx1, :]).tofile(os.path.join(d, "output/recordyz.dat"))
 
 
-def test_qp3d_main_matches_the_oracle_driver(tmp_path):
+@pytest.mark.parametrize("gmax_mode", ["", "2"])
+def test_qp3d_main_matches_the_oracle_driver(tmp_path, monkeypatch, gmax_mode):
+    """Every output file of qp3d_main against the oracle driver's; with QP3D_GMAX_MODE=2 the
+    drop-in call takes G from the update kernels (predicted threshold and fix-up)."""
     assert os.path.exists(MAIN), "qp3d_main is not built"
+    if gmax_mode:
+        monkeypatch.setenv("QP3D_GMAX_MODE", gmax_mode)
     p = dict(nt=80, ntsnap=40, Nx0=40, Ny0=36, Nz0=24, sx0=14, sy0=15, nxs=2, nys=1, dxs=8, dys=6, hx=21, hy=19,
              pml=10)
     model = qp3d.genmodel(p["Nx0"], p["Ny0"], p["Nz0"], 5)  # file layout [y][x][z]
