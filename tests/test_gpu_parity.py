@@ -267,13 +267,15 @@ def test_low_precision_storage_against_fp64(storage):
     assert rel_l2(snaps[1][1], s32[1][1]) <= 2 * LP_TOL[storage]
 
 
-def test_s_field_snapshots_against_the_oracle():
+@pytest.mark.parametrize("gmax_mode", [1, 2])
+def test_s_field_snapshots_against_the_oracle(gmax_mode):
     """S-field snapshots (snap_s): at each snapshot step k the scalar operator S the update of
     step k used, over the padded grid, against the oracle's S of the same step; exactly 1
-    outside [5,n-5)^3 and 1 everywhere at step 0 (u = 0)."""
+    outside [5,n-5)^3 and 1 everywhere at step 0 (u = 0). With gmax_mode 2, G_k comes from
+    the update kernels and the S of the marked cells from the fix-up."""
     s = qp3d.config_setup(hx=37, hy=41, nz0=33, pml=10, nt=101, seed=3, src_depth=14)
-    say("S field snapshots")
-    with qp3d.Plan(s, snapflag=1, ntsnap=50, snap_s=1) as p:
+    say(f"S field snapshots, gmax_mode {gmax_mode}")
+    with qp3d.Plan(s, snapflag=1, ntsnap=50, snap_s=1, gmax_mode=gmax_mode) as p:
         p.forward()
         got = dict(p.snapshots_s())
         snaps = p.snapshots()
