@@ -374,9 +374,11 @@ def test_device_pointers_match_host_pointers(storage):
             b.free()
 
 
-def test_run_on_a_caller_stream():
+@pytest.mark.parametrize("gmax_mode", [1, 2])
+def test_run_on_a_caller_stream(gmax_mode):
     """qp3d_plan_run_on orders the steps with a caller's stream: after the caller's stream has
-    drained, the record equals a plain run's."""
+    drained, the record equals a plain run's (with gmax_mode 2 the marking pass and the fix-up
+    go on the caller's stream as well)."""
     import ctypes as C
     hip = _hip()
     hip.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
@@ -386,7 +388,7 @@ def test_run_on_a_caller_stream():
     st = C.c_void_p()
     assert hip.hipStreamCreate(C.byref(st)) == 0
     try:
-        with qp3d.Plan(s) as p:
+        with qp3d.Plan(s, gmax_mode=gmax_mode) as p:
             ref = p.forward()
             p.reset()
             assert p.lib.qp3d_plan_run_on(p.h, 0, s.nt, st) == 0
