@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <vector>
 
+#include "qp3d_launch.hpp"
 #include "qp3d_plan_impl.hpp"
 
 struct qp3d_batch {
@@ -37,57 +38,16 @@ struct qp3d_batch {
   double timed_ms = 0;
 };
 
-namespace {
-
-template <class S>
-void launch_grad_b(int tile, const WorkList &w, hipStream_t st, const GradArgs<S> *A, Step s)
-{
-  int i = 0;
-#define QP3D_L(wx, rx)                                                                                 \
-  if (tile == i++) {                                                                                   \
-    hipLaunchKernelGGL((k_gradmax_batch<S, Tile<wx, rx>>), dim3(w.n), dim3(TY, wx), 0, st, A, w.d, s); \
-    return;                                                                                            \
-  }
-  QP3D_TILES(QP3D_L)
-#undef QP3D_L
-}
-
-template <class S, bool PML>
-void launch_upd_b(int tile, const WorkList &w, hipStream_t st, const UpdArgs<S> *A, Step s)
-{
-  int i = 0;
-#define QP3D_L(wx, rx)                                                                                       \
-  if (tile == i++) {                                                                                         \
-    hipLaunchKernelGGL((k_update_batch<S, Tile<wx, rx>, PML>), dim3(w.n), dim3(TY, wx), 0, st, A, w.d, s);  \
-    return;                                                                                                  \
-  }
-  QP3D_TILES(QP3D_L)
-#undef QP3D_L
-}
-
-template <class S>
-int occupancy_b(int stage, int tile)
-{
-  int i = 0, occ = 1;
-#define QP3D_O(wx, rx)                                                                                 \
-  if (tile == i++) {                                                                                   \
-    const void *fn = stage == 0 ? reinterpret_cast<const void *>(&k_gradmax_batch<S, Tile<wx, rx>>)   \
-                     : stage == 1 ? reinterpret_cast<const void *>(&k_update_batch<S, Tile<wx, rx>, false>) \
-                                  : reinterpret_cast<const void *>(&k_update_batch<S, Tile<wx, rx>, true>);  \
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, TY * wx, 0) != hipSuccess) occ = 1;     \
-  }
-  QP3D_TILES(QP3D_O)
-#undef QP3D_O
-  return std::max(occ, 1);
-}
-
-}  // namespace
-
 // Work list of one stage over all shots, items in batch format.
 static int build_stage(qp3d_batch *b, int stage)
 {
   int occ = 1;
-  with_storage(b->storage, [&](auto sp) { occ = occupancy_b<decltype(sp)>(stage, b->tile[stage]); });
+  with_storage(b->storage, [&](auto sp) {
+    using S = decltype(sp);
+    occ = stage == 0 ? occupancy_gradmax_batch<S>(b->tile[0])
+          : stage == 1 ? occupancy_update_batch<S, false>(b->tile[1])
+                       : occupancy_update_batch<S, true>(b->tile[stage]);
+  });
   const int n = (int)b->plans.size();
   const int cap = std::max(1, occ * qp3d_cu_count(b->plans[0]) / n);  // the shots share the machine
   std::vector<int4> all;
@@ -140,11 +100,13 @@ static int batch_step_s(qp3d_batch *b, int k, Step st)
     if (tm) HIP_TRY(hipEventRecord(b->ev[0], b->stream));
     if (b->wl[s].n) {
       if (s == 0)
-        launch_grad_b<S>(b->tile[0], b->wl[0], b->stream, (const GradArgs<S> *)b->args_g[par], st);
+        launch_gradmax_batch<S>(b->tile[0], b->wl[0].n, b->wl[0].d, b->stream, (const GradArgs<S> *)b->args_g[par], st);
       else if (s == 1)
-        launch_upd_b<S, false>(b->tile[1], b->wl[1], b->stream, (const UpdArgs<S> *)b->args_u[par], st);
+        launch_update_batch<S, false>(b->tile[1], b->wl[1].n, b->wl[1].d, b->stream, (const UpdArgs<S> *)b->args_u[par],
+                                      st);
       else
-        launch_upd_b<S, true>(b->tile[s], b->wl[s], b->stream, (const UpdArgs<S> *)b->args_u[par], st);
+        launch_update_batch<S, true>(b->tile[s], b->wl[s].n, b->wl[s].d, b->stream, (const UpdArgs<S> *)b->args_u[par],
+                                     st);
       HIP_TRY(hipGetLastError());
     }
     if (tm) {
@@ -179,7 +141,10 @@ static int batch_step(qp3d_batch *b, int k, Step st)
 static void batch_free(qp3d_batch *b)
 {
   if (!b) return;
-  for (size_t i = 0; i < b->spec_saved.size(); i++) b->plans[i]->spec = b->spec_saved[i] != 0;
+  for (size_t i = 0; i < b->spec_saved.size(); i++) {
+    b->plans[i]->spec = b->spec_saved[i] != 0;
+    b->plans[i]->in_batch = 0;
+  }
   for (WorkList &w : b->wl) (void)hipFree(w.d);
   for (int i = 0; i < 2; i++) {
     (void)hipFree(b->args_g[i]);
@@ -212,6 +177,17 @@ extern "C" int qp3d_batch_create(qp3d_plan *const *plans, int nshots, int use_gr
       qp3d_set_error("batch_create: plan %d differs in nt, storage or device from plan 0", i);
       return QP3D_ERR_ARG;
     }
+    // one plan holds one shot's wavefields, record and maxima: it can be driven by one batch
+    // once, never twice in a batch or by two live batches
+    if (p->in_batch) {
+      qp3d_set_error("batch_create: plan %d already belongs to a live batch", i);
+      return QP3D_ERR_STATE;
+    }
+    for (int j = 0; j < i; j++)
+      if (plans[j] == p) {
+        qp3d_set_error("batch_create: plan %d appears twice (also as plan %d)", i, j);
+        return QP3D_ERR_ARG;
+      }
   }
   HIP_TRY(hipSetDevice(c0.device));
   qp3d_batch *b = new qp3d_batch();
@@ -220,6 +196,7 @@ extern "C" int qp3d_batch_create(qp3d_plan *const *plans, int nshots, int use_gr
   for (int i = 0; i < nshots; i++) {
     b->spec_saved.push_back(plans[i]->spec ? 1 : 0);
     plans[i]->spec = false;
+    plans[i]->in_batch = 1;
   }
   b->nt = c0.nt;
   b->storage = c0.storage;

@@ -1,0 +1,38 @@
+// Launchers of the stencil kernels, one per (kernel, storage policy), dispatching a runtime
+// tile index (QP3D_TILES order) to the kernel instantiated for that tile shape; `blocks` is the
+// length of the work list the arguments point to. The kernels are instantiated in their own
+// translation units (qp3d_k_grad.hip and qp3d_k_upd.hip, built once per storage policy and
+// update kind), so that the library compiles in parallel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "qp3d_kernels.hpp"
+
+namespace qp3d {
+
+// gradient-max pass (k_gradmax) and its marking variant (k_spec_mark)
+template <class S>
+void launch_gradmax(int tile, int blocks, hipStream_t st, const GradArgs<S> &a);
+template <class S>
+void launch_mark(int tile, int blocks, hipStream_t st, const GradArgs<S> &a);
+// resident workgroups per CU of k_gradmax for a tile shape
+template <class S>
+int occupancy_gradmax(int tile);
+
+// update pass (k_update<PML>; spec: the predicted-threshold form)
+template <class S, bool PML>
+void launch_update(int tile, int blocks, hipStream_t st, const UpdArgs<S> &a, bool spec);
+template <class S, bool PML>
+int occupancy_update(int tile, bool spec);
+
+// batched forms (argument blocks in device memory, items carry the grid index)
+template <class S>
+void launch_gradmax_batch(int tile, int blocks, const int4 *work, hipStream_t st, const GradArgs<S> *A, Step s);
+template <class S, bool PML>
+void launch_update_batch(int tile, int blocks, const int4 *work, hipStream_t st, const UpdArgs<S> *A, Step s);
+template <class S>
+int occupancy_gradmax_batch(int tile);
+template <class S, bool PML>
+int occupancy_update_batch(int tile);
+
+}  // namespace qp3d

@@ -9,66 +9,8 @@
 #include <cstring>
 #include <vector>
 
+#include "qp3d_launch.hpp"
 #include "qp3d_plan_impl.hpp"
-
-// ---------------------------------------------------------------------------------------
-// tile shapes and launches
-
-template <class S>
-static void launch_gradmax(int tile, const WorkList &w, hipStream_t st, const GradArgs<S> &a)
-{
-  int i = 0;
-#define QP3D_L(wx, rx)                                                                                 \
-  if (tile == i++) {                                                                                   \
-    hipLaunchKernelGGL((k_gradmax<S, Tile<wx, rx>>), dim3(w.n), dim3(TY, wx), 0, st, a);               \
-    return;                                                                                            \
-  }
-  QP3D_TILES(QP3D_L)
-#undef QP3D_L
-}
-
-template <class S>
-static void launch_mark(int tile, const WorkList &w, hipStream_t st, const GradArgs<S> &a)
-{
-  int i = 0;
-#define QP3D_L(wx, rx)                                                                                 \
-  if (tile == i++) {                                                                                   \
-    hipLaunchKernelGGL((k_spec_mark<S, Tile<wx, rx>>), dim3(w.n), dim3(TY, wx), 0, st, a);             \
-    return;                                                                                            \
-  }
-  QP3D_TILES(QP3D_L)
-#undef QP3D_L
-}
-
-template <class S, bool PML>
-static void launch_update(int tile, const WorkList &w, hipStream_t st, const UpdArgs<S> &a, bool spec)
-{
-  int i = 0;
-#define QP3D_L(wx, rx)                                                                                 \
-  if (tile == i++) {                                                                                   \
-    if (spec)                                                                                          \
-      hipLaunchKernelGGL((k_update<S, Tile<wx, rx>, PML, true>), dim3(w.n), dim3(TY, wx), 0, st, a);   \
-    else                                                                                               \
-      hipLaunchKernelGGL((k_update<S, Tile<wx, rx>, PML>), dim3(w.n), dim3(TY, wx), 0, st, a);         \
-    return;                                                                                            \
-  }
-  QP3D_TILES(QP3D_L)
-#undef QP3D_L
-}
-
-template <class F>
-static int occupancy_of(int tile, F pick)
-{
-  int i = 0, occ = 0;
-#define QP3D_O(wx, rx)                                                                                 \
-  if (tile == i++) {                                                                                   \
-    const void *fn = pick(Tile<wx, rx>{});                                                             \
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, TY * wx, 0) != hipSuccess) occ = 1;     \
-  }
-  QP3D_TILES(QP3D_O)
-#undef QP3D_O
-  return std::max(occ, 1);
-}
 
 // ---------------------------------------------------------------------------------------
 // work decomposition
@@ -134,16 +76,23 @@ std::vector<int4> xcd_order(const std::vector<int4> &logical)
 }
 }  // namespace
 
-int qp3d_upload_items(WorkList &w, const std::vector<int4> &logical, bool batch)
+long qp3d_order_items(const std::vector<int4> &logical, bool batch, std::vector<int4> &ordered)
 {
-  std::vector<int4> v = xcd_order(logical);
-  w.n = (int)v.size();
-  w.cells = 0;
+  ordered = xcd_order(logical);
+  long cells = 0;
   for (const int4 &t : logical) {
     const long ny_ = (long)((((unsigned)t.y >> 16) & 0x7fff) - (t.y & 0xffff)) * (t.y < 0 ? 2 : 1);
     const long nz_ = batch ? (long)(((unsigned)t.z >> 16) - (t.z & 0xffff)) : (long)(t.w - t.z);
-    w.cells += (long)(((unsigned)t.x >> 16) - (t.x & 0xffff)) * ny_ * nz_;
+    cells += (long)(((unsigned)t.x >> 16) - (t.x & 0xffff)) * ny_ * nz_;
   }
+  return cells;
+}
+
+int qp3d_upload_items(WorkList &w, const std::vector<int4> &logical, bool batch)
+{
+  std::vector<int4> v;
+  w.cells = qp3d_order_items(logical, batch, v);
+  w.n = (int)v.size();
   (void)hipFree(w.d);
   w.d = nullptr;
   if (w.n == 0) return QP3D_OK;
@@ -287,36 +236,34 @@ int qp3d_build_update_lists(const qp3d_plan *p, int tile_i, int tile_p, int occ_
 // 0.54 ms with 4x4).
 static int default_tile_grad(int storage) { return storage == QP3D_STORAGE_BF16 ? 1 : 6; }
 
-static int build_work(qp3d_plan *p)
+void qp3d_plan_items(const qp3d_plan *p, std::vector<int4> (&items)[4])
 {
   int occ_g = 1, occ_i = 1, occ_p = 1, occ_py = 1;
   with_storage(p->cfg.storage, [&](auto sp) {
     using S = decltype(sp);
-    occ_g = occupancy_of(p->tile_g, [](auto t) { return reinterpret_cast<const void *>(&k_gradmax<S, decltype(t)>); });
-    const bool spec = p->spec;
-    occ_i = occupancy_of(p->tile_u, [spec](auto t) {
-      return spec ? reinterpret_cast<const void *>(&k_update<S, decltype(t), false, true>)
-                : reinterpret_cast<const void *>(&k_update<S, decltype(t), false>);
-    });
-    occ_p = occupancy_of(p->tile_p, [spec](auto t) {
-      return spec ? reinterpret_cast<const void *>(&k_update<S, decltype(t), true, true>)
-                : reinterpret_cast<const void *>(&k_update<S, decltype(t), true>);
-    });
-    occ_py = occupancy_of(p->tile_py, [spec](auto t) {
-      return spec ? reinterpret_cast<const void *>(&k_update<S, decltype(t), true, true>)
-                : reinterpret_cast<const void *>(&k_update<S, decltype(t), true>);
-    });
+    occ_g = occupancy_gradmax<S>(p->tile_g);
+    occ_i = occupancy_update<S, false>(p->tile_u, p->spec);
+    occ_p = occupancy_update<S, true>(p->tile_p, p->spec);
+    occ_py = occupancy_update<S, true>(p->tile_py, p->spec);
   });
-  int rc = qp3d_build_uniform_list(p, p->tile_g, occ_g, p->wl_grad);
-  if (rc) return rc;
+  const int ncu = qp3d_cu_count(p);
+  for (std::vector<int4> &v : items) v.clear();
+  qp3d_items_uniform(p, p->tile_g, occ_g * ncu, items[0]);
   // The PML update is two launches: x and z faces, then the y faces (folded tiles fill half a
   // wavefront per face and run best with a narrower tile; measured at 576^3).
-  std::vector<int4> it, ip, ipy;
-  const int ncu = qp3d_cu_count(p);
-  qp3d_items_update(p, p->tile_u, p->tile_p, occ_i * ncu, occ_p * ncu, it, ip, &ipy, p->tile_py, occ_py * ncu);
-  if ((rc = upload_list(p->wl_int, it))) return rc;
-  if ((rc = upload_list(p->wl_pml, ip))) return rc;
-  if ((rc = upload_list(p->wl_pmly, ipy))) return rc;
+  qp3d_items_update(p, p->tile_u, p->tile_p, occ_i * ncu, occ_p * ncu, items[1], items[2], &items[3], p->tile_py,
+                    occ_py * ncu);
+}
+
+static int build_work(qp3d_plan *p)
+{
+  std::vector<int4> items[4];
+  qp3d_plan_items(p, items);
+  int rc;
+  if ((rc = upload_list(p->wl_grad, items[0]))) return rc;
+  if ((rc = upload_list(p->wl_int, items[1]))) return rc;
+  if ((rc = upload_list(p->wl_pml, items[2]))) return rc;
+  if ((rc = upload_list(p->wl_pmly, items[3]))) return rc;
   p->n_int_cells = p->wl_int.cells;
   p->n_pml_cells = p->wl_pml.cells + p->wl_pmly.cells;
   return QP3D_OK;
@@ -325,62 +272,55 @@ static int build_work(qp3d_plan *p)
 // ---------------------------------------------------------------------------------------
 // lifecycle
 
-extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
+// Validates a configuration (no device access); `who` prefixes the error message.
+int qp3d_check_config(const qp3d_config &c, const char *who)
 {
-  if (!cfg || !out) {
-    qp3d_set_error("plan_create: null argument");
-    return QP3D_ERR_ARG;
-  }
-  *out = nullptr;
-  const qp3d_config &c = *cfg;
   if (c.pml < 6 || c.nx < 2 * c.pml + 1 || c.ny < 2 * c.pml + 1 || c.nz < 2 * c.pml + 1 || c.nt < 0) {
-    qp3d_set_error("plan_create: need pml >= 6 and nx, ny, nz > 2*pml (got nx=%d ny=%d nz=%d pml=%d nt=%d)", c.nx,
+    qp3d_set_error("%s: need pml >= 6 and nx, ny, nz > 2*pml (got nx=%d ny=%d nz=%d pml=%d nt=%d)", who, c.nx,
                    c.ny, c.nz, c.pml, c.nt);
     return QP3D_ERR_ARG;
   }
   if (!(c.dx > 0 && c.dy > 0 && c.dz > 0 && c.dt > 0)) {
-    qp3d_set_error("plan_create: spacings and dt must be positive");
+    qp3d_set_error("%s: spacings and dt must be positive", who);
     return QP3D_ERR_ARG;
   }
   if (c.sz < 0 || c.sz >= c.nz) {
-    qp3d_set_error("plan_create: receiver plane sz=%d outside [0,%d)", c.sz, c.nz);
+    qp3d_set_error("%s: receiver plane sz=%d outside [0,%d)", who, c.sz, c.nz);
     return QP3D_ERR_ARG;
   }
   if (c.snapflag == 1 && c.ntsnap <= 0) {
-    qp3d_set_error("plan_create: ntsnap must be positive when snapflag == 1");
+    qp3d_set_error("%s: ntsnap must be positive when snapflag == 1", who);
     return QP3D_ERR_ARG;
   }
   if (c.snap_s != 0 && (c.snap_s != 1 || c.snapflag != 1)) {
-    qp3d_set_error("plan_create: snap_s = 1 needs snapflag == 1 (got snap_s=%d snapflag=%d)", c.snap_s, c.snapflag);
+    qp3d_set_error("%s: snap_s = 1 needs snapflag == 1 (got snap_s=%d snapflag=%d)", who, c.snap_s, c.snapflag);
     return QP3D_ERR_ARG;
   }
   if (c.tile_update < 0 || c.tile_update > NTILES || c.tile_grad < 0 || c.tile_grad > NTILES || c.tile_pml < 0 ||
       c.tile_pml > NTILES || c.tile_pml_y < 0 || c.tile_pml_y > NTILES || c.streams < 0 || c.streams > 2) {
-    qp3d_set_error("plan_create: tile shapes must be in [0,%d] and streams in [0,2]", NTILES);
+    qp3d_set_error("%s: tile shapes must be in [0,%d] and streams in [0,2]", who, NTILES);
     return QP3D_ERR_ARG;
   }
   if (c.gmax_mode < 0 || c.gmax_mode > 2) {
-    qp3d_set_error("plan_create: gmax_mode %d outside [0,2]", c.gmax_mode);
+    qp3d_set_error("%s: gmax_mode %d outside [0,2]", who, c.gmax_mode);
     return QP3D_ERR_ARG;
   }
   if (c.storage != QP3D_STORAGE_FP32 && c.storage != QP3D_STORAGE_BF16 && c.storage != QP3D_STORAGE_FP16) {
-    qp3d_set_error("plan_create: unknown storage mode %d", c.storage);
+    qp3d_set_error("%s: unknown storage mode %d", who, c.storage);
     return QP3D_ERR_ARG;
   }
   const double cells = (double)c.nx * c.ny * c.nz;
   if (cells >= 1073741824.0 || c.nx > 32767 || c.ny > 32767) {
-    qp3d_set_error("plan_create: grid of %.4g cells exceeds the 2^30-cell limit of one plan", cells);
+    qp3d_set_error("%s: grid of %.4g cells exceeds the 2^30-cell limit of one plan", who, cells);
     return QP3D_ERR_ARG;
   }
-  int ndev = 0;
-  HIP_TRY(hipGetDeviceCount(&ndev));
-  if (c.device < 0 || c.device >= ndev) {
-    qp3d_set_error("plan_create: device %d not present (%d devices)", c.device, ndev);
-    return QP3D_ERR_ARG;
-  }
-  HIP_TRY(hipSetDevice(c.device));
+  return QP3D_OK;
+}
 
-  qp3d_plan *p = new qp3d_plan();
+// The host-side description of a plan for a valid configuration: geometry, tile shapes and
+// modes (allocates nothing).
+void qp3d_plan_describe(qp3d_plan *p, const qp3d_config &c)
+{
   p->cfg = c;
   p->hx = c.nx - 2 * c.pml;
   p->hy = c.ny - 2 * c.pml;
@@ -397,6 +337,29 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
   p->spec = c.gmax_mode == 2;
   // a captured loop runs on one stream, and so does a step whose fix-up follows all updates
   p->two_streams = c.streams == 2 && !c.use_graph && !p->spec;
+}
+
+
+extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
+{
+  if (!cfg || !out) {
+    qp3d_set_error("plan_create: null argument");
+    return QP3D_ERR_ARG;
+  }
+  *out = nullptr;
+  const qp3d_config &c = *cfg;
+  int rc0 = qp3d_check_config(c, "plan_create");
+  if (rc0) return rc0;
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (c.device < 0 || c.device >= ndev) {
+    qp3d_set_error("plan_create: device %d not present (%d devices)", c.device, ndev);
+    return QP3D_ERR_ARG;
+  }
+  HIP_TRY(hipSetDevice(c.device));
+
+  qp3d_plan *p = new qp3d_plan();
+  qp3d_plan_describe(p, c);
 
   auto fail = [&](int rc) {
     qp3d_plan_destroy(p);
@@ -571,6 +534,65 @@ static int to_host(const float *src, size_t n, std::vector<float> &v)
   return QP3D_OK;
 }
 
+// PML coefficient table per face family (x: dlr, y: dfb, z: dtb) and layer j, and the FD
+// coefficients of the plan (p->coef), from profiles and tables in host or device memory:
+//   K1 = (1+d dt)^2 - 3, K2 = 2 d dt - 1, K3 = 1 - dt d, E = d'   (GPU_kernel.cu:568-576)
+int qp3d_make_tables(qp3d_plan *p, const float *dlr, const float *ddlr, const float *dtb, const float *ddtb,
+                     const float *dfb, const float *ddfb, const float *c, const float *c2, std::vector<float4> &tab)
+{
+  const qp3d_config &cf = p->cfg;
+  std::vector<float> hdlr, hddlr, hdtb, hddtb, hdfb, hddfb, hc, hc2;
+  int rc;
+  if ((rc = to_host(dlr, (size_t)cf.pml, hdlr)) || (rc = to_host(ddlr, (size_t)cf.pml, hddlr)) ||
+      (rc = to_host(dtb, (size_t)cf.pml, hdtb)) || (rc = to_host(ddtb, (size_t)cf.pml, hddtb)) ||
+      (rc = to_host(dfb, (size_t)cf.pml, hdfb)) || (rc = to_host(ddfb, (size_t)cf.pml, hddfb)) ||
+      (rc = to_host(c, 36, hc)) || (rc = to_host(c2, 36, hc2)))
+    return rc;
+  tab.resize(3 * (size_t)cf.pml);
+  const float *dd[3] = {hdlr.data(), hdfb.data(), hdtb.data()}, *de[3] = {hddlr.data(), hddfb.data(), hddtb.data()};
+  for (int f = 0; f < 3; f++)
+    for (int j = 0; j < cf.pml; j++) {
+      const float d = dd[f][j], dt = cf.dt;
+      float4 t;
+      t.x = (float)(std::pow((double)(d * dt + 1), 2) - 3);
+      t.y = 2 * d * dt - 1;
+      t.z = 1 - dt * d;
+      t.w = de[f][j];
+      tab[(size_t)f * cf.pml + j] = t;
+    }
+  // FD coefficients: the 10th-order rows of the 6x6 tables (CPU_function.c:123-127,143-147)
+  Coef &k = p->coef;
+  for (int m = 1; m <= H; m++) {
+    k.c1[m - 1] = hc[30 + m];
+    k.c2[m - 1] = hc2[30 + m];
+  }
+  k.ix = 1.0f / cf.dx; k.iy = 1.0f / cf.dy; k.iz = 1.0f / cf.dz;
+  k.ixx = (float)(1.0 / ((double)cf.dx * cf.dx));
+  k.iyy = (float)(1.0 / ((double)cf.dy * cf.dy));
+  k.izz = (float)(1.0 / ((double)cf.dz * cf.dz));
+  k.s2 = 2.0f * (k.c2[0] + k.c2[1] + k.c2[2] + k.c2[3] + k.c2[4]);
+  for (int m = 0; m < H; m++) {
+    k.g1x[m] = k.c1[m] * k.ix;
+    k.g1y[m] = k.c1[m] * k.iy;
+    k.g1z[m] = k.c1[m] * k.iz;
+  }
+  k.dt = cf.dt;
+  return QP3D_OK;
+}
+
+void qp3d_launch_model(float *v2, const float *eps, float *ed, float dt, long n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_model, dim3((unsigned)std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, s, v2, eps, ed, dt,
+                     n);
+}
+
+void qp3d_launch_source_box(const float *src, int nx, int ny, int nz, int *box, hipStream_t s)
+{
+  const long n = (long)nx * ny * nz;
+  hipLaunchKernelGGL(k_source_box, dim3((unsigned)std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, s, src, nx, ny,
+                     nz, box);
+}
+
 extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps, const float *del,
                                 const float *source, const float *wavelet, const float *dlr, const float *ddlr,
                                 const float *dtb, const float *ddtb, const float *dfb, const float *ddfb,
@@ -591,51 +613,15 @@ extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps,
   HIP_TRY(hipMemcpyAsync(p->vdt2, vp, sizeof(float) * (size_t)N, hipMemcpyDefault, p->stream));
   HIP_TRY(hipMemcpyAsync(p->eps, eps, sizeof(float) * (size_t)N, hipMemcpyDefault, p->stream));
   HIP_TRY(hipMemcpyAsync(p->epsd, del, sizeof(float) * (size_t)N, hipMemcpyDefault, p->stream));
-  hipLaunchKernelGGL(k_model, dim3((unsigned)std::min<long>((N + 255) / 256, 8192)), dim3(256), 0, p->stream, p->vdt2,
-                     (const float *)p->eps, p->epsd, cf.dt, N);
+  qp3d_launch_model(p->vdt2, p->eps, p->epsd, cf.dt, N, p->stream);
   HIP_TRY(hipGetLastError());
-  std::vector<float> hw, hdlr, hddlr, hdtb, hddtb, hdfb, hddfb, hc, hc2;
-  int rc;
-  if ((rc = to_host(wavelet, (size_t)cf.nt, hw)) || (rc = to_host(dlr, (size_t)cf.pml, hdlr)) ||
-      (rc = to_host(ddlr, (size_t)cf.pml, hddlr)) || (rc = to_host(dtb, (size_t)cf.pml, hdtb)) ||
-      (rc = to_host(ddtb, (size_t)cf.pml, hddtb)) || (rc = to_host(dfb, (size_t)cf.pml, hdfb)) ||
-      (rc = to_host(ddfb, (size_t)cf.pml, hddfb)) || (rc = to_host(c, 36, hc)) || (rc = to_host(c2, 36, hc2)))
-    return rc;
-  wavelet = hw.data();
-  dlr = hdlr.data(); ddlr = hddlr.data(); dtb = hdtb.data(); ddtb = hddtb.data(); dfb = hdfb.data(); ddfb = hddfb.data();
-  c = hc.data(); c2 = hc2.data();
-  // PML coefficient table per face family (x: dlr, y: dfb, z: dtb) and layer j:
-  //   K1 = (1+d dt)^2 - 3, K2 = 2 d dt - 1, K3 = 1 - dt d, E = d'   (GPU_kernel.cu:568-576)
-  std::vector<float4> tab(3 * (size_t)cf.pml);
-  const float *dd[3] = {dlr, dfb, dtb}, *de[3] = {ddlr, ddfb, ddtb};
-  for (int f = 0; f < 3; f++)
-    for (int j = 0; j < cf.pml; j++) {
-      const float d = dd[f][j], dt = cf.dt;
-      float4 t;
-      t.x = (float)(std::pow((double)(d * dt + 1), 2) - 3);
-      t.y = 2 * d * dt - 1;
-      t.z = 1 - dt * d;
-      t.w = de[f][j];
-      tab[(size_t)f * cf.pml + j] = t;
-    }
+  std::vector<float4> tab;
+  int rc = qp3d_make_tables(p, dlr, ddlr, dtb, ddtb, dfb, ddfb, c, c2, tab);
+  if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(p->pmltab, tab.data(), sizeof(float4) * tab.size(), hipMemcpyHostToDevice, p->stream));
-  // FD coefficients: the 10th-order rows of the 6x6 tables (CPU_function.c:123-127,143-147)
-  Coef &k = p->coef;
-  for (int m = 1; m <= H; m++) {
-    k.c1[m - 1] = c[30 + m];
-    k.c2[m - 1] = c2[30 + m];
-  }
-  k.ix = 1.0f / cf.dx; k.iy = 1.0f / cf.dy; k.iz = 1.0f / cf.dz;
-  k.ixx = (float)(1.0 / ((double)cf.dx * cf.dx));
-  k.iyy = (float)(1.0 / ((double)cf.dy * cf.dy));
-  k.izz = (float)(1.0 / ((double)cf.dz * cf.dz));
-  k.s2 = 2.0f * (k.c2[0] + k.c2[1] + k.c2[2] + k.c2[3] + k.c2[4]);
-  for (int m = 0; m < H; m++) {
-    k.g1x[m] = k.c1[m] * k.ix;
-    k.g1y[m] = k.c1[m] * k.iy;
-    k.g1z[m] = k.c1[m] * k.iz;
-  }
-  k.dt = cf.dt;
+  std::vector<float> hw;
+  if ((rc = to_host(wavelet, (size_t)cf.nt, hw))) return rc;
+  wavelet = hw.data();
   // source: bounding box of the non-zero cells of the full-grid spatial source
   int z0 = cf.nz, z1 = -1, x0 = cf.nx, x1 = -1, y0 = cf.ny, y1 = -1;
   const bool src_dev = on_device(source);
@@ -644,8 +630,7 @@ extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps,
     HIP_TRY(hipMalloc((void **)&dbox, sizeof box));
     hipError_t e = hipMemcpyAsync(dbox, box, sizeof box, hipMemcpyHostToDevice, p->stream);
     if (e == hipSuccess) {
-      hipLaunchKernelGGL(k_source_box, dim3((unsigned)std::min<long>((N + 255) / 256, 8192)), dim3(256), 0, p->stream,
-                         source, cf.nx, cf.ny, cf.nz, dbox);
+      qp3d_launch_source_box(source, cf.nx, cf.ny, cf.nz, dbox, p->stream);
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
@@ -790,7 +775,7 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on
   UpdArgs<S> ua;
   make_args<S>(p, k & 1, st, ga, ua);
   if (!p->spec) {
-    launch_gradmax(p->tile_g, p->wl_grad, s0, ga);
+    launch_gradmax<S>(p->tile_g, p->wl_grad.n, s0, ga);
     HIP_TRY(hipGetLastError());
   }
   if (e) HIP_TRY(hipEventRecord(e[1], s0));
@@ -801,26 +786,26 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on
 
   if (p->wl_int.n) {
     ua.work = p->wl_int.d;
-    launch_update<S, false>(p->tile_u, p->wl_int, s0, ua, p->spec);
+    launch_update<S, false>(p->tile_u, p->wl_int.n, s0, ua, p->spec);
     HIP_TRY(hipGetLastError());
   }
   if (e) HIP_TRY(hipEventRecord(e[2], s0));
   hipStream_t ps = two ? p->stream2 : s0;
   if (p->wl_pml.n) {
     ua.work = p->wl_pml.d;
-    launch_update<S, true>(p->tile_p, p->wl_pml, ps, ua, p->spec);
+    launch_update<S, true>(p->tile_p, p->wl_pml.n, ps, ua, p->spec);
     HIP_TRY(hipGetLastError());
   }
   if (p->wl_pmly.n) {
     ua.work = p->wl_pmly.d;
-    launch_update<S, true>(p->tile_py, p->wl_pmly, ps, ua, p->spec);
+    launch_update<S, true>(p->tile_py, p->wl_pmly.n, ps, ua, p->spec);
     HIP_TRY(hipGetLastError());
   }
   if (e && all) HIP_TRY(hipEventRecord(e[3], ps));
   if (p->spec) {
     // after every update of the step (G_k complete): the marking pass (its workgroups exit at
     // once when the prediction held), then the fix-up of the marked cells
-    launch_mark(p->tile_g, p->wl_grad, s0, ga);
+    launch_mark<S>(p->tile_g, p->wl_grad.n, s0, ga);
     HIP_TRY(hipGetLastError());
     const int blocks = (int)std::min<long>((p->fixwords / 4 + 256) / 256, 8192);
     hipLaunchKernelGGL(k_spec_fix<S>, dim3(blocks), dim3(256), 0, s0, ua, p->fixwords, cf.nt);

@@ -66,6 +66,7 @@ struct qp3d_plan {
   unsigned *gmax = nullptr;  // [nt]
   int *flag = nullptr;
   bool spec = false;             // gmax_mode 2 (a batch turns it off for its plans)
+  int in_batch = 0;              // 1 while a live batch drives the plan
   unsigned *fixmask = nullptr;   // gmax_mode 2: one bit per cell, all clear between steps
   float *gpred = nullptr;        // gmax_mode 2: [nt] predicted G per step
   long fixwords = 0;
@@ -185,3 +186,22 @@ int qp3d_cu_count(const qp3d_plan *p);
 int qp3d_forward_step(qp3d_plan *p, int k);
 // The same on stream `on` (one stream), with the step index the kernels read given by `st`.
 int qp3d_forward_step_on(qp3d_plan *p, int k, Step st, hipStream_t on);
+// Validates a configuration (no device access); `who` prefixes the error message.
+int qp3d_check_config(const qp3d_config &c, const char *who);
+// Host-side description of a plan for a valid configuration (geometry, tiles, modes); no
+// allocation.
+void qp3d_plan_describe(qp3d_plan *p, const qp3d_config &c);
+// Work items of the plan's four launches (gradient max, interior, PML x/z faces, PML y faces)
+// in logical order, for the resident-workgroup capacity of the plan's device.
+void qp3d_plan_items(const qp3d_plan *p, std::vector<int4> (&items)[4]);
+// Items in XCD-aware launch order (padded to a multiple of 8); returns the cells they cover.
+long qp3d_order_items(const std::vector<int4> &logical, bool batch, std::vector<int4> &ordered);
+// PML coefficient table (3 x pml float4) and p->coef from profiles and FD tables in host or
+// device memory.
+int qp3d_make_tables(qp3d_plan *p, const float *dlr, const float *ddlr, const float *dtb, const float *ddtb,
+                     const float *dfb, const float *ddfb, const float *c, const float *c2, std::vector<float4> &tab);
+// In place over (vp, eps, del) copies: v2 = (vp dt)^2, ed = eps - del.
+void qp3d_launch_model(float *v2, const float *eps, float *ed, float dt, long n, hipStream_t s);
+// Bounding box {z0, z1, x0, x1, y0, y1} of the non-zero cells of a device source array into
+// device memory `box`, which must hold {nz, -1, nx, -1, ny, -1} beforehand.
+void qp3d_launch_source_box(const float *src, int nx, int ny, int nz, int *box, hipStream_t s);

@@ -97,12 +97,19 @@ int qp3d_modeling3d(int nx, int ny, int nz, int nt, int ntsnap, float dx, float 
   const char *gm = getenv("QP3D_GMAX_MODE");
   cfg.gmax_mode = gm ? atoi(gm) : 0;
   qp3d_plan *p = NULL;
+  int nonfinite = 0;
   int rc = qp3d_plan_create(&cfg, &p);
   if (rc) return rc;
   if ((rc = qp3d_plan_upload(p, vp, epsilon, delta, source, wavelet, dlr, ddlr, dtb, ddtb, dfb, ddfb, c, c2)) ||
-      (rc = qp3d_plan_reset(p)) || (rc = qp3d_plan_run(p, 0, nt)) || (rc = qp3d_plan_sync(p)) ||
-      (rc = qp3d_plan_record(p, record)))
+      (rc = qp3d_plan_reset(p)) || (rc = qp3d_plan_run(p, 0, nt)))
     goto out;
+  /* A non-finite wavefield (CFL violation) still returns its record and snapshots, as the
+     reference's would hold the inf/NaN values; the status says what happened. */
+  if ((rc = qp3d_plan_sync(p)) == QP3D_ERR_NONFINITE) {
+    nonfinite = 1;
+    rc = QP3D_OK;
+  }
+  if (rc || (rc = qp3d_plan_record(p, record))) goto out;
   if (cfg.snapflag) {
     int hx, hy, hz;
     qp3d_plan_dims(p, &hx, &hy, &hz);
@@ -133,6 +140,10 @@ int qp3d_modeling3d(int nx, int ny, int nz, int nt, int ntsnap, float dx, float 
   }
 out:
   qp3d_plan_destroy(p);
+  if (rc == QP3D_OK && nonfinite) {
+    qp3d_set_error("wavefield became non-finite (inf/NaN): dt violates the stability limit for this model");
+    rc = QP3D_ERR_NONFINITE;
+  }
   return rc;
 }
 

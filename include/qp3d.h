@@ -40,7 +40,11 @@ void qp3d_set_error(const char *fmt, ...);
  *   snapflag, ntsnap, snap_file  when snapflag == 1 and k % ntsnap == 0, writes
  *                                <snap_file>k.dat (interior cube) and <snap_file>_{xz,yz,xy}k.dat.
  * sx, sy are unused (the source position is carried by `source`), as in the reference.
- * Runs on HIP device 0, synchronously; failures are reported by qp3d_last_error(). */
+ * Runs on HIP device 0, synchronously; failures are reported by qp3d_last_error(). A wavefield
+ * that becomes non-finite (CFL violation) still leaves its record (and snapshot files) written,
+ * holding the inf/NaN values, as the reference's would.
+ * The library also exports this entry point with C++ linkage (include/qp3d_cxx.h), the symbol a
+ * driver compiled with g++ against the reference's header binds. */
 void modeling3d(int nx, int ny, int nz, int nt, int ntsnap, float dx, float dy, float dz, float dt, int pml,
                 int snapflag, int sx, int sy, int sz, float *vp, float *epsilon, float *delta, float *source,
                 float *wavelet, float *record, float *dlr, float *ddlr, float *dtb, float *ddtb, float *dfb,

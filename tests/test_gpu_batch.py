@@ -101,3 +101,24 @@ def test_batch_runs_predicted_threshold_plans_with_the_gradient_pass():
     finally:
         for p in plans1 + plans2:
             p.close()
+
+
+def test_batch_rejects_duplicate_and_shared_plans():
+    """A plan holds one shot's wavefields, record and maxima: it cannot appear twice in a batch
+    or belong to two live batches; once its batch is destroyed it can join another."""
+    setups, plans = survey(nt=12)
+    try:
+        with pytest.raises(qp3d.Qp3dError) as ei:
+            qp3d.Batch([plans[0], plans[1], plans[0]])
+        assert ei.value.code == -1
+        with qp3d.Batch(plans[:2]) as b:
+            with pytest.raises(qp3d.Qp3dError) as ei:
+                qp3d.Batch(plans[1:3])
+            assert ei.value.code == -6
+            b.forward()
+        with qp3d.Batch(plans[1:3]) as b2:
+            recs = b2.forward()
+        assert np.array_equal(recs[0], alone([plans[1]])[0][0])
+    finally:
+        for p in plans:
+            p.close()

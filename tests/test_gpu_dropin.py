@@ -155,3 +155,83 @@ def test_qp3d_main_writes_s_field_snapshots(tmp_path, monkeypatch):
             a = np.fromfile(g / "output" / n, np.float32)
             b = np.fromfile(c / "output" / n, np.float32)
             assert a.size == b.size and (rel_l2(a, b) <= TOL if np.any(b) else not np.any(a)), n
+
+
+REF_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref")
+REF_MAIN_GPU = os.path.join(REF_DIR, "ref_main_qp3d")
+REF_MAIN_CPU = os.path.join(REF_DIR, "ref_main_oracle")
+
+
+@pytest.mark.skipif(not (os.path.exists(REF_MAIN_GPU) and os.path.exists(REF_MAIN_CPU)),
+                    reason="the reference's driver is built (make -C oracle ref) only where its checkout is")
+def test_reference_driver_linked_unchanged_against_the_library(tmp_path):
+    """The reference's own Main.c (Main.c:24-246), compiled unchanged with g++ and linked against
+    libqp3d.so through the C++-linkage modeling3d, against the same objects linked against the
+    FP32 oracle: the link-time swap the reference makes between its CPU and GPU bodies
+    (Makefile:8, compile_GPU.sh:11). Every output file: same names and sizes, values within
+    relative L2 1e-5. This driver also runs the reference's direct-wave call with its
+    all-zero model (Main.c:163-175), where the oracle and the GPU agree too."""
+    p = dict(nt=70, ntsnap=30, Nx0=34, Ny0=30, Nz0=22, sx0=15, sy0=14, nxs=2, nys=1, dxs=6, dys=6, hx=19, hy=17,
+             pml=9)
+    model = qp3d.genmodel(p["Nx0"], p["Ny0"], p["Nz0"], 11)
+    dirs = {}
+    for tag, exe in (("gpu", REF_MAIN_GPU), ("cpu", REF_MAIN_CPU)):
+        d = tmp_path / tag
+        for sub in ("input", "model", "output"):
+            os.makedirs(d / sub)
+        for name, a in zip(("vp", "epsilon", "delta"), model):
+            a.tofile(d / "model" / f"{name}.dat")
+        (d / "input" / "parameter.txt").write_text(PAR.format(**p))
+        print(f"running the reference driver linked against {'libqp3d.so' if tag == 'gpu' else 'the oracle'}",
+              flush=True)
+        r = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=180)
+        print(r.stdout[-300:], r.stderr[-300:], flush=True)
+        assert r.returncode == 0, tag
+        dirs[tag] = d
+    names = sorted(os.listdir(dirs["cpu"] / "output"))
+    assert sorted(os.listdir(dirs["gpu"] / "output")) == names
+    assert {"Record_direct.dat", "Record_fullwave.dat", "Record_mute.dat", "snapshot_fullwave30.dat"} <= set(names)
+    for n in names:
+        a = np.fromfile(dirs["gpu"] / "output" / n, np.float32)
+        b = np.fromfile(dirs["cpu"] / "output" / n, np.float32)
+        assert a.size == b.size and a.size > 0, n
+        if np.any(b):
+            assert rel_l2(a, b) <= TOL, n
+        else:
+            assert not np.any(a), n
+
+
+def test_void_entry_points_through_ctypes():
+    """The legacy void modeling3d, with C linkage and with C++ linkage (the mangled symbol a
+    g++-built driver binds), called through ctypes against the oracle."""
+    import ctypes as C
+    from qp3d._lib import fptr
+    s = qp3d.config_setup(hx=23, hy=21, nz0=19, pml=9, nt=70, seed=9, src_depth=2)
+    orec = O.modeling3d(s, "f32")
+    lib = C.CDLL(qp3d.LIB_PATH)
+    F = C.POINTER(C.c_float)
+    sig = [C.c_int] * 5 + [C.c_float] * 4 + [C.c_int] * 5 + [F] * 14 + [C.c_char_p]
+    for sym in ("modeling3d", "_Z10modeling3diiiiiffffiiiiiPfS_S_S_S_S_S_S_S_S_S_S_S_S_PKc"):
+        fn = getattr(lib, sym)
+        fn.restype = None
+        fn.argtypes = sig
+        rec = np.zeros((s.nt, s.hx, s.hy), np.float32)
+        arrs = [np.ascontiguousarray(a, np.float32).reshape(-1) for a in
+                (s.vp, s.eps, s.dele, s.source, s.wavelet, rec, s.dlr, s.ddlr, s.dtb, s.ddtb, s.dfb, s.ddfb, s.c,
+                 s.c2)]
+        sx, sy, _ = s.src_cell
+        print(f"void entry point {sym[:16]}", flush=True)
+        fn(s.nx, s.ny, s.nz, s.nt, 1, s.dx, s.dy, s.dz, s.dt, s.pml, 0, sx, sy, s.sz, *[fptr(a) for a in arrs], None)
+        got = arrs[5].reshape(s.nt, s.hx, s.hy)
+        assert rel_l2(got, orec) <= TOL, sym
+
+
+def test_dropin_returns_the_record_of_a_blown_up_run():
+    """A CFL violation: the status-returning entry point reports QP3D_ERR_NONFINITE and still
+    writes the record, which holds the non-finite values as the reference's would."""
+    s = qp3d.config_setup(hx=21, hy=21, nz0=21, pml=8, nt=400, seed=0, dt=0.004)
+    rec = np.zeros((s.nt, s.hx, s.hy), np.float32)
+    with pytest.raises(qp3d.Qp3dError) as ei:
+        qp3d.modeling3d(s, record=rec)
+    assert ei.value.code == -5
+    assert not np.all(np.isfinite(rec)) and np.any(rec[:50] != 0)

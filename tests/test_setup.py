@@ -192,3 +192,63 @@ def test_plan_create_rejects_bad_config_without_touching_the_device():
     for cfg in bad:
         assert lib.qp3d_plan_create(C.byref(cfg), C.byref(h)) == QP3D_ERR_ARG
         assert lib.qp3d_last_error().decode()
+
+
+# The reference declares modeling3d without extern "C" and compiles its driver with g++
+# (3dPureqPModeling_single_shot.h:4-6, Makefile:3,11-13): a driver built that way refers to the
+# C++-mangled symbol, which the library exports beside the C one.
+CXX_CALLER = r"""
+void modeling3d(int nx, int ny, int nz, int nt, int ntsnap, float dx, float dy, float dz, float dt, int pml,
+                int snapflag, int sx, int sy, int sz, float *vp, float *epsilon, float *delta, float *source,
+                float *wavelet, float *record, float *dlr, float *ddlr, float *dtb, float *ddtb, float *dfb,
+                float *ddfb, float *c, float *c2, const char *snap_file);
+int main(int argc, char **argv)
+{
+  if (argc > 99) modeling3d(0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0);
+  return 0;
+}
+"""
+
+
+def _cxx_header_declaration():
+    text = open(os.path.join(REPO, "include", "qp3d_cxx.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return re.search(r"void modeling3d\(.*?\);", text, flags=re.S).group(0)
+
+
+def test_cxx_linkage_modeling3d_links_from_a_gpp_driver(tmp_path):
+    """A g++-compiled caller of the reference's declaration links against libqp3d.so with no
+    header edit; the declaration in include/qp3d_cxx.h is that same one, and the C++ entry point
+    is exported next to the C one."""
+    import shutil
+    import subprocess
+    lib_dir = os.path.dirname(qp3d.LIB_PATH)
+    src = tmp_path / "caller.cpp"
+    src.write_text(CXX_CALLER)
+    gxx = shutil.which("g++")
+    assert gxx
+    r = subprocess.run([gxx, "-o", str(tmp_path / "caller"), str(src), "-L" + lib_dir, "-lqp3d",
+                        "-Wl,-rpath," + lib_dir], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    decl = " ".join(_cxx_header_declaration().split())
+    want = " ".join(re.search(r"void modeling3d\(.*?\);", CXX_CALLER, flags=re.S).group(0).split())
+    assert decl == want
+    nm = subprocess.run(["nm", "-D", "--defined-only", qp3d.LIB_PATH], capture_output=True, text=True).stdout
+    assert re.search(r" T modeling3d$", nm, flags=re.M)
+    assert re.search(r" T _Z10modeling3diiiiiffffiiiiiPfS_S_S_S_S_S_S_S_S_S_S_S_S_PKc$", nm, flags=re.M)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REPO, "oracle", "_ref", "Main.o")),
+                    reason="the reference's driver object is built only where the reference checkout is")
+def test_reference_driver_object_links_against_the_library(tmp_path):
+    """The reference's Main.c, compiled unchanged with g++ (oracle/Makefile), links against
+    libqp3d.so in place of modeling3D.o: its only unresolved solver symbol is the C++ one."""
+    import subprocess
+    ref = os.path.join(REPO, "oracle", "_ref")
+    objs = [os.path.join(ref, f) for f in ("Main.o", "array_new.o", "read_write.o", "CPU_function.o")]
+    und = subprocess.run(["nm", "-u", objs[0]], capture_output=True, text=True).stdout
+    assert "_Z10modeling3diiiiiffffiiiiiPfS_S_S_S_S_S_S_S_S_S_S_S_S_PKc" in und
+    lib_dir = os.path.dirname(qp3d.LIB_PATH)
+    r = subprocess.run(["g++", "-o", str(tmp_path / "ref_main"), *objs, "-L" + lib_dir, "-lqp3d",
+                        "-Wl,-rpath," + lib_dir, "-lm"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
