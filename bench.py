@@ -4,14 +4,20 @@ Workload (the 512^3 single-GPU configuration): 512x512x512 interior, 32-cell PML
 faces (padded 576^3), dx=dy=dz=12.5 m, dt=1 ms, 3000 time steps, 25 Hz Ricker at the surface
 centre, the full 512x512 surface receiver plane, interior snapshots every 500 steps; model =
 four VTI layers with a smooth +-3% random perturbation (seed 1). A bench "step" is one
-complete shot (reset + 3000 time steps) with the model already resident in HBM.
+complete shot (reset + 3000 time steps) with the model already resident in HBM; the timed
+shots replay the captured 10-step hipGraph (--graph 0: direct launches).
 
 metric: Gcell-updates/s = updated cells (nx-10)(ny-10)(nz-10) x time steps / wall time.
 roofline: the dominant launch, the interior update kernel: its algorithmic bytes per launch
 (24 B per interior cell: u_cur, u_prev, three model fields, u_next) over its mean launch time,
-from HIP events on the plan's stream.
+from HIP events on the plan's stream around every interior-update launch of one instrumented
+shot run right after the timed shots (replayed graphs carry no per-launch events).
+roofline_step: the whole time step: algorithmic bytes of all its launches (28 B per interior
+cell, 68 B per PML cell) over the timed shots' wall time per time step.
 cpu_baseline: the CPU oracle (the reference's arithmetic, bit-exact with its CPU kernels)
-on the host's CPU share, on a 20-step window that starts from the GPU's state at step 300.
+on the host's CPU share, on a 20-step window that starts from the GPU's state at step 300;
+config.cpu_baseline_single_thread: the same on one thread (the reference's CPU path is
+single-threaded) over 2 steps.
 
     python bench.py [--steps K] [--warmup W] [--cpu-steps S] [--no-cpu]
 
@@ -59,37 +65,32 @@ def calib_copy(nbytes):
     hip.hipFree(b)
 
 
-def cpu_baseline(setup, plan, start, nsteps):
-    """Time the oracle for nsteps starting from the GPU's wavefield after `start` steps."""
-    import numpy as np
+def cpu_baseline(setup, plan, start, runs):
+    """Time the oracle for each (threads, nsteps) of `runs`, starting from the GPU's wavefield
+    after `start` steps (threads 0 = the host's CPU share)."""
     from oracle import oracle as O
 
     plan.reset()
     plan.run(0, start)
     plan.sync()
     fields = [plan.field(w) for w in range(9)]
-    threads = O.default_threads()
-    o = O.OracleState(setup, "f32", nthreads=threads)
-    o.import_state(*fields)
+    out = []
+    for threads, nsteps in runs:
+        threads = threads or O.default_threads()
+        o = O.OracleState(setup, "f32", nthreads=threads)
+        o.import_state(*fields)
+        t0 = time.perf_counter()
+        for k in range(start, start + nsteps):
+            o.step(setup.wavelet[k])
+        dt = time.perf_counter() - t0
+        o.close()
+        out.append({"value": round(setup.n_upd * nsteps / dt / 1e9, 4), "unit": "Gcell-updates/s", "cores": threads,
+                    "kind": "port",
+                    "sample": f"CPU oracle (FP32, reference arithmetic) on the {setup.nx}^3 grid, {nsteps} time "
+                              f"steps {start}..{start + nsteps - 1} from the GPU state at step {start}, "
+                              f"{threads} thread(s); {dt:.1f} s"})
     del fields
-    t0 = time.perf_counter()
-    for k in range(start, start + nsteps):
-        o.step(setup.wavelet[k])
-    dt = time.perf_counter() - t0
-    o.close()
-    return {"value": round(setup.n_upd * nsteps / dt / 1e9, 4), "unit": "Gcell-updates/s", "cores": threads,
-            "kind": "port",
-            "sample": f"CPU oracle (FP32, reference arithmetic) on the 576^3 grid, {nsteps} time steps "
-                      f"{start}..{start + nsteps - 1} from the GPU state at step {start}; {dt:.1f} s"}
-
-
-def smoothed_setup(qp3d, hx, hy, nz0, pml, nt, seed, sigma, fm=25.0, src_depth=0):
-    """Setup of the layered-plus-random model rule smoothed with a Gaussian of `sigma` cells."""
-    from scipy.ndimage import gaussian_filter
-    vp, ep, de = qp3d.genmodel(hx, hy, nz0, seed)
-    sm = [gaussian_filter(a.astype(np.float64), sigma, mode="nearest").astype(np.float32) for a in (vp, ep, de)]
-    g = [qp3d.file_to_grid(a) for a in sm]
-    return qp3d.build_setup(*g, pml, nt, 12.5, 12.5, 12.5, 0.001, fm, src_depth=src_depth)
+    return out
 
 
 def bench_fwi(args):
@@ -103,7 +104,7 @@ def bench_fwi(args):
     nt = 2000 if args.nt == 3000 else args.nt
     t = time.perf_counter()
     s_true = qp3d.config_setup(hx=n, hy=n, nz0=n, pml=args.pml, nt=nt, seed=2)
-    s0 = smoothed_setup(qp3d, n, n, n, args.pml, nt, seed=2, sigma=15.0)
+    s0 = qp3d.smoothed_config_setup(n, n, n, args.pml, nt, seed=2, sigma=15.0)
     log(f"fwi setup {s0.nx}^3, nt {nt}, {time.perf_counter() - t:.1f} s")
     with qp3d.Plan(s_true, gmax_mode=args.gmax_mode) as p:
         obs = p.forward()
@@ -151,45 +152,46 @@ def bench_fwi(args):
     print(json.dumps(out), flush=True)
 
 
-def fd_study(qp3d, ndirs, n=64, nt=400, pml=20, rel=1e-5):
+def fd_study(qp3d, ndirs, n=64, nt=400, pml=20, rel=1e-4):
     """Directional derivatives of the GPU adjoint against central differences of FP64 oracle
-    misfits on the 64^3 grid: random smooth perturbations of one of vp, eps, del in turn, at
-    the smoothed seed-2 model and at the same model made elliptic (del = eps, where S = 1 on
-    both sides of the scalar operator's threshold and the misfit is smooth)."""
+    misfits on the 64^3 grid (pml 20, 400 steps, seed-2 observed data, start model smoothed
+    sigma = 15): random smooth directions cycling through vp, eps and del; the perturbed oracle
+    runs replay the base run's threshold decisions (the linearisation the adjoint transposes;
+    with free decisions the misfit jumps wherever a cell crosses the 0.01*G threshold)."""
     import dataclasses
     from scipy.ndimage import gaussian_filter
     from oracle import oracle as O
     s_true = qp3d.config_setup(hx=n, hy=n, nz0=n, pml=pml, nt=nt, seed=2)
     with qp3d.Plan(s_true) as p:
         obs = p.forward().astype(np.float64)
-    base_s = smoothed_setup(qp3d, n, n, n, pml, nt, seed=2, sigma=15.0)
-    out = {"grid": [base_s.nx, base_s.ny, base_s.nz], "nt": nt, "step": rel}
-    rng = np.random.default_rng(7)
+    s0 = qp3d.smoothed_config_setup(n, n, n, pml, nt, seed=2, sigma=15.0)
+    o = O.OracleState(s0, "f64")
+    bits = O.decision_bits(s0)
+    o.set_decisions(bits, 1)
+    o.run_record()
+    o.close()
+    with qp3d.Plan(s0) as p, qp3d.Fwi(p, interval=50) as f:
+        rec = f.forward().astype(np.float64)
+        g = f.gradient((rec - obs).astype(np.float32))
+    out = {"grid": [s0.nx, s0.ny, s0.nz], "nt": nt, "step": rel, "decisions": "frozen at the base run"}
     t0 = time.perf_counter()
-    for model in ("vti", "elliptic"):
-        s0 = base_s if model == "vti" else dataclasses.replace(base_s, dele=base_s.eps.copy())
-        with qp3d.Plan(s0) as p, qp3d.Fwi(p, interval=50) as f:
-            rec = f.forward().astype(np.float64)
-            g = f.gradient((rec - obs).astype(np.float32))
-        nd = ndirs if model == "vti" else max(3, ndirs // 4)
-        errs = {"vp": [], "eps": [], "dele": []}
-        for i in range(nd):
-            name = ("vp", "eps", "dele")[i % 3]
-            scale = float(np.abs(s0.vp).max()) if name == "vp" else 1.0
-            d = gaussian_filter(rng.standard_normal((s0.nz, s0.nx, s0.ny)), 2.0)
-            d = d / np.abs(d).max()
-            h = rel * scale
-            base = getattr(s0, name).astype(np.float64)
-            mp, mm = (base + h * d).astype(np.float32), (base - h * d).astype(np.float32)
-            jp = O.modeling3d(dataclasses.replace(s0, **{name: mp}), "f64").astype(np.float64)
-            jm = O.modeling3d(dataclasses.replace(s0, **{name: mm}), "f64").astype(np.float64)
-            dj = 0.5 * np.sum((jp - obs) ** 2) - 0.5 * np.sum((jm - obs) ** 2)
-            ad = float(np.sum(g[("vp", "eps", "dele").index(name)] * (mp.astype(np.float64) - mm.astype(np.float64))))
-            errs[name].append(abs(ad - dj) / abs(dj))
-            log(f"fd {model} {name} {i}: adjoint {ad:.8e} fd {dj:.8e} rel {errs[name][-1]:.2e} "
-                f"({time.perf_counter() - t0:.0f} s)")
-        out[model] = {k: {"n": len(v), "max_rel_err": float(max(v)), "median_rel_err": float(np.median(v))}
-                      for k, v in errs.items() if v}
+    names = ("vp", "eps", "dele")
+    errs = {k: [] for k in names}
+    for i in range(ndirs):
+        name = names[i % 3]
+        d = gaussian_filter(np.random.default_rng(100 + i).standard_normal((s0.nz, s0.nx, s0.ny)), 2.0)
+        d = d / np.abs(d).max()
+        h = rel * (float(np.abs(s0.vp).max()) if name == "vp" else 1.0)
+        base = getattr(s0, name).astype(np.float64)
+        mp, mm = (base + h * d).astype(np.float32), (base - h * d).astype(np.float32)
+        jp = 0.5 * np.sum((O.record_frozen(dataclasses.replace(s0, **{name: mp}), bits) - obs) ** 2)
+        jm = 0.5 * np.sum((O.record_frozen(dataclasses.replace(s0, **{name: mm}), bits) - obs) ** 2)
+        dj = jp - jm
+        ad = float(np.sum(g[names.index(name)] * (mp.astype(np.float64) - mm.astype(np.float64))))
+        errs[name].append(abs(ad - dj) / abs(dj))
+        log(f"fd {name} {i}: adjoint {ad:.9e} frozen fd {dj:.9e} rel {errs[name][-1]:.2e} "
+            f"({time.perf_counter() - t0:.0f} s)")
+    out["rel_err"] = {k: {"n": len(v), "max": float(max(v)), "median": float(np.median(v))} for k, v in errs.items() if v}
     return out
 
 
@@ -272,16 +274,10 @@ def bench_batch(args):
     give the baseline."""
     import qp3d
     nt = 2500 if args.nt == 3000 else args.nt
-    rng = np.random.default_rng(3)
     t = time.perf_counter()
     model = qp3d.genmodel(800, 800, args.depth, 3)
-    setups = []
-    for i in range(args.shots):
-        wx, wy = (int(v) for v in rng.integers(args.win[0], args.win[1] + 1, size=2))
-        sx, sy = (int(v) for v in rng.integers(0, 800, size=2))
-        x0 = min(max(sx - wx // 2, 0), 800 - wx)
-        y0 = min(max(sy - wy // 2, 0), 800 - wy)
-        setups.append(qp3d.window_setup(model, x0, y0, wx, wy, args.pml, nt, 20.0))
+    setups = [qp3d.window_setup(model, x0, y0, wx, wy, args.pml, nt, 20.0)
+              for x0, y0, wx, wy in qp3d.survey_windows(800, 800, args.shots, args.win[0], args.win[1], 3)]
     del model
     log(f"batch setup: {args.shots} shots in {time.perf_counter() - t:.1f} s")
     plans = [qp3d.Plan(s, storage=args.storage, use_graph=True) for s in setups]
@@ -301,13 +297,15 @@ def bench_batch(args):
         wall = time.perf_counter() - t0
         kb = b.step_bytes()
     log(f"batched: {wall:.2f} s")
-    t0 = time.perf_counter()
-    for p in plans:
-        p.reset()
-        p.run()
-        p.sync()
-    seq = time.perf_counter() - t0
-    log(f"one after another: {seq:.2f} s")
+    seq = None
+    if args.sequential:
+        t0 = time.perf_counter()
+        for p in plans:
+            p.reset()
+            p.run()
+            p.sync()
+        seq = time.perf_counter() - t0
+        log(f"one after another: {seq:.2f} s")
     for p in plans:
         p.close()
     step_bytes = kb["gradmax"] + kb["update"]
@@ -321,8 +319,8 @@ def bench_batch(args):
                       "cells_updated_per_step": kb["cells_updated"], "autotuned_tiles": tune["tiles"],
                       "autotuned_per_shot_launches": tune["per_shot"],
                       "autotune_ms_per_step": [round(x, 4) for x in tune["ms"]],
-                      "batched_s": round(wall, 3), "sequential_s": round(seq, 3),
-                      "speedup_vs_sequential": round(seq / wall, 3),
+                      "batched_s": round(wall, 3), "sequential_s": round(seq, 3) if seq else None,
+                      "speedup_vs_sequential": round(seq / wall, 3) if seq else None,
                       "hbm_gbs_algorithmic": round(step_bytes * nt / wall / 1e9, 1)}}
     print(json.dumps(out), flush=True)
 
@@ -337,8 +335,9 @@ def main():
     ap.add_argument("--pml", type=int, default=32)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--cpu-start", type=int, default=300)
+    ap.add_argument("--cpu-steps-1", type=int, default=2, help="time steps of the single-thread CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--graph", type=int, default=0)
+    ap.add_argument("--graph", type=int, default=1, help="1: timed shots replay the captured graph; 0: direct")
     ap.add_argument("--tile-update", type=int, default=0)
     ap.add_argument("--tile-grad", type=int, default=0)
     ap.add_argument("--tile-pml", type=int, default=0)
@@ -353,6 +352,7 @@ def main():
     ap.add_argument("--shots", type=int, default=32, help="batch: number of shots")
     ap.add_argument("--win", type=int, nargs=2, default=[160, 320], help="batch: window size range in x and y")
     ap.add_argument("--depth", type=int, default=300, help="batch: model depth (cells)")
+    ap.add_argument("--sequential", type=int, default=1, help="batch: also run the shots one after another")
     ap.add_argument("--workload", default="forward", choices=["forward", "fwi", "precision", "batch", "dropin"],
                     help="forward: the 512^3 shot (default); fwi: the 256^3 adjoint gradient; "
                          "precision: FP32/BF16/FP16 storage against the FP64 oracle on a 256^3 crop; "
@@ -390,10 +390,10 @@ def main():
         plan.forward_async()
         plan.sync()
         log(f"warmup shot {i}: {time.perf_counter() - t:.3f} s")
-    # The roofline kernel's time comes from HIP events around every interior-update launch of
-    # the timed shots (two events per step). The other launches' times come from one extra shot
-    # with events around every launch, after the timed region. Replayed graphs carry no events,
-    # so with --graph the interior time comes from that extra shot too.
+    # Timed shots: graph replays (no per-launch events); with --graph 0, direct launches with
+    # events around the interior update only (two per step). After the timed region one shot
+    # with events around the interior update only gives the roofline kernel's mean launch time,
+    # and one shot with events around every launch the other launches' times.
     plan.set_instrument(0 if args.graph else 2)
     plan.sync()
     t0 = time.perf_counter()
@@ -403,12 +403,16 @@ def main():
     t1 = time.perf_counter()
     wall = t1 - t0
     kt_timed = plan.kernel_times()
+    if args.graph:
+        plan.set_instrument(2)
+        plan.forward_async()
+        plan.sync()
+        kt_timed = plan.kernel_times()
     plan.set_instrument(1)
     plan.forward_async()
     plan.sync()
     kt = plan.kernel_times()
-    if not args.graph:
-        kt["update_interior"] = kt_timed["update_interior"]
+    kt["update_interior"] = kt_timed["update_interior"]
     kb = plan.step_bytes()
     plan.set_instrument(0)
     from qp3d.threshold import steps_in_band
@@ -422,19 +426,25 @@ def main():
     value = cells / wall / 1e9
     achieved = gbs["update_interior"]
     step_bytes = kb["gradmax"] + kb["update_interior"] + kb["update_pml"]
-    traffic = None
+    ms_time_step = wall / args.steps / s.nt * 1e3
+    step_gbs = step_bytes / (ms_time_step * 1e-3) / 1e9
+    traffic = step_traffic = None
     if os.path.exists(TRAFFIC_FILE):
         try:
             with open(TRAFFIC_FILE) as f:
                 tr = json.load(f)
             if tr.get("nx") == s.nx and tr.get("storage", "fp32") == args.storage:
                 traffic = tr.get("bytes_per_launch")
+                step_traffic = tr.get("bytes_per_step") or sum(k["bytes"] for k in tr.get("per_kernel", {}).values())
         except (OSError, ValueError):
-            traffic = None
-    cpu = None
+            traffic = step_traffic = None
+    cpu = cpu1 = None
     if not args.no_cpu and args.cpu_steps > 0:
-        cpu = cpu_baseline(s, plan, min(args.cpu_start, s.nt - args.cpu_steps), args.cpu_steps)
-        log(f"cpu baseline {cpu}")
+        runs = [(0, args.cpu_steps)] + ([(1, args.cpu_steps_1)] if args.cpu_steps_1 > 0 else [])
+        res = cpu_baseline(s, plan, min(args.cpu_start, s.nt - args.cpu_steps), runs)
+        cpu = res[0]
+        cpu1 = res[1] if len(res) > 1 else None
+        log(f"cpu baseline {res}")
     plan.close()
 
     out = {
@@ -449,22 +459,29 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp32" if args.storage == "fp32" else f"fp32 arithmetic, {args.storage} storage",
         "data": "synthetic (layered VTI + smooth random perturbation, seed 1)",
         "config": {"workload": f"3D pure-qP VTI forward shot, {args.n}^3 interior, pml {args.pml} "
                                f"({s.nx}^3 padded), {s.nt} steps, dt 1 ms, dx 12.5 m, 25 Hz Ricker, "
                                f"{args.n}x{args.n} receivers, snapshots every 500 steps",
+                   "step_unit": f"one shot: reset + {s.nt} time steps (ms_per_step is per shot)",
+                   "ms_per_shot": round(wall / args.steps * 1e3, 3),
+                   "ms_per_time_step": round(ms_time_step, 5),
+                   "time_loop": "replayed 10-step hipGraph" if args.graph else "direct launches",
                    "padded": [s.nx, s.ny, s.nz], "nt": s.nt, "n_upd": s.n_upd, "n_int": s.n_int,
                    "gcells_int_per_s": round(s.n_int * s.nt * args.steps / wall / 1e9, 3),
-                   "ms_per_time_step": round(wall / args.steps / s.nt * 1e3, 5),
                    "ms_gradmax": round(kt["gradmax"], 5), "ms_update_interior": round(kt["update_interior"], 5),
                    "ms_update_pml": round(kt["update_pml"], 5), "ms_update_stage": round(kt["update_stage"], 5),
                    "gbs_gradmax": round(gbs.get("gradmax", 0), 1), "gbs_update_pml": round(gbs.get("update_pml", 0), 1),
                    "storage": args.storage, "gmax_mode": gmode, "steps_in_predicted_band": in_band,
-                   "roofline_kernel": f"k_update<interior> ({kb['update_interior'] / kb['cells_interior']:.0f} B per interior cell)",
-                   "hbm_frac_step": round(step_bytes * s.nt * args.steps / wall / 1e9 / HBM_PEAK_GBS, 4)},
+                   "roofline_kernel": f"k_update<interior> ({kb['update_interior'] / kb['cells_interior']:.0f} B per "
+                                      f"interior cell, {kb['cells_interior']:.0f} cells per launch)",
+                   "cpu_baseline_single_thread": cpu1},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "roofline_step": {"bound": "hbm", "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(step_gbs / HBM_PEAK_GBS, 4), "traffic": step_traffic,
+                          "bytes": step_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -8,7 +8,8 @@ qp3d.autograd (imported separately, it needs torch) holds the torch.autograd shi
 """
 from ._lib import EXPORTED, LIB_PATH, NTILES, Qp3dError, load
 from .plan import Batch, Fwi, Plan, modeling3d
-from .setup import Setup, build_setup, config_setup, fd_tables, file_to_grid, genmodel, window_setup
+from .setup import (Setup, build_setup, config_setup, fd_tables, file_to_grid, genmodel, smoothed_config_setup,
+                    survey_windows, window_setup)
 
 __all__ = ["EXPORTED", "LIB_PATH", "NTILES", "Qp3dError", "load", "Batch", "Fwi", "Plan", "modeling3d", "Setup", "build_setup",
-           "config_setup", "fd_tables", "file_to_grid", "genmodel", "window_setup"]
+           "config_setup", "fd_tables", "file_to_grid", "genmodel", "smoothed_config_setup", "survey_windows", "window_setup"]

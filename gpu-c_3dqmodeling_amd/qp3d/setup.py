@@ -132,3 +132,27 @@ def window_setup(model, x0, y0, hx, hy, pml, nt, fm, h=12.5, dt=0.001, src_depth
     if any(w.shape[:2] != (hy, hx) for w in win):
         raise ValueError("window outside the model")
     return build_setup(*[file_to_grid(w) for w in win], pml, nt, h, h, h, dt, fm, src_depth=src_depth)
+
+
+def survey_windows(nx0, ny0, nshots, wmin, wmax, seed):
+    """Shot windows of a survey over an nx0 x ny0 model surface: shot positions uniform on the
+    surface grid, each window wmin..wmax cells in x and y (independently), centred on its shot
+    and moved inside the model where it would cross an edge. Returns [(x0, y0, wx, wy)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nshots):
+        wx, wy = (int(v) for v in rng.integers(wmin, wmax + 1, size=2))
+        sx, sy = int(rng.integers(0, nx0)), int(rng.integers(0, ny0))
+        x0 = min(max(sx - wx // 2, 0), nx0 - wx)
+        y0 = min(max(sy - wy // 2, 0), ny0 - wy)
+        out.append((x0, y0, wx, wy))
+    return out
+
+
+def smoothed_config_setup(hx, hy, nz0, pml, nt, seed, sigma, fm=25.0, src_depth=0, h=12.5, dt=0.001):
+    """The shot of config_setup with its model smoothed by a Gaussian of `sigma` cells (the
+    starting model of an inversion built from the same model rule)."""
+    from scipy.ndimage import gaussian_filter
+    vp, ep, de = genmodel(hx, hy, nz0, seed)
+    sm = [gaussian_filter(a.astype(np.float64), sigma, mode="nearest").astype(np.float32) for a in (vp, ep, de)]
+    return build_setup(*[file_to_grid(a) for a in sm], pml, nt, h, h, h, dt, fm, src_depth=src_depth)

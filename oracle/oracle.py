@@ -61,6 +61,12 @@ def _load_oracle():
             f = getattr(lib, f"oracle_last_G_{sfx}")
             f.restype = C.c_double
             f.argtypes = [C.c_void_p]
+            f = getattr(lib, f"oracle_set_decisions_{sfx}")
+            f.restype = None
+            f.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+            f = getattr(lib, f"oracle_run_record_{sfx}")
+            f.restype = None
+            f.argtypes = [C.c_void_p, _F, C.c_int, C.c_int, RP]
             f = getattr(lib, f"oracle_destroy_{sfx}")
             f.restype = None
             f.argtypes = [C.c_void_p]
@@ -218,6 +224,25 @@ class OracleState(_StateBase):
     def G(self):
         return getattr(self.lib, f"oracle_last_G_{self.sfx}")(self.h)
 
+    def set_decisions(self, bits, mode):
+        """Test hook: mode 1 records each following step's per-cell decision |grad u| > 0.01*G
+        into `bits` (uint8, nt x ceil(N/8)), mode 2 replays them in place of the comparison (the
+        linearisation point of the adjoint: threshold and decisions held), 0 turns it off."""
+        self._bits = bits
+        ptr = None if bits is None else bits.ctypes.data_as(C.c_void_p)
+        getattr(self.lib, f"oracle_set_decisions_{self.sfx}")(self.h, ptr, int(mode))
+
+    def run_record(self, nt=None, wavelet=None):
+        """nt steps (the setup's wavelet by default); returns the record [nt][hx][hy]."""
+        s = self.s
+        nt = s.nt if nt is None else nt
+        w = np.ascontiguousarray(s.wavelet if wavelet is None else wavelet, np.float32)
+        rec = np.zeros((nt, s.hx, s.hy), self.dtype)
+        R = C.c_float if self.sfx == "f32" else C.c_double
+        getattr(self.lib, f"oracle_run_record_{self.sfx}")(self.h, _fp(w), int(nt), int(s.sz),
+                                                           rec.ctypes.data_as(C.POINTER(R)))
+        return rec
+
     def close(self):
         if self.h:
             getattr(self.lib, f"oracle_destroy_{self.sfx}")(self.h)
@@ -260,6 +285,22 @@ class RefState(_StateBase):
 
     def __del__(self):
         self.close()
+
+
+def decision_bits(s, nt=None):
+    """A zeroed buffer for set_decisions: nt steps of one bit per cell."""
+    n = s.nx * s.ny * s.nz
+    return np.zeros(((s.nt if nt is None else nt), (n + 7) // 8), np.uint8)
+
+
+def record_frozen(s, bits, precision="f64", nthreads=None):
+    """Record of a run whose per-cell threshold decisions are replayed from `bits`."""
+    o = OracleState(s, precision, nthreads=nthreads)
+    try:
+        o.set_decisions(bits, 2)
+        return o.run_record()
+    finally:
+        o.close()
 
 
 def modeling3d(s, precision="f32", nthreads=None, snapflag=0, ntsnap=1, snap_file=None, nt=None):

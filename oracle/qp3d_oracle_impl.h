@@ -32,6 +32,12 @@ typedef struct ST {
   long aux_n[6];
   REAL G;
   int nthreads;
+  /* frozen decisions (test hook): dmode 1 records, per step, one bit per cell for
+     |grad u| > 0.01*G into dbits; dmode 2 takes each cell's decision from dbits instead of
+     comparing (the threshold G and every cell's side of it held at a base run's) */
+  unsigned char *dbits;
+  int dmode;
+  long kstep;
 } ST;
 
 /* CPU_function.c:10-22 absMaxval: max of fabs over all cells; '>=' compare, so NaN never wins. */
@@ -74,7 +80,8 @@ void SFX(oracle_grad)(const REAL *u2, REAL *ux, REAL *uy, REAL *uz, const float 
  * S = 0.5*(1+sqrt(1 - 8(eps-del)(nx^2+ny^2)nz^2 / (1+2eps(nx^2+ny^2))^2)).
  * The numerator is REAL arithmetic, the denominator and the outer sqrt are double (pow). */
 void SFX(oracle_scalar_operator)(REAL G, const REAL *ux, const REAL *uy, const REAL *uz, const float *eps,
-                                 const float *del, REAL *S, int nx, int ny, int nz, int nthreads)
+                                 const float *del, REAL *S, int nx, int ny, int nz, int nthreads, unsigned char *dbits,
+                                 int dmode)
 {
   const long pln = (long)nx * ny;
 #pragma omp parallel for num_threads(nthreads) schedule(static)
@@ -86,7 +93,13 @@ void SFX(oracle_scalar_operator)(REAL G, const REAL *ux, const REAL *uy, const R
         if (y >= 5 && y < ny - 5 && x >= 5 && x < nx - 5 && z >= 5 && z < nz - 5) {
           REAL mode = SQRTR(ux[i] * ux[i] + uy[i] * uy[i] + uz[i] * uz[i]);
           REAL nxv, nyv, nzv;
-          if ((double)mode > 1.0e-02 * (double)G) {
+          int above = (double)mode > 1.0e-02 * (double)G;
+          if (dmode == 1 && above) {
+#pragma omp atomic
+            dbits[i >> 3] |= (unsigned char)(1u << (i & 7));
+          }
+          if (dmode == 2) above = (dbits[i >> 3] >> (i & 7)) & 1;
+          if (above) {
             nxv = (REAL)((double)ux[i] / ((double)mode + 2.22e-17));
             nyv = (REAL)((double)uy[i] / ((double)mode + 2.22e-17));
             nzv = (REAL)((double)uz[i] / ((double)mode + 2.22e-17));
@@ -290,10 +303,37 @@ static void SFX(oracle_step_impl)(ST *s, float w, double Gthr)
   G = G > gz ? G : gz;
   s->G = G;
   if (Gthr >= 0) G = (REAL)Gthr;
-  SFX(oracle_scalar_operator)(G, s->ux, s->uy, s->uz, s->eps, s->del, s->S, s->nx, s->ny, s->nz, s->nthreads);
+  unsigned char *bits = s->dmode ? s->dbits + s->kstep * ((n + 7) / 8) : NULL;
+  if (s->dmode == 1) memset(bits, 0, (size_t)((n + 7) / 8));
+  SFX(oracle_scalar_operator)(G, s->ux, s->uy, s->uz, s->eps, s->del, s->S, s->nx, s->ny, s->nz, s->nthreads, bits,
+                              s->dmode);
   SFX(oracle_wavefield_update)(s);
   SFX(oracle_addsource)(s->src, w, s->u3, n, s->nthreads);
   SFX(oracle_exchange)(s);
+  s->kstep++;
+}
+
+/* Frozen decisions (test hook; see ST): mode 1 records the decisions of the following steps
+ * into bits (ceil(N/8) bytes per step, for as many steps as will be run), mode 2 replays them,
+ * mode 0 turns the hook off. The step count restarts at 0. */
+void SFX(oracle_set_decisions)(ST *s, unsigned char *bits, int mode)
+{
+  s->dbits = bits;
+  s->dmode = bits ? mode : 0;
+  s->kstep = 0;
+}
+
+/* nt steps with the given wavelet samples; record[k] = u(sz) interior plane after step k. */
+void SFX(oracle_run_record)(ST *s, const float *wavelet, int nt, int sz, REAL *record)
+{
+  const int hx = s->nx - 2 * s->pml, hy = s->ny - 2 * s->pml;
+  const long pln = (long)s->nx * s->ny;
+  for (int k = 0; k < nt; k++) {
+    SFX(oracle_step)(s, wavelet[k]);
+    for (int x = 0; x < hx; x++)
+      for (int y = 0; y < hy; y++)
+        record[(long)k * hx * hy + (long)x * hy + y] = s->u2[sz * pln + (long)(x + s->pml) * s->ny + (y + s->pml)];
+  }
 }
 
 /* Field access for tests: 0..6 = u1 u2 u3 ux uy uz S. */
