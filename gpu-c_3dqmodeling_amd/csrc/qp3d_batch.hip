@@ -23,6 +23,10 @@ struct qp3d_batch {
   // each shot's own launches (its plan's work lists and tiles) one shot after another within a
   // step, instead of one launch per stage for all shots; the autotuner sets it when faster
   bool per_shot = false;
+  // work items of the shared launches: each shot cut as its own plan cuts it, for the whole
+  // machine (false), or for its share of the machine (true: fewer, longer items); the
+  // autotuner picks
+  bool share = false;
   // the batch kernels take G from the gradient pass: plans with gmax_mode 2 run with it off
   // while they belong to the batch, and get it back when the batch is destroyed
   std::vector<char> spec_saved;
@@ -49,7 +53,7 @@ static int build_stage(qp3d_batch *b, int stage)
                        : occupancy_update_batch<S, true>(b->tile[stage]);
   });
   const int n = (int)b->plans.size();
-  const int cap = std::max(1, occ * qp3d_cu_count(b->plans[0]) / n);  // the shots share the machine
+  const int cap = std::max(1, occ * qp3d_cu_count(b->plans[0]) / (b->share ? n : 1));
   std::vector<int4> all;
   for (int i = 0; i < n; i++) {
     std::vector<int4> it, ip, ipy;
@@ -362,12 +366,18 @@ extern "C" int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, dou
     if (ms_out) ms_out[s] = best;
     if (tiles_out) tiles_out[s] = bt + 1;
   }
-  // whole steps, direct launches: batched stages against each shot's own launches
+  // whole steps, direct launches: batched stages with items cut for the whole machine per
+  // shot, batched stages with items cut for each shot's share, each shot's own launches
   const bool graph = b->use_graph;
   b->use_graph = false;
-  double t[2] = {0, 0};
-  for (int mode = 0; mode < 2 && rc == QP3D_OK; mode++) {
-    b->per_shot = mode == 1;
+  double t[3] = {0, 0, 0};
+  for (int mode = 0; mode < 3 && rc == QP3D_OK; mode++) {
+    b->per_shot = mode == 2;
+    if (mode < 2 && b->share != (mode == 1)) {
+      b->share = mode == 1;
+      for (int s = 0; s < 4 && rc == QP3D_OK; s++) rc = build_stage(b, s);
+      if (rc) break;
+    }
     if ((rc = qp3d_batch_reset(b)) || (rc = qp3d_batch_run(b, 0, 2))) break;
     float ms = 0;
     if (hipEventRecord(b->ev[0], b->stream) != hipSuccess) rc = QP3D_ERR_HIP;
@@ -383,7 +393,13 @@ extern "C" int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, dou
     if (rc == QP3D_ERR_HIP) qp3d_set_error("batch_autotune: timing the step modes failed");
     return rc;
   }
-  b->per_shot = t[1] < t[0];
+  const int best = t[2] < std::min(t[0], t[1]) ? 2 : (t[1] < t[0] ? 1 : 0);
+  b->per_shot = best == 2;
+  if (best < 2 && b->share != (best == 1)) {
+    b->share = best == 1;
+    for (int s = 0; s < 4; s++)
+      if ((rc = build_stage(b, s))) return rc;
+  }
   // a graph captured before holds the other mode's launches
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);

@@ -223,9 +223,10 @@ int qp3d_batch_destroy(qp3d_batch *b);
 int qp3d_batch_reset(qp3d_batch *b);
 int qp3d_batch_run(qp3d_batch *b, int k0, int k1);
 int qp3d_batch_sync(qp3d_batch *b);
-/* Picks each stage's tile shape by timing every candidate over `steps` steps, then whether a
- * step goes out as one launch per stage for all shots or as each shot's own launches one after
- * another (faster when every shot fills the GPU by itself), then resets;
+/* Picks each stage's tile shape by timing every candidate over `steps` steps, then how a step
+ * goes out: one launch per stage for all shots with each shot's work items cut for the whole
+ * machine (as its own plan cuts them) or for its share of the machine, or each shot's own
+ * launches one after another; then resets;
  * tiles_out[4] / ms_out[4] (may be NULL): gradient max, interior, PML x/z faces, PML y faces. */
 int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, double *ms_out);
 /* 1 when the batch runs each shot's own launches (chosen by qp3d_batch_autotune), 0 when it
