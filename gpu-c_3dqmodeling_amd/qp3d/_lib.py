@@ -49,6 +49,8 @@ _SIGS = [
     ("qp3d_set_error", None, [C.c_char_p]),
     ("modeling3d", None, [C.c_int] * 5 + [C.c_float] * 4 + [C.c_int] * 5 + [_F] * 14 + [C.c_char_p]),
     ("qp3d_modeling3d", C.c_int, [C.c_int] * 5 + [C.c_float] * 4 + [C.c_int] * 5 + [_F] * 14 + [C.c_char_p]),
+    ("qp3d_modeling3d_noalloc", C.c_int,
+     [C.c_int] * 5 + [C.c_float] * 4 + [C.c_int] * 5 + [_F] * 14 + [C.c_char_p, C.c_void_p]),
     ("qp3d_fdcoff1", None, [_F]),
     ("qp3d_fdcoff2", None, [_F]),
     ("qp3d_pmlcoff", None, [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float] + [_F] * 6),

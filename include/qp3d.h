@@ -58,6 +58,28 @@ int qp3d_modeling3d(int nx, int ny, int nz, int nt, int ntsnap, float dx, float 
                     const float *dlr, const float *ddlr, const float *dtb, const float *ddtb, const float *dfb,
                     const float *ddfb, const float *c, const float *c2, const char *snap_file);
 
+/* Allocation-free variant of qp3d_modeling3d. No counterpart in the reference; it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments and meaning as qp3d_modeling3d, except:
+ *   - every array (model, source, wavelet, tables and the record) is device memory of the
+ *     device current on the calling thread;
+ *   - snapflag must be 0 (snapshots would need memory this entry point does not allocate);
+ *   - stream (a hipStream_t, NULL = the null stream) orders the run; the call returns after the
+ *     nt steps have finished (QP3D_ERR_NONFINITE as qp3d_modeling3d).
+ * It calls no hipMalloc/hipFree: the wavefields, PML state, converted model, PML table,
+ * per-step maxima, flag, source box and work lists live in one fixed device array of
+ * QP3D_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until the process ends
+ * (about 44 bytes per grid cell: a 141^3 grid fits); larger grids are rejected with
+ * QP3D_ERR_ARG. Every call on a device shares that device's array: calls that overlap on one
+ * GPU (other threads, other streams) race and must be serialised by the caller. */
+#define QP3D_NOALLOC_SCRATCH_BYTES (160ull << 20)
+int qp3d_modeling3d_noalloc(int nx, int ny, int nz, int nt, int ntsnap, float dx, float dy, float dz, float dt,
+                            int pml, int snapflag, int sx, int sy, int sz, const float *vp, const float *epsilon,
+                            const float *delta, const float *source, const float *wavelet, float *record,
+                            const float *dlr, const float *ddlr, const float *dtb, const float *ddtb,
+                            const float *dfb, const float *ddfb, const float *c, const float *c2,
+                            const char *snap_file, void *stream);
+
 /* ---- host setup math (bit-identical to the reference's tables) ---- */
 void qp3d_fdcoff1(float *c);  /* CPU_function.c:111-128 fdcoff1, c[36] */
 void qp3d_fdcoff2(float *c);  /* CPU_function.c:131-148 fdcoff2, c[36] */
