@@ -206,21 +206,22 @@ __global__ __launch_bounds__(T::NT) void k_adj_prod(AdjArgs a)
       const int x = x0 + L.row(r);
       const float c0 = q[r][H];
       float Lxx = 0.f, Lyy = 0.f, Lzz = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
+      const float c02 = 2.f * c0;
 #pragma unroll
       for (int k = 1; k <= H; k++) {
         const float ax = s[X + k][Y], bx = s[X - k][Y];
         const float ay = s[X][Y + k], by = s[X][Y - k];
         const float az = q[r][H + k], bz = q[r][H - k];
-        Lxx = fmaf(a.c.c2[k - 1], ax + bx, Lxx);
-        Lyy = fmaf(a.c.c2[k - 1], ay + by, Lyy);
-        Lzz = fmaf(a.c.c2[k - 1], az + bz, Lzz);
+        Lxx = d2tap(a.c.c2[k - 1], ax, bx, c02, Lxx);
+        Lyy = d2tap(a.c.c2[k - 1], ay, by, c02, Lyy);
+        Lzz = d2tap(a.c.c2[k - 1], az, bz, c02, Lzz);
         ux = fmaf(a.c.c1[k - 1], ax - bx, ux);
         uy = fmaf(a.c.c1[k - 1], ay - by, uy);
         uz = fmaf(a.c.c1[k - 1], az - bz, uz);
       }
-      const float Uxx = fmaf(-a.c.s2, c0, Lxx) * a.c.ixx;
-      const float Uyy = fmaf(-a.c.s2, c0, Lyy) * a.c.iyy;
-      const float Uzz = fmaf(-a.c.s2, c0, Lzz) * a.c.izz;
+      const float Uxx = Lxx * a.c.ixx;
+      const float Uyy = Lyy * a.c.iyy;
+      const float Uzz = Lzz * a.c.izz;
       ux *= a.c.ix; uy *= a.c.iy; uz *= a.c.iz;
       const unsigned o = L.off[r];
       const float e = C.e[r], ed = C.ed[r], v2 = C.v2[r];
@@ -411,18 +412,19 @@ __global__ __launch_bounds__(T::NT) void k_adj_apply(ApplyArgs a)
       if (!rows_on[r]) continue;
       const int X = H + L.row(r), Y = L.Y;
       float lxx = 0.f, lyy = 0.f, lzz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
+      const float px2 = 2.f * sxx[X][Y], py2 = 2.f * syy[X][Y], pz2 = 2.f * qzz[r][(H + it) % QN];
 #pragma unroll
       for (int k = 1; k <= H; k++) {
-        lxx = fmaf(a.c.c2[k - 1], sxx[X + k][Y] + sxx[X - k][Y], lxx);
-        lyy = fmaf(a.c.c2[k - 1], syy[X][Y + k] + syy[X][Y - k], lyy);
-        lzz = fmaf(a.c.c2[k - 1], qzz[r][(H + k + it) % QN] + qzz[r][(H - k + it) % QN], lzz);
+        lxx = d2tap(a.c.c2[k - 1], sxx[X + k][Y], sxx[X - k][Y], px2, lxx);
+        lyy = d2tap(a.c.c2[k - 1], syy[X][Y + k], syy[X][Y - k], py2, lyy);
+        lzz = d2tap(a.c.c2[k - 1], qzz[r][(H + k + it) % QN], qzz[r][(H - k + it) % QN], pz2, lzz);
         dx = fmaf(a.c.c1[k - 1], sqx[X - k][Y] - sqx[X + k][Y], dx);
         dy = fmaf(a.c.c1[k - 1], sqy[X][Y - k] - sqy[X][Y + k], dy);
         dz = fmaf(a.c.c1[k - 1], qqz[r][(H - k + it) % QN] - qqz[r][(H + k + it) % QN], dz);
       }
-      float acc = fmaf(-a.c.s2, sxx[X][Y], lxx) * a.c.ixx;
-      acc = fmaf(fmaf(-a.c.s2, syy[X][Y], lyy), a.c.iyy, acc);
-      acc = fmaf(fmaf(-a.c.s2, qzz[r][(H + it) % QN], lzz), a.c.izz, acc);
+      float acc = lxx * a.c.ixx;
+      acc = fmaf(lyy, a.c.iyy, acc);
+      acc = fmaf(lzz, a.c.izz, acc);
       acc = fmaf(dx, a.c.ix, acc);
       acc = fmaf(dy, a.c.iy, acc);
       acc = fmaf(dz, a.c.iz, acc);

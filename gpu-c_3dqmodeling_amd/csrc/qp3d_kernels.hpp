@@ -128,7 +128,6 @@ __device__ __forceinline__ void unpack(int4 w, int &x0, int &x1, int &y0, int &y
 
 struct Coef {
   float c1[H], c2[H];        // 10th-order rows of the FD tables (entries 31..35)
-  float s2;                  // 2 * sum(c2)
   float ix, iy, iz;          // 1/dx, 1/dy, 1/dz
   float ixx, iyy, izz;       // 1/dx^2, 1/dy^2, 1/dz^2
   float dt;
@@ -382,6 +381,16 @@ __device__ __forceinline__ float scalar_op_m(float mode, float ux, float uy, flo
 __device__ __forceinline__ float scalar_op(float ux, float uy, float uz, float e, float ed, float thr)
 {
   return scalar_op_m(cell_mode(ux, uy, uz), ux, uy, uz, e, ed, thr);
+}
+
+// One tap of a 10th-order second derivative: acc + c2_m ((u+m - 2 u0) + u-m), with the
+// reference's term-wise second difference (CPU_function.c:383-404), twou0 = 2 u0 (exact). Summing
+// c2_m (u+m + u-m) and subtracting 2 u0 sum(c2) instead cancels two terms three times the
+// size of u0 and carries about three times the rounding noise, which the threshold of the
+// scalar operator amplifies over long runs.
+__device__ __forceinline__ float d2tap(float c2m, float up, float um, float twou0, float acc)
+{
+  return fmaf(c2m, (up - twou0) + um, acc);
 }
 
 // The threshold 0.01*G of the scalar operator: the reference compares |grad u| > 0.01*G in
@@ -717,24 +726,25 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
         const int X = H + L.row(r), Y = L.Y;
         const int x = x0 + L.row(r);
         const float c0 = q[r][(H + it) % QN];
-        // second derivatives as sum_m c2_m (u+m + u-m) - 2 u0 sum_m c2_m, first derivatives as
-        // sum_m c1_m (u+m - u-m); each pair's sum and difference are shared.
+        // second derivatives as sum_m c2_m ((u+m - 2 u0) + u-m) (d2tap), first derivatives as
+        // sum_m c1_m (u+m - u-m), from the same pair of values.
         float Lxx = 0.f, Lyy = 0.f, Lzz = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
+        const float c02 = 2.f * c0;
 #pragma unroll
         for (int k = 1; k <= H; k++) {
           const float ax = s[X + k][Y], bx = s[X - k][Y];
           const float ay = s[X][Y + k], by = s[X][Y - k];
           const float az = q[r][(H + k + it) % QN], bz = q[r][(H - k + it) % QN];
-          Lxx = fmaf(a.c.c2[k - 1], ax + bx, Lxx);
-          Lyy = fmaf(a.c.c2[k - 1], ay + by, Lyy);
-          Lzz = fmaf(a.c.c2[k - 1], az + bz, Lzz);
+          Lxx = d2tap(a.c.c2[k - 1], ax, bx, c02, Lxx);
+          Lyy = d2tap(a.c.c2[k - 1], ay, by, c02, Lyy);
+          Lzz = d2tap(a.c.c2[k - 1], az, bz, c02, Lzz);
           ux = fmaf(a.c.c1[k - 1], ax - bx, ux);
           uy = fmaf(a.c.c1[k - 1], ay - by, uy);
           uz = fmaf(a.c.c1[k - 1], az - bz, uz);
         }
-        const float Uxx = fmaf(-a.c.s2, c0, Lxx) * a.c.ixx;
-        const float Uyy = fmaf(-a.c.s2, c0, Lyy) * a.c.iyy;
-        const float Uzz = fmaf(-a.c.s2, c0, Lzz) * a.c.izz;
+        const float Uxx = Lxx * a.c.ixx;
+        const float Uyy = Lyy * a.c.iyy;
+        const float Uzz = Lzz * a.c.izz;
         ux *= a.c.ix; uy *= a.c.iy; uz *= a.c.iz;
         const float e = C.e[r], ed = C.ed[r], v2 = C.v2[r];
         const float mode = cell_mode(ux, uy, uz);
@@ -910,15 +920,16 @@ __device__ __forceinline__ void fix_cell(const UpdArgs<S> &a, unsigned i, float 
   if (z < H || z >= g.nz - H || x < H || x >= g.nx - H || y < H || y >= g.ny - H) return;
   const unsigned b = 4u * i, bx = 4u * (unsigned)g.ny, bz = 4u * g.pln;
   const float c0 = S::ld(a.ucur, b);
+  const float c02 = 2.f * c0;
   float Lxx = 0.f, Lyy = 0.f, Lzz = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
 #pragma unroll
   for (int m = 1; m <= H; m++) {
     const float ax = S::ld(a.ucur, b + m * bx), bx_ = S::ld(a.ucur, b - m * bx);
     const float ay = S::ld(a.ucur, b + 4u * m), by = S::ld(a.ucur, b - 4u * m);
     const float az = S::ld(a.ucur, b + m * bz), bz_ = S::ld(a.ucur, b - m * bz);
-    Lxx = fmaf(a.c.c2[m - 1], ax + bx_, Lxx);
-    Lyy = fmaf(a.c.c2[m - 1], ay + by, Lyy);
-    Lzz = fmaf(a.c.c2[m - 1], az + bz_, Lzz);
+    Lxx = d2tap(a.c.c2[m - 1], ax, bx_, c02, Lxx);
+    Lyy = d2tap(a.c.c2[m - 1], ay, by, c02, Lyy);
+    Lzz = d2tap(a.c.c2[m - 1], az, bz_, c02, Lzz);
     ux = fmaf(a.c.c1[m - 1], ax - bx_, ux);
     uy = fmaf(a.c.c1[m - 1], ay - by, uy);
     uz = fmaf(a.c.c1[m - 1], az - bz_, uz);
@@ -926,9 +937,9 @@ __device__ __forceinline__ void fix_cell(const UpdArgs<S> &a, unsigned i, float 
   ux *= a.c.ix; uy *= a.c.iy; uz *= a.c.iz;
   const float mode = cell_mode(ux, uy, uz);
   if ((mode > ts) == (mode > tt)) return;
-  const float Uxx = fmaf(-a.c.s2, c0, Lxx) * a.c.ixx;
-  const float Uyy = fmaf(-a.c.s2, c0, Lyy) * a.c.iyy;
-  const float Uzz = fmaf(-a.c.s2, c0, Lzz) * a.c.izz;
+  const float Uxx = Lxx * a.c.ixx;
+  const float Uyy = Lyy * a.c.iyy;
+  const float Uzz = Lzz * a.c.izz;
   const float e = a.eps[i], ed = a.epsd[i], v2 = a.vdt2[i];
   const float dS = scalar_op_m(mode, ux, uy, uz, e, ed, tt) - scalar_op_m(mode, ux, uy, uz, e, ed, ts);
   const float e1 = fmaf(2.f, e, 1.f);

@@ -570,7 +570,6 @@ int qp3d_make_tables(qp3d_plan *p, const float *dlr, const float *ddlr, const fl
   k.ixx = (float)(1.0 / ((double)cf.dx * cf.dx));
   k.iyy = (float)(1.0 / ((double)cf.dy * cf.dy));
   k.izz = (float)(1.0 / ((double)cf.dz * cf.dz));
-  k.s2 = 2.0f * (k.c2[0] + k.c2[1] + k.c2[2] + k.c2[3] + k.c2[4]);
   for (int m = 0; m < H; m++) {
     k.g1x[m] = k.c1[m] * k.ix;
     k.g1y[m] = k.c1[m] * k.iy;
@@ -1102,15 +1101,13 @@ extern "C" int qp3d_work_items(const qp3d_config *cfg, int which, int capacity, 
     return QP3D_ERR_ARG;
   }
   qp3d_plan p;  // host-side description only: nothing is allocated
-  p.cfg = c;
-  const int tu = c.tile_update > 0 ? c.tile_update - 1 : 0;
-  const int tg = c.tile_grad > 0 ? c.tile_grad - 1 : default_tile_grad(c.storage);
-  const int tp = c.tile_pml > 0 ? c.tile_pml - 1 : 5, tpy = c.tile_pml_y > 0 ? c.tile_pml_y - 1 : 1;
+  qp3d_plan_describe(&p, c);
   std::vector<int4> it, ip, ipy;
-  if (which == 0)
-    qp3d_items_uniform(&p, tg, capacity, it);
-  else
-    qp3d_items_update(&p, tu, tp, capacity, capacity, it, ip, &ipy, tpy, capacity);
+  if (which == 0) {
+    qp3d_items_uniform(&p, p.tile_g, capacity, it);
+  } else {
+    qp3d_items_update(&p, p.tile_u, p.tile_p, capacity, capacity, it, ip, &ipy, p.tile_py, capacity);
+  }
   const std::vector<int4> &v = which == 3 ? ipy : which == 2 ? ip : it;
   for (size_t i = 0; i < v.size() && (int)i < cap; i++) {
     out[4 * i] = v[i].x;

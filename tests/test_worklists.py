@@ -42,6 +42,9 @@ CASES = [
     (95, 97, 88, 36, 2, 1, 6, 4, 7, 300),
     (85, 77, 90, 13, 4, 2, 1, 2, 0, 100),
     (70, 75, 66, 12, 7, 7, 7, 7, 9, 200),
+    # ny % 4 == 0 with the default gradient tile: four y cells per lane, 64-aligned y segments
+    (41, 64, 37, 10, 0, 0, 0, 0, 0, 256),
+    (53, 140, 31, 12, 0, 0, 0, 0, 7, 96),
 ]
 
 
