@@ -30,7 +30,9 @@ struct qp3d_batch {
   // the batch kernels take G from the gradient pass: plans with gmax_mode 2 run with it off
   // while they belong to the batch, and get it back when the batch is destroyed
   std::vector<char> spec_saved;
-  WorkList wl[4];
+  // stage work lists; wl[4]: y-face items of shots without compact y-face arrays (pml > 32),
+  // launched with wl[3] (update kind 1 against kind 2)
+  WorkList wl[5];
   void *args_g[2] = {}, *args_u[2] = {};  // per-parity argument blocks of every shot
   int *kdev = nullptr;
   hipGraph_t graph = nullptr;
@@ -42,33 +44,42 @@ struct qp3d_batch {
   double timed_ms = 0;
 };
 
-// Work list of one stage over all shots, items in batch format.
+// Work list of one stage over all shots, items in batch format (stage 3 fills wl[3] and wl[4]).
 static int build_stage(qp3d_batch *b, int stage)
 {
-  int occ = 1;
+  int occ[3] = {1, 1, 1};  // stage 3: update kinds 1 and 2
   with_storage(b->storage, [&](auto sp) {
     using S = decltype(sp);
-    occ = stage == 0 ? occupancy_gradmax_batch<S>(b->tile[0])
-          : stage == 1 ? occupancy_update_batch<S, false>(b->tile[1])
-                       : occupancy_update_batch<S, true>(b->tile[stage]);
+    if (stage == 0) occ[0] = occupancy_gradmax_batch<S>(b->tile[0]);
+    else if (stage == 1) occ[0] = occupancy_update_batch<S, 0>(b->tile[1]);
+    else if (stage == 2) occ[0] = occupancy_update_batch<S, 1>(b->tile[2]);
+    else {
+      occ[1] = occupancy_update_batch<S, 1>(b->tile[3]);
+      occ[2] = occupancy_update_batch<S, 2>(b->tile[3]);
+    }
   });
   const int n = (int)b->plans.size();
-  const int cap = std::max(1, occ * qp3d_cu_count(b->plans[0]) / (b->share ? n : 1));
-  std::vector<int4> all;
+  std::vector<int4> all, all1;
   for (int i = 0; i < n; i++) {
+    qp3d_plan *p = b->plans[i];
+    const int o = stage == 3 ? occ[p->yw ? 2 : 1] : occ[0];
+    const int cap = std::max(1, o * qp3d_cu_count(p) / (b->share ? n : 1));
     std::vector<int4> it, ip, ipy;
     if (stage == 0)
-      qp3d_items_uniform(b->plans[i], b->tile[0], cap, it);
+      qp3d_items_uniform(p, b->tile[0], cap, it);
     else
-      qp3d_items_update(b->plans[i], b->tile[1], b->tile[2], cap, cap, it, ip, &ipy, b->tile[3], cap);
+      qp3d_items_update(p, b->tile[1], b->tile[2], cap, cap, it, ip, ipy, b->tile[3], cap);
     const std::vector<int4> &v = stage == 3 ? ipy : stage == 2 ? ip : it;
-    for (int4 w : v) all.push_back(make_int4(w.x, w.y, w.z | (w.w << 16), i));
+    std::vector<int4> &dst = stage == 3 && !p->yw ? all1 : all;
+    for (int4 w : v) dst.push_back(make_int4(w.x, w.y, w.z | (w.w << 16), i));
   }
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->gexec = nullptr;
   b->graph = nullptr;
-  return qp3d_upload_items(b->wl[stage], all, true);
+  int rc = qp3d_upload_items(b->wl[stage], all, true);
+  if (!rc && stage == 3) rc = qp3d_upload_items(b->wl[4], all1, true);
+  return rc;
 }
 
 template <class S>
@@ -102,15 +113,20 @@ static int batch_step_s(qp3d_batch *b, int k, Step st)
   for (int s = 0; s < 4; s++) {
     const bool tm = b->timing && s == b->timed_stage;
     if (tm) HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+    const UpdArgs<S> *A = (const UpdArgs<S> *)b->args_u[par];
     if (b->wl[s].n) {
       if (s == 0)
         launch_gradmax_batch<S>(b->tile[0], b->wl[0].n, b->wl[0].d, b->stream, (const GradArgs<S> *)b->args_g[par], st);
       else if (s == 1)
-        launch_update_batch<S, false>(b->tile[1], b->wl[1].n, b->wl[1].d, b->stream, (const UpdArgs<S> *)b->args_u[par],
-                                      st);
+        launch_update_batch<S, 0>(b->tile[1], b->wl[1].n, b->wl[1].d, b->stream, A, st);
+      else if (s == 2)
+        launch_update_batch<S, 1>(b->tile[2], b->wl[2].n, b->wl[2].d, b->stream, A, st);
       else
-        launch_update_batch<S, true>(b->tile[s], b->wl[s].n, b->wl[s].d, b->stream, (const UpdArgs<S> *)b->args_u[par],
-                                     st);
+        launch_update_batch<S, 2>(b->tile[3], b->wl[3].n, b->wl[3].d, b->stream, A, st);
+      HIP_TRY(hipGetLastError());
+    }
+    if (s == 3 && b->wl[4].n) {
+      launch_update_batch<S, 1>(b->tile[3], b->wl[4].n, b->wl[4].d, b->stream, A, st);
       HIP_TRY(hipGetLastError());
     }
     if (tm) {

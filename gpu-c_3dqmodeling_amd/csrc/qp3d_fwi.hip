@@ -123,7 +123,7 @@ extern "C" int qp3d_fwi_create(qp3d_plan *p, int interval, int ckpt_storage, qp3
     // as in the forward plan, the y-face PML tiles go to a launch of their own
     std::vector<int4> it, ip, ipy;
     const int ncu = qp3d_cu_count(p);
-    qp3d_items_update(p, TILE_PROD, TILE_PROD, oi * ncu, op * ncu, it, ip, &ipy, TILE_PROD, op * ncu);
+    qp3d_items_update(p, TILE_PROD, TILE_PROD, oi * ncu, op * ncu, it, ip, ipy, TILE_PROD, op * ncu);
     if ((rc = qp3d_upload_items(f->wl_i, it, false)) || (rc = qp3d_upload_items(f->wl_p, ip, false)) ||
         (rc = qp3d_upload_items(f->wl_py, ipy, false)))
       return fail(rc);

@@ -1,13 +1,14 @@
 // Instantiations of the update kernels (plain, predicted-threshold, batched) for every tile
 // shape, for the storage policy QP3D_STORE and the tile kind QP3D_PML (0: interior tiles,
-// 1: PML tiles); the Makefile builds this file once per combination.
+// 1: PML tiles, 2: folded y-face tiles over compact arrays); the Makefile builds this file once
+// per combination.
 #include "qp3d_launch.hpp"
 
 #ifndef QP3D_STORE
 #error "QP3D_STORE (StoreF32, StoreBF16 or StoreF16) must be defined"
 #endif
 #ifndef QP3D_PML
-#error "QP3D_PML (0 or 1) must be defined"
+#error "QP3D_PML (0, 1 or 2) must be defined"
 #endif
 
 namespace qp3d {
@@ -21,7 +22,7 @@ int occ_of(const void *fn, int threads)
 }
 }  // namespace
 
-template <class S, bool PML>
+template <class S, int PML>
 void launch_update(int tile, int blocks, hipStream_t st, const UpdArgs<S> &a, bool spec)
 {
   int i = 0;
@@ -37,7 +38,7 @@ void launch_update(int tile, int blocks, hipStream_t st, const UpdArgs<S> &a, bo
 #undef QP3D_L
 }
 
-template <class S, bool PML>
+template <class S, int PML>
 int occupancy_update(int tile, bool spec)
 {
   int i = 0;
@@ -51,7 +52,7 @@ int occupancy_update(int tile, bool spec)
   return 1;
 }
 
-template <class S, bool PML>
+template <class S, int PML>
 void launch_update_batch(int tile, int blocks, const int4 *work, hipStream_t st, const UpdArgs<S> *A, Step s)
 {
   int i = 0;
@@ -64,7 +65,7 @@ void launch_update_batch(int tile, int blocks, const int4 *work, hipStream_t st,
 #undef QP3D_L
 }
 
-template <class S, bool PML>
+template <class S, int PML>
 int occupancy_update_batch(int tile)
 {
   int i = 0;

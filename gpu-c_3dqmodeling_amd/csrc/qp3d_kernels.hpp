@@ -12,7 +12,9 @@
 //                    (GPU_kernel.cu:552-565), source injection (GPU_kernel.cu:720-731) and the
 //                    receiver-plane write (3dPureqPModeling_single_shot.cu:333-337);
 //   k_update<PML=1>  the same for the PML tiles, with the split-field recursions
-//                    (GPU_kernel.cu:566-637).
+//                    (GPU_kernel.cu:566-637);
+//   k_update<PML=2>  the same for the folded y-face tiles, whose model and PML state live in
+//                    compact arrays (YCompact).
 // k_sfield forms the scalar-operator field of a snapshot step on request (the reference's
 // writer for it is commented out, 3dPureqPModeling_single_shot.cu:381-409).
 // Optionally (SPEC, gmax_mode 2) there is no gradient pass: k_update<..., SPEC> takes a
@@ -142,6 +144,34 @@ struct Step {
   __device__ __forceinline__ int get() const { return (kdev ? *kdev : 0) + koff; }
 };
 
+// The y faces of the PML fold into one tile per x row group when each is at most 27 cells
+// wide (pml <= 32: the two faces with their 5-cell halos fit one 64-lane wavefront).
+__host__ __device__ constexpr bool y_fold(int pml) { return pml <= 32; }
+
+// Compact arrays of the y-face family (folded faces only). Each y-face row is a 27-cell piece of
+// a 128-byte line in the full-grid layout, so per-cell loads of the y-face tiles would fetch
+// whole lines for partial use; these arrays hold the model and the seven PML state fields of the
+// y-face cells as [nz][nx - 2 pml][2 yw], yw = pml - 5 cells per face: row x - pml holds the low
+// face's y = 5..pml-1 in columns [0, yw) and the high face's y = ny-pml..ny-6 in [yw, 2 yw), so a
+// tile's rows are contiguous. u stays in the full-grid layout.
+template <class S>
+struct YCompact {
+  typename S::T *w1p, *w3p, *pp;        // PML slots k-1 (overwritten with the new value)
+  const typename S::T *w1c, *w3c, *pc;  // PML slots k
+  typename S::T *w21;
+  const float *vdt2, *eps, *epsd;
+  unsigned pln;  // cells per plane: (nx - 2 pml) * 2 yw
+  int yw;        // cells per face; 0: the y faces use the full-grid arrays
+};
+
+// Compact index (cells) of a cell of the y-face family: x in [pml, nx-pml), y in [5, pml) or
+// [ny-pml, ny-5).
+__device__ __forceinline__ unsigned ycompact_cell(int ny, int pml, int yw, unsigned pln, int z, int x, int y)
+{
+  const int col = y < pml ? y - H : yw + (y - (ny - pml));
+  return (unsigned)z * pln + (unsigned)(x - pml) * (unsigned)(2 * yw) + (unsigned)col;
+}
+
 // Parameters of one k_update launch.
 template <class S>
 struct UpdArgs {
@@ -153,6 +183,7 @@ struct UpdArgs {
   typename S::T *w1p, *w3p, *pp;            // PML slots k-1 (overwritten with the new value)
   const typename S::T *w1c, *w3c, *pc;      // PML slots k
   typename S::T *w21;                       // PML accumulator (updated in place)
+  YCompact<S> yc;                           // model and PML state of the y-face family, compact
   const float4 *pmltab;                     // [3][pml]: K1, K2, K3, E per face family and layer
   unsigned *gmax;                           // [nt] G per step (float bits); SPEC: G_k reduced here
   float *gpred;                             // SPEC: [nt] predicted G per step (fix-up writes k+1)
@@ -584,34 +615,70 @@ __global__ __launch_bounds__(T::NT) void k_gradmax_batch(const GradArgs<S> *A, c
 }
 
 
+// Update kinds (the PML template parameter of the update kernels): 0 interior tiles, 1 PML
+// tiles over the full-grid arrays, 2 folded y-face tiles over the compact arrays (YCompact).
+
+// The per-cell arrays a tile reads besides u_cur (model; u_prev or the PML state), with their
+// plane stride in cells.
+template <class S>
+struct CellSrc {
+  const float *v2, *e, *ed;
+  const typename S::T *uprev;  // interior tiles: u at step k-1
+  typename S::T *w1p, *w3p, *pp;
+  const typename S::T *w1c, *w3c, *pc;
+  typename S::T *w21;
+  size_t pln;
+};
+
+// The arrays of a tile of update kind PML: the full-grid arrays, read at the lane's own offsets
+// (Lane::off), or for kind 2 the compact arrays, read at the offsets this writes into yoff
+// (bytes of 4-byte cells; folded tile: lanes 0..31 hold the low face, 32..63 the high face).
+template <class S, class T, int PML>
+__device__ __forceinline__ CellSrc<S> cell_src(const UpdArgs<S> &a, const Lane<T> &L, int x0, int x1,
+                                               unsigned (&yoff)[T::RX])
+{
+  if constexpr (PML == 2) {
+    const YCompact<S> &c = a.yc;
+    const int col = (L.ty >> 5) ? c.yw + (L.yc - (a.g.ny - a.g.pml)) : L.yc - H;
+#pragma unroll
+    for (int r = 0; r < T::RX; r++) {
+      const int x = min(x0 + L.row(r), x1 - 1);
+      yoff[r] = 4u * ((unsigned)(x - a.g.pml) * (unsigned)(2 * c.yw) + (unsigned)col);
+    }
+    return CellSrc<S>{c.vdt2, c.eps, c.epsd, nullptr, c.w1p, c.w3p, c.pp, c.w1c, c.w3c, c.pc, c.w21, c.pln};
+  } else {
+    return CellSrc<S>{a.vdt2, a.eps, a.epsd, a.unext, a.w1p, a.w3p, a.pp, a.w1c, a.w3c, a.pc, a.w21, a.g.pln};
+  }
+}
+
 // Per-cell inputs of k_update other than u_cur: model, and either u_prev (interior tiles) or
 // the seven PML state values (PML tiles).
-template <class S, class T, bool PML>
+template <class S, class T, int PML>
 struct CellLoad {
   static constexpr int NS = PML ? 7 : 1;
   float v2[T::RX], e[T::RX], ed[T::RX];
   float st[T::RX][NS];  // interior: st[0] = u_prev; PML: w1p w1c pp pc w3p w3c w21
 
-  __device__ __forceinline__ void issue(const UpdArgs<S> &a, const Lane<T> &L, int z)
+  __device__ __forceinline__ void issue(const CellSrc<S> &c, const unsigned (&offs)[T::RX], int z)
   {
-    const size_t zo = (size_t)z * a.g.pln;
-    const float *pv = a.vdt2 + zo, *pe = a.eps + zo, *pd = a.epsd + zo;
+    const size_t zo = (size_t)z * c.pln;
+    const float *pv = c.v2 + zo, *pe = c.e + zo, *pd = c.ed + zo;
 #pragma unroll
     for (int r = 0; r < T::RX; r++) {
-      v2[r] = ld_model(pv, L.off[r]);
-      e[r] = ld_model(pe, L.off[r]);
-      ed[r] = ld_model(pd, L.off[r]);
+      v2[r] = ld_model(pv, offs[r]);
+      e[r] = ld_model(pe, offs[r]);
+      ed[r] = ld_model(pd, offs[r]);
     }
     if constexpr (!PML) {
-      const typename S::T *pu = a.unext + zo;
+      const typename S::T *pu = c.uprev + zo;
 #pragma unroll
-      for (int r = 0; r < T::RX; r++) st[r][0] = S::ld_stream(pu, L.off[r]);
+      for (int r = 0; r < T::RX; r++) st[r][0] = S::ld_stream(pu, offs[r]);
     } else {
-      const typename S::T *src[7] = {a.w1p + zo, a.w1c + zo, a.pp + zo, a.pc + zo, a.w3p + zo, a.w3c + zo, a.w21 + zo};
+      const typename S::T *src[7] = {c.w1p + zo, c.w1c + zo, c.pp + zo, c.pc + zo, c.w3p + zo, c.w3c + zo, c.w21 + zo};
 #pragma unroll
       for (int j = 0; j < 7; j++)
 #pragma unroll
-        for (int r = 0; r < T::RX; r++) st[r][j] = S::ld(src[j], L.off[r]);  // cached: 6% faster than non-temporal
+        for (int r = 0; r < T::RX; r++) st[r][j] = S::ld(src[j], offs[r]);  // cached: 6% faster than non-temporal
     }
   }
 };
@@ -619,7 +686,7 @@ struct CellLoad {
 // One work item of the update pass (w: packed tile, kstep: time step).
 // SPEC (gmax_mode 2): the threshold is predicted (spec_band), cells in the band are marked in
 // a.fixmask, and the tile's maximum gradient component goes into G_k.
-template <class S, class T, bool PML, bool SPEC = false>
+template <class S, class T, int PML, bool SPEC = false>
 __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kstep)
 {
   __shared__ float sm[2][T::LX + 1][T::LY];
@@ -629,6 +696,11 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
   unpack(w, x0, x1, y0, y1, z0, z1, fold);
   if (z0 >= z1) return;
   Lane<T> L(g, x0, x1, y0, y1, fold);
+  // per-cell arrays and the lane's offsets into them (kind 2: compact y-face arrays; the host
+  // gives kind 2 folded tiles only)
+  unsigned yoff[T::RX];
+  const CellSrc<S> cs = cell_src<S, T, PML>(a, L, x0, x1, yoff);
+  const unsigned(&coff)[T::RX] = PML == 2 ? yoff : L.off;
   const int y = L.y;
   const bool ycmp = L.yon;
   const int nx = g.nx, ny = g.ny, nz = g.nz, pml = g.pml;
@@ -661,7 +733,7 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
   float4 trow[T::RX];
   float sgn = 1.f;
   if constexpr (PML) {
-    fam = (x1 <= pml || x0 >= nx - pml) ? 0 : ((y1 <= pml || y0 >= ny - pml) ? 1 : 2);
+    fam = PML == 2 ? 1 : (x1 <= pml || x0 >= nx - pml) ? 0 : ((y1 <= pml || y0 >= ny - pml) ? 1 : 2);
     if (fam == 0) {
       sgn = x1 <= pml ? 1.f : -1.f;
 #pragma unroll
@@ -688,7 +760,7 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
   // of plane `it` as CL[it & 1] at compile time; the rolled PML march copies the next into
   // the current
   CellLoad<S, T, PML> CL[2];
-  CL[0].issue(a, L, z0);
+  CL[0].issue(cs, coff, z0);
   float centre[T::RX];
 #pragma unroll
   for (int r = 0; r < T::RX; r++) {
@@ -705,11 +777,15 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
       const int z = zb + it;
       const bool more = z + 1 < z1;
       L.refresh();
+      if constexpr (PML == 2) {
+#pragma unroll
+        for (int r = 0; r < T::RX; r++) asm volatile("" : "+v"(yoff[r]));
+      }
       CellLoad<S, T, PML> &C = CL[PML ? 0 : (it & 1)];
       CellLoad<S, T, PML> &Cn = CL[PML ? 1 : ((it + 1) & 1)];
       if (more) {
         P.issue(a.ucur, g, L, z + 1);
-        Cn.issue(a, L, z + 1);
+        Cn.issue(cs, coff, z + 1);
       }
       // interior: unrolled by 22, the buffer of plane `it` is it & 1 at compile time
       float (*s)[T::LY] = sm[PML ? b : (it & 1)];
@@ -774,10 +850,12 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
           const float pn = S::rnd(-t.x * pc + t.y * pp + sgn * (Cn_ * t.w * gn));
           const float w21n = S::rnd(a.c.dt * pn + w21 * t.z);
           const float w3n = S::rnd(B * Tg + 2.f * w3c - w3p);
-          S::st(a.w1p + zo, o, w1n);
-          S::st(a.pp + zo, o, pn);
-          S::st(a.w21 + zo, o, w21n);
-          S::st(a.w3p + zo, o, w3n);
+          const size_t zs = (size_t)z * cs.pln;
+          const unsigned os = coff[r];
+          S::st(cs.w1p + zs, os, w1n);
+          S::st(cs.pp + zs, os, pn);
+          S::st(cs.w21 + zs, os, w21n);
+          S::st(cs.w3p + zs, os, w3n);
           un = w1n + w21n + w3n;
         }
         if (src_plane && x >= a.sb_x0 && x <= a.sb_x1 && y >= a.sb_y0 && y <= a.sb_y1) {
@@ -822,13 +900,13 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
   if constexpr (SPEC) reduce_gmax<T>(gm, nonfinite, L.tx, L.ty, a.gmax + kstep, a.flag);
 }
 
-template <class S, class T, bool PML, bool SPEC = false>
+template <class S, class T, int PML, bool SPEC = false>
 __global__ __launch_bounds__(T::NT) void k_update(UpdArgs<S> a)
 {
   update_tile<S, T, PML, SPEC>(a, a.work[blockIdx.x], a.step.get());
 }
 
-template <class S, class T, bool PML>
+template <class S, class T, int PML>
 __global__ __launch_bounds__(T::NT) void k_update_batch(const UpdArgs<S> *A, const int4 *work, Step st)
 {
   int shot;
@@ -967,15 +1045,20 @@ __device__ __forceinline__ void fix_cell(const UpdArgs<S> &a, unsigned i, float 
       sgn = z < pml ? 1.f : -1.f;
       dC = dB; Unn = Uzz; Tg = e1 * (Uxx + Uyy); gn = uz;
     }
-    const float w1 = S::ld(a.w1p, b), p = S::ld(a.pp, b), w21 = S::ld(a.w21, b), w3 = S::ld(a.w3p, b);
+    // state of a y-face cell in the compact arrays when the plan keeps them
+    const bool cy = !xf && yf && a.yc.yw > 0;
+    typename S::T *const w1a = cy ? a.yc.w1p : a.w1p, *const pa = cy ? a.yc.pp : a.pp;
+    typename S::T *const w21a = cy ? a.yc.w21 : a.w21, *const w3a = cy ? a.yc.w3p : a.w3p;
+    const unsigned bs = cy ? 4u * ycompact_cell(g.ny, pml, a.yc.yw, a.yc.pln, z, x, y) : b;
+    const float w1 = S::ld(w1a, bs), p = S::ld(pa, bs), w21 = S::ld(w21a, bs), w3 = S::ld(w3a, bs);
     const float w1n = S::rnd(fmaf(dC, Unn, w1));
     const float pn = S::rnd(fmaf(sgn, dC * t.w * gn, p));
     const float w21n = S::rnd(fmaf(a.c.dt, pn - p, w21));
     const float w3n = S::rnd(fmaf(dB, Tg, w3));
-    S::st(a.w1p, b, w1n);
-    S::st(a.pp, b, pn);
-    S::st(a.w21, b, w21n);
-    S::st(a.w3p, b, w3n);
+    S::st(w1a, bs, w1n);
+    S::st(pa, bs, pn);
+    S::st(w21a, bs, w21n);
+    S::st(w3a, bs, w3n);
     unn = S::rnd(un + ((w1n - w1) + (w21n - w21) + (w3n - w3)));
   }
   S::st(a.unext, b, unn);

@@ -19,20 +19,20 @@ void launch_mark(int tile, int blocks, hipStream_t st, const GradArgs<S> &a);
 template <class S>
 int occupancy_gradmax(int tile);
 
-// update pass (k_update<PML>; spec: the predicted-threshold form)
-template <class S, bool PML>
+// update pass (k_update<PML>, PML = update kind 0, 1 or 2; spec: the predicted-threshold form)
+template <class S, int PML>
 void launch_update(int tile, int blocks, hipStream_t st, const UpdArgs<S> &a, bool spec);
-template <class S, bool PML>
+template <class S, int PML>
 int occupancy_update(int tile, bool spec);
 
 // batched forms (argument blocks in device memory, items carry the grid index)
 template <class S>
 void launch_gradmax_batch(int tile, int blocks, const int4 *work, hipStream_t st, const GradArgs<S> *A, Step s);
-template <class S, bool PML>
+template <class S, int PML>
 void launch_update_batch(int tile, int blocks, const int4 *work, hipStream_t st, const UpdArgs<S> *A, Step s);
 template <class S>
 int occupancy_gradmax_batch(int tile);
-template <class S, bool PML>
+template <class S, int PML>
 int occupancy_update_batch(int tile);
 
 }  // namespace qp3d

@@ -149,12 +149,11 @@ extern "C" int qp3d_modeling3d_noalloc(int nx, int ny, int nz, int nt, int ntsna
   const size_t N = (size_t)P.N;
   const size_t fs = field_stride(sizeof(float) * N);
   Carve cv{base};
-  P.esz = 4;
-  P.fstride = fs;
-  P.state_bytes = 9 * fs;
-  P.state = (char *)cv.take(P.state_bytes);
+  // state block (the plan's layout, FP32), (vp dt)^2 and eps - del, the compact y-face model
+  char *state = (char *)cv.take(P.state_bytes);
   P.vdt2 = (float *)cv.take(fs);
   P.epsd = (float *)cv.take(fs);
+  float *ymodel = P.yw ? (float *)cv.take(3 * field_stride(sizeof(float) * (size_t)P.ycells)) : nullptr;
   P.pmltab = (float4 *)cv.take(sizeof(float4) * 3 * (size_t)pml);
   P.gmax = (unsigned *)cv.take(sizeof(unsigned) * (size_t)(nt > 0 ? nt : 1));
   P.flag = (int *)cv.take(sizeof(int));
@@ -174,12 +173,8 @@ extern "C" int qp3d_modeling3d_noalloc(int nx, int ny, int nz, int nt, int ntsna
   }
   P.n_int_cells = P.wl_int.cells;
   P.n_pml_cells = P.wl_pml.cells + P.wl_pmly.cells;
-  char *st = P.state;
-  P.u[0] = st; P.u[1] = st + fs;
-  P.w1[0] = st + 2 * fs; P.w1[1] = st + 3 * fs;
-  P.w3[0] = st + 4 * fs; P.w3[1] = st + 5 * fs;
-  P.pp[0] = st + 6 * fs; P.pp[1] = st + 7 * fs;
-  P.w21 = st + 8 * fs;
+  qp3d_bind_state(&P, state, nullptr);
+  if (P.yw) qp3d_bind_ymodel(&P, ymodel);
   P.eps = const_cast<float *>(epsilon);  // read only
   P.wav = const_cast<float *>(wavelet);  // read only
   P.rec = record;
@@ -188,6 +183,8 @@ extern "C" int qp3d_modeling3d_noalloc(int nx, int ny, int nz, int nt, int ntsna
   HIP_TRY(hipMemcpyAsync(P.vdt2, vp, sizeof(float) * N, hipMemcpyDeviceToDevice, s));
   HIP_TRY(hipMemcpyAsync(P.epsd, delta, sizeof(float) * N, hipMemcpyDeviceToDevice, s));
   qp3d_launch_model(P.vdt2, P.eps, P.epsd, dt, (long)N, s);
+  HIP_TRY(hipGetLastError());
+  qp3d_launch_ymodel(&P, s);
   HIP_TRY(hipGetLastError());
   std::vector<float4> tab;
   if ((rc = qp3d_make_tables(&P, dlr, ddlr, dtb, ddtb, dfb, ddfb, c, c2, tab))) return rc;

@@ -136,7 +136,7 @@ int qp3d_build_uniform_list(const qp3d_plan *p, int tile, int occ, WorkList &w)
 // Update tiles aligned to the PML boundary in x and y, z split at the PML planes: interior
 // tiles (tile shape tile_i) hold only interior cells, PML tiles (tile_p) one face family each.
 void qp3d_items_update(const qp3d_plan *p, int tile_i, int tile_p, int cap_i, int cap_p, std::vector<int4> &it,
-                       std::vector<int4> &ip, std::vector<int4> *ipy, int tile_py, int cap_py)
+                       std::vector<int4> &ip, std::vector<int4> &ipy, int tile_py, int cap_py)
 {
   const qp3d_config &c = p->cfg;
   const int pml = c.pml;
@@ -154,7 +154,7 @@ void qp3d_items_update(const qp3d_plan *p, int tile_i, int tile_p, int cap_i, in
   std::vector<bool> xui, xi, xiy, yi;
   segs(xu, xui, c.nx, pml, tile_tx(tile_i));
   segs(xs, xi, c.nx, pml, tile_tx(tile_p));
-  segs(xsy, xiy, c.nx, pml, tile_tx(ipy ? tile_py : tile_p));
+  segs(xsy, xiy, c.nx, pml, tile_tx(tile_py));
   segs(ys, yi, c.ny, pml, TY);
   // x-face tiles own every y (edges and corners included), so their y cuts need not follow the
   // y faces: 64-cell pieces on 64-cell boundaries
@@ -164,7 +164,7 @@ void qp3d_items_update(const qp3d_plan *p, int tile_i, int tile_p, int cap_i, in
     for (size_t b = 0; b < ys.size(); b++) nint_tiles += xui[a] && yi[b];
   // The two y faces of an x-interior tile column share one folded tile when each face fits
   // half a wavefront with its 5-cell halo.
-  const bool fold = pml <= 32;
+  const bool fold = y_fold(pml);
   std::vector<std::pair<int4, bool>> ptiles, ytiles;  // PML xy tiles over the whole z range
   for (size_t a = 0; a < xs.size(); a++)
     if (!xi[a])
@@ -179,14 +179,10 @@ void qp3d_items_update(const qp3d_plan *p, int tile_i, int tile_p, int cap_i, in
         if (!yi[b]) ytiles.push_back({pack(xsy[a].a, xsy[a].b, ys[b].a, ys[b].b, 0, 0), false});
     }
   }
-  if (!ipy) {
-    ptiles.insert(ptiles.end(), ytiles.begin(), ytiles.end());
-    ytiles.clear();
-  }
   const int zmid = c.nz - 2 * pml, zpml = pml - H;
   const int zc_i = choose_zchunk(nint_tiles, zmid, cap_i, c.zchunk);
   const int zc_p = choose_zchunk((int)ptiles.size(), c.nz - 2 * H, cap_p, c.zchunk);
-  if (ipy && !ytiles.empty()) {
+  if (!ytiles.empty()) {
     const int zc_y = choose_zchunk((int)ytiles.size(), c.nz - 2 * H, cap_py, c.zchunk);
     for (int z = H; z < c.nz - H; z += zc_y)
       for (const auto &t : ytiles) {
@@ -194,7 +190,7 @@ void qp3d_items_update(const qp3d_plan *p, int tile_i, int tile_p, int cap_i, in
         w.z = z;
         w.w = std::min(z + zc_y, c.nz - H);
         if (t.second) w.y = (int)((unsigned)w.y | 0x80000000u);
-        ipy->push_back(w);
+        ipy.push_back(w);
       }
   }
   for (int z = pml; z < c.nz - pml; z += zc_i)
@@ -220,17 +216,6 @@ void qp3d_items_update(const qp3d_plan *p, int tile_i, int tile_p, int cap_i, in
   }
 }
 
-int qp3d_build_update_lists(const qp3d_plan *p, int tile_i, int tile_p, int occ_i, int occ_p, WorkList &wi,
-                            WorkList &wp)
-{
-  std::vector<int4> it, ip;
-  const int ncu = qp3d_cu_count(p);
-  qp3d_items_update(p, tile_i, tile_p, occ_i * ncu, occ_p * ncu, it, ip, nullptr, 0, 0);
-  int rc = upload_list(wi, it);
-  if (rc) return rc;
-  return upload_list(wp, ip);
-}
-
 // Default gradient-pass tile (QP3D_TILES index): the 4x4 run tile, or 16x1 with BF16 storage
 // (measured at 576^3: 0.39 ms FP16 and 0.25 ms FP32 with 4x4; BF16 0.45 ms with 16x1 against
 // 0.54 ms with 4x4).
@@ -242,16 +227,16 @@ void qp3d_plan_items(const qp3d_plan *p, std::vector<int4> (&items)[4])
   with_storage(p->cfg.storage, [&](auto sp) {
     using S = decltype(sp);
     occ_g = occupancy_gradmax<S>(p->tile_g);
-    occ_i = occupancy_update<S, false>(p->tile_u, p->spec);
-    occ_p = occupancy_update<S, true>(p->tile_p, p->spec);
-    occ_py = occupancy_update<S, true>(p->tile_py, p->spec);
+    occ_i = occupancy_update<S, 0>(p->tile_u, p->spec);
+    occ_p = occupancy_update<S, 1>(p->tile_p, p->spec);
+    occ_py = p->yw ? occupancy_update<S, 2>(p->tile_py, p->spec) : occupancy_update<S, 1>(p->tile_py, p->spec);
   });
   const int ncu = qp3d_cu_count(p);
   for (std::vector<int4> &v : items) v.clear();
   qp3d_items_uniform(p, p->tile_g, occ_g * ncu, items[0]);
   // The PML update is two launches: x and z faces, then the y faces (folded tiles fill half a
   // wavefront per face and run best with a narrower tile; measured at 576^3).
-  qp3d_items_update(p, p->tile_u, p->tile_p, occ_i * ncu, occ_p * ncu, items[1], items[2], &items[3], p->tile_py,
+  qp3d_items_update(p, p->tile_u, p->tile_p, occ_i * ncu, occ_p * ncu, items[1], items[2], items[3], p->tile_py,
                     occ_py * ncu);
 }
 
@@ -337,6 +322,54 @@ void qp3d_plan_describe(qp3d_plan *p, const qp3d_config &c)
   p->spec = c.gmax_mode == 2;
   // a captured loop runs on one stream, and so does a step whose fix-up follows all updates
   p->two_streams = c.streams == 2 && !c.use_graph && !p->spec;
+  p->esz = c.storage == QP3D_STORAGE_FP32 ? 4 : 2;
+  // folded y faces keep their model and PML state in compact arrays (YCompact)
+  p->yw = y_fold(c.pml) ? c.pml - H : 0;
+  p->ypln = (unsigned)(c.nx - 2 * c.pml) * (unsigned)(2 * p->yw);
+  p->ycells = (long)c.nz * p->ypln;
+  qp3d_state_layout(p);
+}
+
+static size_t ymodel_stride(const qp3d_plan *p) { return p->yw ? field_stride(sizeof(float) * (size_t)p->ycells) : 0; }
+
+void qp3d_state_layout(qp3d_plan *p)
+{
+  p->fstride = field_stride(p->esz * (size_t)p->N);
+  p->ystride = p->yw ? field_stride(p->esz * (size_t)p->ycells) : 0;
+  p->state_bytes = 9 * p->fstride + 7 * p->ystride;
+  p->model_bytes = 3 * field_stride(sizeof(float) * (size_t)p->N) + 3 * ymodel_stride(p);
+}
+
+// model == nullptr: the state only (the caller places the model arrays).
+void qp3d_bind_state(qp3d_plan *p, char *s, float *model)
+{
+  const size_t fs = p->fstride, ys = p->ystride;
+  p->state = s;
+  p->u[0] = s; p->u[1] = s + fs;
+  p->w1[0] = s + 2 * fs; p->w1[1] = s + 3 * fs;
+  p->w3[0] = s + 4 * fs; p->w3[1] = s + 5 * fs;
+  p->pp[0] = s + 6 * fs; p->pp[1] = s + 7 * fs;
+  p->w21 = s + 8 * fs;
+  if (p->yw) {
+    char *y = s + 9 * fs;
+    p->yw1[0] = y; p->yw1[1] = y + ys;
+    p->yw3[0] = y + 2 * ys; p->yw3[1] = y + 3 * ys;
+    p->ypp[0] = y + 4 * ys; p->ypp[1] = y + 5 * ys;
+    p->yw21 = y + 6 * ys;
+  }
+  if (model) {
+    const size_t ms = field_stride(sizeof(float) * (size_t)p->N) / sizeof(float);
+    p->model = model;
+    p->vdt2 = model; p->eps = model + ms; p->epsd = model + 2 * ms;
+    if (p->yw) qp3d_bind_ymodel(p, model + 3 * ms);
+  }
+}
+
+// The compact y-face model copies at ym (3 * ymodel_stride bytes).
+void qp3d_bind_ymodel(qp3d_plan *p, float *ym)
+{
+  const size_t yms = ymodel_stride(p) / sizeof(float);
+  p->yvdt2 = ym; p->yeps = ym + yms; p->yepsd = ym + 2 * yms;
 }
 
 
@@ -374,7 +407,6 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
     }
     return 0;
   };
-  const size_t fb = sizeof(float) * (size_t)p->N;
   if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
     qp3d_set_error("plan_create: hipStreamCreate failed");
     return fail(QP3D_ERR_HIP);
@@ -391,20 +423,12 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
     }
   }
   int rc;
-  p->esz = c.storage == QP3D_STORAGE_FP32 ? 4 : 2;
-  const size_t fs = field_stride(p->esz * (size_t)p->N);
-  p->fstride = fs;
-  p->state_bytes = 9 * fs;
-  if ((rc = dmalloc((void **)&p->state, 9 * fs, "wavefields and PML state"))) return fail(rc);
-  char *s = p->state;
-  p->u[0] = s; p->u[1] = s + fs;
-  p->w1[0] = s + 2 * fs; p->w1[1] = s + 3 * fs;
-  p->w3[0] = s + 4 * fs; p->w3[1] = s + 5 * fs;
-  p->pp[0] = s + 6 * fs; p->pp[1] = s + 7 * fs;
-  p->w21 = s + 8 * fs;
-  const size_t ms = field_stride(fb) / sizeof(float);  // floats between model fields
-  if ((rc = dmalloc((void **)&p->model, 3 * ms * sizeof(float), "model"))) return fail(rc);
-  p->vdt2 = p->model; p->eps = p->model + ms; p->epsd = p->model + 2 * ms;
+  char *state = nullptr;
+  float *model = nullptr;
+  if ((rc = dmalloc((void **)&state, p->state_bytes, "wavefields and PML state"))) return fail(rc);
+  p->state = state;
+  if ((rc = dmalloc((void **)&model, p->model_bytes, "model"))) return fail(rc);
+  qp3d_bind_state(p, state, model);
   if ((rc = dmalloc((void **)&p->pmltab, sizeof(float4) * 3 * c.pml, "PML table"))) return fail(rc);
   if ((rc = dmalloc((void **)&p->gmax, sizeof(unsigned) * (size_t)(c.nt > 0 ? c.nt : 1), "step maxima")))
     return fail(rc);
@@ -521,6 +545,46 @@ static __global__ void k_source_box(const float *src, int nx, int ny, int nz, in
   }
 }
 
+// Full-grid cell index of compact y-face cell i (YCompact layout; planes of ypln cells).
+static __device__ __forceinline__ long ycompact_to_full(long i, Geo g, int yw, unsigned ypln)
+{
+  const int z = (int)(i / ypln);
+  const int rem = (int)(i - (long)z * ypln);
+  const int xr = rem / (2 * yw), col = rem - xr * (2 * yw);
+  const int y = col < yw ? H + col : g.ny - g.pml + (col - yw);
+  return (long)z * g.pln + (long)(xr + g.pml) * g.ny + y;
+}
+
+// Compact y-face copies of the three model arrays.
+static __global__ void k_ymodel(const float *v2, const float *eps, const float *ed, float *yv2, float *yeps, float *yed,
+                                Geo g, int yw, unsigned ypln, long n)
+{
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long f = ycompact_to_full(i, g, yw, ypln);
+    yv2[i] = v2[f];
+    yeps[i] = eps[f];
+    yed[i] = ed[f];
+  }
+}
+
+// A compact y-face state field written into the y-face cells of its full-grid field.
+template <class E>
+static __global__ void k_yexpand(const E *cmp, E *full, Geo g, int yw, unsigned ypln, long n)
+{
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    full[ycompact_to_full(i, g, yw, ypln)] = cmp[i];
+}
+
+static unsigned grid_blocks(long n) { return (unsigned)std::min<long>((n + 255) / 256, 8192); }
+
+void qp3d_launch_ymodel(const qp3d_plan *p, hipStream_t s)
+{
+  if (!p->yw) return;
+  hipLaunchKernelGGL(k_ymodel, dim3(grid_blocks(p->ycells)), dim3(256), 0, s, (const float *)p->vdt2,
+                     (const float *)p->eps, (const float *)p->epsd, p->yvdt2, p->yeps, p->yepsd, p->geo, p->yw, p->ypln,
+                     p->ycells);
+}
+
 static __global__ void k_scale(float *a, long n, float f)
 {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) a[i] *= f;
@@ -613,6 +677,8 @@ extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps,
   HIP_TRY(hipMemcpyAsync(p->eps, eps, sizeof(float) * (size_t)N, hipMemcpyDefault, p->stream));
   HIP_TRY(hipMemcpyAsync(p->epsd, del, sizeof(float) * (size_t)N, hipMemcpyDefault, p->stream));
   qp3d_launch_model(p->vdt2, p->eps, p->epsd, cf.dt, N, p->stream);
+  HIP_TRY(hipGetLastError());
+  qp3d_launch_ymodel(p, p->stream);
   HIP_TRY(hipGetLastError());
   std::vector<float4> tab;
   int rc = qp3d_make_tables(p, dlr, ddlr, dtb, ddtb, dfb, ddfb, c, c2, tab);
@@ -785,19 +851,22 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on
 
   if (p->wl_int.n) {
     ua.work = p->wl_int.d;
-    launch_update<S, false>(p->tile_u, p->wl_int.n, s0, ua, p->spec);
+    launch_update<S, 0>(p->tile_u, p->wl_int.n, s0, ua, p->spec);
     HIP_TRY(hipGetLastError());
   }
   if (e) HIP_TRY(hipEventRecord(e[2], s0));
   hipStream_t ps = two ? p->stream2 : s0;
   if (p->wl_pml.n) {
     ua.work = p->wl_pml.d;
-    launch_update<S, true>(p->tile_p, p->wl_pml.n, ps, ua, p->spec);
+    launch_update<S, 1>(p->tile_p, p->wl_pml.n, ps, ua, p->spec);
     HIP_TRY(hipGetLastError());
   }
   if (p->wl_pmly.n) {
     ua.work = p->wl_pmly.d;
-    launch_update<S, true>(p->tile_py, p->wl_pmly.n, ps, ua, p->spec);
+    if (p->yw)
+      launch_update<S, 2>(p->tile_py, p->wl_pmly.n, ps, ua, p->spec);
+    else
+      launch_update<S, 1>(p->tile_py, p->wl_pmly.n, ps, ua, p->spec);
     HIP_TRY(hipGetLastError());
   }
   if (e && all) HIP_TRY(hipEventRecord(e[3], ps));
@@ -1021,17 +1090,29 @@ extern "C" int qp3d_plan_field(const qp3d_plan *p, int which, float *out)
   HIP_TRY(hipSetDevice(p->cfg.device));
   HIP_TRY(hipStreamSynchronize(p->stream));
   const int newest = (p->steps_done - 1) & 1, older = newest ^ 1;
-  const void *src = nullptr;
+  void *src = nullptr, *ysrc = nullptr;
   switch (which) {
   case 0: src = p->u[newest]; break;
   case 1: src = p->u[older]; break;
-  case 2: src = p->w1[newest]; break;
-  case 3: src = p->w1[older]; break;
-  case 4: src = p->w3[newest]; break;
-  case 5: src = p->w3[older]; break;
-  case 6: src = p->pp[newest]; break;
-  case 7: src = p->pp[older]; break;
-  case 8: src = p->w21; break;
+  case 2: src = p->w1[newest]; ysrc = p->yw1[newest]; break;
+  case 3: src = p->w1[older]; ysrc = p->yw1[older]; break;
+  case 4: src = p->w3[newest]; ysrc = p->yw3[newest]; break;
+  case 5: src = p->w3[older]; ysrc = p->yw3[older]; break;
+  case 6: src = p->pp[newest]; ysrc = p->ypp[newest]; break;
+  case 7: src = p->pp[older]; ysrc = p->ypp[older]; break;
+  case 8: src = p->w21; ysrc = p->yw21; break;
+  }
+  if (ysrc && p->yw) {
+    // the y-face cells of a PML state field live in its compact array; the same cells of the
+    // full-grid field are otherwise unused, so the compact values are written there first
+    if (p->esz == 4)
+      hipLaunchKernelGGL(k_yexpand<float>, dim3(grid_blocks(p->ycells)), dim3(256), 0, p->stream,
+                         (const float *)ysrc, (float *)src, p->geo, p->yw, p->ypln, p->ycells);
+    else
+      hipLaunchKernelGGL(k_yexpand<unsigned short>, dim3(grid_blocks(p->ycells)), dim3(256), 0, p->stream,
+                         (const unsigned short *)ysrc, (unsigned short *)src, p->geo, p->yw, p->ypln, p->ycells);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(p->stream));
   }
   if (p->esz == 4) return copy_out(p, out, (const float *)src, (size_t)p->N);
   const bool dev = on_device(out);
@@ -1106,7 +1187,7 @@ extern "C" int qp3d_work_items(const qp3d_config *cfg, int which, int capacity, 
   if (which == 0) {
     qp3d_items_uniform(&p, p.tile_g, capacity, it);
   } else {
-    qp3d_items_update(&p, p.tile_u, p.tile_p, capacity, capacity, it, ip, &ipy, p.tile_py, capacity);
+    qp3d_items_update(&p, p.tile_u, p.tile_p, capacity, capacity, it, ip, ipy, p.tile_py, capacity);
   }
   const std::vector<int4> &v = which == 3 ? ipy : which == 2 ? ip : it;
   for (size_t i = 0; i < v.size() && (int)i < cap; i++) {

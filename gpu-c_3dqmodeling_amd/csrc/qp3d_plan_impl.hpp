@@ -53,15 +53,26 @@ struct qp3d_plan {
   long n_int_cells = 0, n_pml_cells = 0;
   hipStream_t stream = nullptr;
 
-  // u[2], w1[2], w3[2], p[2], w21: 9 fields of N cells of esz bytes (float, bfloat16 or half)
+  // State block: u[2], w1[2], w3[2], p[2], w21 (9 fields of N cells of esz bytes: float,
+  // bfloat16 or half), then with folded y faces (yw > 0) the compact y-face PML state yw1[2],
+  // yw3[2], ypp[2], yw21 (7 fields of ycells cells, YCompact layout); the full-grid fields'
+  // y-face cells are then unused.
   char *state = nullptr;
   size_t esz = 4;
-  size_t fstride = 0;      // bytes from one state field to the next (N*esz plus a pad)
-  size_t state_bytes = 0;  // 9 * fstride
+  size_t fstride = 0;      // bytes from one full-grid state field to the next (N*esz plus a pad)
+  size_t ystride = 0;      // bytes from one compact field to the next (0 without compact arrays)
+  size_t state_bytes = 0;  // 9 * fstride + 7 * ystride
   void *u[2]{}, *w1[2]{}, *w3[2]{}, *pp[2]{}, *w21 = nullptr;
+  void *yw1[2]{}, *yw3[2]{}, *ypp[2]{}, *yw21 = nullptr;
+  int yw = 0;           // cells per y face in the compact arrays (pml - 5), 0 = none
+  unsigned ypln = 0;    // compact cells per z plane, (nx - 2 pml) * 2 yw
+  long ycells = 0;      // nz * ypln
   int scale_log2 = 0;  // half storage: fields are stored times 2^scale_log2
-  float *model = nullptr;  // vdt2, eps, epsd
+  // model block: vdt2, eps, epsd over the grid, then their compact y-face copies
+  float *model = nullptr;
+  size_t model_bytes = 0;
   float *vdt2 = nullptr, *eps = nullptr, *epsd = nullptr;
+  float *yvdt2 = nullptr, *yeps = nullptr, *yepsd = nullptr;
   float4 *pmltab = nullptr;
   unsigned *gmax = nullptr;  // [nt]
   int *flag = nullptr;
@@ -152,6 +163,13 @@ inline void make_args(const qp3d_plan *p, int prev, Step st, GradArgs<S> &ga, Up
   ua.w3p = (T *)p->w3[prev]; ua.w3c = (const T *)p->w3[cur];
   ua.pp = (T *)p->pp[prev]; ua.pc = (const T *)p->pp[cur];
   ua.w21 = (T *)p->w21;
+  ua.yc.w1p = (T *)p->yw1[prev]; ua.yc.w1c = (const T *)p->yw1[cur];
+  ua.yc.w3p = (T *)p->yw3[prev]; ua.yc.w3c = (const T *)p->yw3[cur];
+  ua.yc.pp = (T *)p->ypp[prev]; ua.yc.pc = (const T *)p->ypp[cur];
+  ua.yc.w21 = (T *)p->yw21;
+  ua.yc.vdt2 = p->yvdt2; ua.yc.eps = p->yeps; ua.yc.epsd = p->yepsd;
+  ua.yc.pln = p->ypln;
+  ua.yc.yw = p->yw;
   ua.pmltab = p->pmltab;
   ua.gmax = p->gmax;
   ua.flag = p->flag;
@@ -168,17 +186,25 @@ inline void make_args(const qp3d_plan *p, int prev, Step st, GradArgs<S> &ga, Up
   ua.work = nullptr;
 }
 
-// Work lists (qp3d_plan.hip). Interior and PML update lists for the given tile shapes and
-// resident-workgroup capacities per CU; a uniform list over [5,n-5)^3 for a tile shape.
-int qp3d_build_update_lists(const qp3d_plan *p, int tile_i, int tile_p, int occ_i, int occ_p, WorkList &wi,
-                            WorkList &wp);
+// Sizes of the state and model blocks (esz and the geometry set): fstride, ystride,
+// state_bytes, model_bytes (qp3d_plan_describe calls it). qp3d_bind_state points the fields into
+// blocks of those sizes; qp3d_bind_ymodel only the compact model copies (3 fields of
+// field_stride(4 * ycells) bytes).
+void qp3d_state_layout(qp3d_plan *p);
+void qp3d_bind_state(qp3d_plan *p, char *state, float *model);
+void qp3d_bind_ymodel(qp3d_plan *p, float *ym);
+// Fills the compact y-face model copies from the full-grid model (after every model change).
+void qp3d_launch_ymodel(const qp3d_plan *p, hipStream_t s);
+
+// Work lists (qp3d_plan.hip): a uniform list over [5,n-5)^3 for a tile shape.
 int qp3d_build_uniform_list(const qp3d_plan *p, int tile, int occ, WorkList &w);
 // The same items in host memory (logical order) for `capacity` resident workgroups.
 void qp3d_items_uniform(const qp3d_plan *p, int tile, int capacity, std::vector<int4> &items);
-// ipy == nullptr: the y-face PML tiles join ip; otherwise they go to *ipy with tile shape
-// tile_py (their own launch, filling cap_py workgroups).
+// Interior update items (tile_i, cap_i workgroups) into it, PML items of the x and z faces
+// (tile_p, cap_p) into ip, of the y faces (tile_py, cap_py: a launch of their own, folded tiles
+// when y_fold(pml)) into ipy.
 void qp3d_items_update(const qp3d_plan *p, int tile_i, int tile_p, int cap_i, int cap_p, std::vector<int4> &it,
-                       std::vector<int4> &ip, std::vector<int4> *ipy, int tile_py, int cap_py);
+                       std::vector<int4> &ip, std::vector<int4> &ipy, int tile_py, int cap_py);
 // Uploads items in XCD-aware order (batch: items hold z0 | z1 << 16 and a grid index).
 int qp3d_upload_items(WorkList &w, const std::vector<int4> &logical, bool batch);
 int qp3d_cu_count(const qp3d_plan *p);

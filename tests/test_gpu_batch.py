@@ -72,6 +72,26 @@ def test_batch_equals_shots_alone(storage):
             p.close()
 
 
+def test_batch_mixes_compact_and_full_grid_y_faces():
+    """Shots whose y faces keep their PML state in compact arrays (pml <= 32, folded faces) and a
+    shot whose do not (pml 34) in one batch: the y-face stage launches both update kinds, and
+    every shot matches its plan run alone."""
+    model = qp3d.genmodel(70, 64, 30, 3)
+    cases = [((0, 0, 31, 27), 10), ((20, 9, 24, 20), 34), ((5, 30, 26, 23), 32)]
+    setups = [qp3d.window_setup(model, x0, y0, hx, hy, pml=pml, nt=30, fm=20.0, src_depth=2)
+              for (x0, y0, hx, hy), pml in cases]
+    plans = [qp3d.Plan(s) for s in setups]
+    try:
+        ref = alone(plans)
+        for graph in (False, True):
+            say(f"mixed pml batch graph={graph}")
+            with qp3d.Batch(plans, use_graph=graph) as b:
+                same(together(b, plans), ref)
+    finally:
+        for p in plans:
+            p.close()
+
+
 def test_batch_rejects_a_plan_uploaded_again():
     setups, plans = survey(nt=12)
     try:
