@@ -6,6 +6,7 @@ The misfit is J(m) = 1/2 sum (record(m) - observed)^2 over all steps and receive
 PML profiles held fixed. The finite differences are central differences of the misfit computed by
 the FP64 CPU oracle, with model steps of 1e-4 of max(vp) or 1e-4 in epsilon and delta."""
 import dataclasses
+import os
 import sys
 
 import numpy as np
@@ -139,6 +140,7 @@ def test_predicted_threshold_gradient():
 
 
 def test_autograd_shim_matches_direct_gradient():
+    """The shim with host tensors against the Fwi object's gradient."""
     import torch
 
     import qp3d.autograd
@@ -158,6 +160,22 @@ def test_autograd_shim_matches_direct_gradient():
     with qp3d.Plan(s) as p, qp3d.Fwi(p, interval=30) as f:
         r = f.forward().astype(np.float64)
         g = f.gradient((r - obs).astype(np.float32))
+    np.testing.assert_array_equal(rec.detach().numpy(), r.astype(np.float32))
     for t, ref in zip((vp, eps, de), g):
         a = t.grad.numpy().astype(np.float64)
         assert np.linalg.norm(a - ref) <= 1e-6 * np.linalg.norm(ref)
+
+
+def test_autograd_shim_with_gpu_tensors():
+    """GPU tensors through the shim: the model goes to the C ABI by device pointer and the
+    record and gradients come back as tensors on the GPU, identical to the host path. Runs in a
+    child process that imports torch before libqp3d.so loads, so that both use one HIP runtime
+    (this process loaded the library on its own runtime first)."""
+    import subprocess
+
+    child = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autograd_child.py")
+    say("child process: " + child)
+    r = subprocess.run([sys.executable, "-u", child], capture_output=True, text=True, timeout=240)
+    say(r.stdout)
+    say(r.stderr[-3000:])
+    assert r.returncode == 0
