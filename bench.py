@@ -318,6 +318,7 @@ def bench_batch(args):
                                   f"pml {args.pml}, {nt} steps, 20 Hz, storage {args.storage}",
                       "cells_updated_per_step": kb["cells_updated"], "autotuned_tiles": tune["tiles"],
                       "autotuned_per_shot_launches": tune["per_shot"],
+                      "autotuned_items_per_shot_share": tune["share"],
                       "autotune_ms_per_step": [round(x, 4) for x in tune["ms"]],
                       "batched_s": round(wall, 3), "sequential_s": round(seq, 3) if seq else None,
                       "speedup_vs_sequential": round(seq / wall, 3) if seq else None,

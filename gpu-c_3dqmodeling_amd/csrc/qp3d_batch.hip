@@ -343,20 +343,11 @@ extern "C" int qp3d_batch_sync(qp3d_batch *b)
   return QP3D_OK;
 }
 
-// Times every tile shape for each stage over `steps` time steps from rest, keeps the fastest,
-// then times whole steps with one launch per stage against each shot's own launches and keeps
-// the faster, and resets the batch. tiles_out (4 ints, may be NULL) receives the tile choice
-// (1-based, as in qp3d_config.tile_*: gradient max, interior, PML x/z faces, PML y faces);
-// ms_out (4 doubles, may be NULL) the chosen stages' mean ms per step.
-extern "C" int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, double *ms_out)
+// Times every tile shape of each stage over `steps` time steps from rest (events around that
+// stage's launch) and keeps the fastest; ms[s] receives the chosen shape's mean ms per step.
+static int tune_stages(qp3d_batch *b, int steps, int *tiles, double *ms)
 {
-  if (!b || steps < 1 || steps + 2 > b->nt) {
-    qp3d_set_error("batch_autotune: need 1 <= steps <= nt - 2");
-    return QP3D_ERR_ARG;
-  }
-  HIP_TRY(hipSetDevice(b->device));
   int rc;
-  b->per_shot = false;
   for (int s = 0; s < 4; s++) {
     double best = DBL_MAX;
     int bt = b->tile[s];
@@ -371,58 +362,86 @@ extern "C" int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, dou
       rc = qp3d_batch_run(b, 2, 2 + steps);
       b->timing = false;
       if (rc) return rc;
-      const double ms = b->timed_ms / steps;
-      if (ms < best) {
-        best = ms;
+      const double t_ms = b->timed_ms / steps;
+      if (t_ms < best) {
+        best = t_ms;
         bt = t;
       }
     }
     b->tile[s] = bt;
     if ((rc = build_stage(b, s))) return rc;
-    if (ms_out) ms_out[s] = best;
-    if (tiles_out) tiles_out[s] = bt + 1;
+    tiles[s] = bt;
+    ms[s] = best;
   }
-  // whole steps, direct launches: batched stages with items cut for the whole machine per
-  // shot, batched stages with items cut for each shot's share, each shot's own launches
+  return QP3D_OK;
+}
+
+// Device time (ms) of `steps` whole time steps from rest, direct launches.
+static int time_steps(qp3d_batch *b, int steps, double &ms_out)
+{
+  int rc;
+  if ((rc = qp3d_batch_reset(b)) || (rc = qp3d_batch_run(b, 0, 2))) return rc;
+  float ms = 0;
+  HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+  if ((rc = qp3d_batch_run(b, 2, 2 + steps))) return rc;
+  HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+  HIP_TRY(hipEventSynchronize(b->ev[1]));
+  HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+  ms_out = ms;
+  return QP3D_OK;
+}
+
+// For each way of cutting the shared launches' work items (each shot cut for the whole machine,
+// or for its share of it) times every tile shape per stage and then whole steps with the
+// fastest shapes; times whole steps of each shot's own launches one after another; keeps the
+// fastest of the three and resets the batch. tiles_out (4 ints, may be NULL) receives the tile
+// choice (1-based, as in qp3d_config.tile_*: gradient max, interior, PML x/z faces, PML y
+// faces) of the shared launches; ms_out (4 doubles, may be NULL) their stages' mean ms per step.
+extern "C" int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, double *ms_out)
+{
+  if (!b || steps < 1 || steps + 2 > b->nt) {
+    qp3d_set_error("batch_autotune: need 1 <= steps <= nt - 2");
+    return QP3D_ERR_ARG;
+  }
+  HIP_TRY(hipSetDevice(b->device));
   const bool graph = b->use_graph;
   b->use_graph = false;
-  double t[3] = {0, 0, 0};
-  for (int mode = 0; mode < 3 && rc == QP3D_OK; mode++) {
-    b->per_shot = mode == 2;
-    if (mode < 2 && b->share != (mode == 1)) {
-      b->share = mode == 1;
-      for (int s = 0; s < 4 && rc == QP3D_OK; s++) rc = build_stage(b, s);
-      if (rc) break;
-    }
-    if ((rc = qp3d_batch_reset(b)) || (rc = qp3d_batch_run(b, 0, 2))) break;
-    float ms = 0;
-    if (hipEventRecord(b->ev[0], b->stream) != hipSuccess) rc = QP3D_ERR_HIP;
-    if (!rc) rc = qp3d_batch_run(b, 2, 2 + steps);
-    if (!rc && (hipEventRecord(b->ev[1], b->stream) != hipSuccess || hipEventSynchronize(b->ev[1]) != hipSuccess ||
-                hipEventElapsedTime(&ms, b->ev[0], b->ev[1]) != hipSuccess))
-      rc = QP3D_ERR_HIP;
-    t[mode] = ms;
+  int tiles[2][4] = {}, rc = QP3D_OK;
+  double ms[2][4] = {}, t[3] = {0, 0, 0};
+  b->per_shot = false;
+  for (int sh = 0; sh < 2 && rc == QP3D_OK; sh++) {
+    b->share = sh == 1;
+    if ((rc = tune_stages(b, steps, tiles[sh], ms[sh]))) break;
+    rc = time_steps(b, steps, t[sh]);
+  }
+  if (rc == QP3D_OK) {
+    b->per_shot = true;
+    rc = time_steps(b, steps, t[2]);
+    b->per_shot = false;
   }
   b->use_graph = graph;
-  if (rc) {
-    b->per_shot = false;
-    if (rc == QP3D_ERR_HIP) qp3d_set_error("batch_autotune: timing the step modes failed");
-    return rc;
-  }
+  if (rc) return rc;
   const int best = t[2] < std::min(t[0], t[1]) ? 2 : (t[1] < t[0] ? 1 : 0);
+  const int sh = best == 1 ? 1 : 0;  // the shared launches' form kept (also behind per-shot mode)
   b->per_shot = best == 2;
-  if (best < 2 && b->share != (best == 1)) {
-    b->share = best == 1;
-    for (int s = 0; s < 4; s++)
-      if ((rc = build_stage(b, s))) return rc;
+  b->share = sh == 1;
+  for (int s = 0; s < 4; s++) {
+    b->tile[s] = tiles[sh][s];
+    if ((rc = build_stage(b, s))) return rc;
+    if (tiles_out) tiles_out[s] = tiles[sh][s] + 1;
+    if (ms_out) ms_out[s] = ms[sh][s];
   }
-  // a graph captured before holds the other mode's launches
+  // a graph captured before holds another mode's launches
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->gexec = nullptr;
   b->graph = nullptr;
   return qp3d_batch_reset(b);
 }
+
+// How the shared launches cut each shot's work items: 1 for the shot's share of the machine,
+// 0 for the whole machine.
+extern "C" int qp3d_batch_share(const qp3d_batch *b) { return b ? (b->share ? 1 : 0) : QP3D_ERR_ARG; }
 
 extern "C" int qp3d_batch_per_shot(const qp3d_batch *b) { return b ? (b->per_shot ? 1 : 0) : QP3D_ERR_ARG; }
 

@@ -311,9 +311,10 @@ extern "C" int qp3d_fwi_gradient(qp3d_fwi *f, const float *resid, double *g_vp, 
   hipLaunchKernelGGL(k_grad_finish, dim3(grid_of((long)N)), dim3(256), 0, p->stream, f->gsum,
                      (const float *)p->vdt2, c.dt, (long)N);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(g_vp, f->gsum, sizeof(double) * N, hipMemcpyDefault, p->stream));
-  HIP_TRY(hipMemcpyAsync(g_eps, f->gsum + N, sizeof(double) * N, hipMemcpyDefault, p->stream));
-  HIP_TRY(hipMemcpyAsync(g_del, f->gsum + 2 * N, sizeof(double) * N, hipMemcpyDefault, p->stream));
+  // [nz][nx][ny] outputs from the row-pitched sums
+  double *outs[3] = {g_vp, g_eps, g_del};
+  for (int i = 0; i < 3; i++)
+    if ((rc = qp3d_copy_field(p, outs[i], f->gsum + i * N, sizeof(double), false, p->stream))) return rc;
   HIP_TRY(hipStreamSynchronize(p->stream));
   f->ms_recompute = ms_re;
   f->ms_adjoint = ms_adj;

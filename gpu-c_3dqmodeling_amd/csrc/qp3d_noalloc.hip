@@ -149,9 +149,11 @@ extern "C" int qp3d_modeling3d_noalloc(int nx, int ny, int nz, int nt, int ntsna
   const size_t N = (size_t)P.N;
   const size_t fs = field_stride(sizeof(float) * N);
   Carve cv{base};
-  // state block (the plan's layout, FP32), (vp dt)^2 and eps - del, the compact y-face model
+  // state block (the plan's layout, FP32), the model in the plan's row-pitched layout
+  // ((vp dt)^2, eps, eps - del), the compact y-face model
   char *state = (char *)cv.take(P.state_bytes);
   P.vdt2 = (float *)cv.take(fs);
+  P.eps = (float *)cv.take(fs);
   P.epsd = (float *)cv.take(fs);
   float *ymodel = P.yw ? (float *)cv.take(3 * field_stride(sizeof(float) * (size_t)P.ycells)) : nullptr;
   P.pmltab = (float4 *)cv.take(sizeof(float4) * 3 * (size_t)pml);
@@ -167,7 +169,7 @@ extern "C" int qp3d_modeling3d_noalloc(int nx, int ny, int nz, int nt, int ntsna
   }
   if (!cv.fits) {
     qp3d_set_error("modeling3d_noalloc: a %dx%dx%d grid with %d steps needs more than the %zu-byte scratch "
-                   "array (about 44 bytes per cell); use qp3d_modeling3d",
+                   "array (about 48 bytes per cell); use qp3d_modeling3d",
                    nx, ny, nz, nt, (size_t)QP3D_NOALLOC_SCRATCH_BYTES);
     return QP3D_ERR_ARG;
   }
@@ -175,13 +177,16 @@ extern "C" int qp3d_modeling3d_noalloc(int nx, int ny, int nz, int nt, int ntsna
   P.n_pml_cells = P.wl_pml.cells + P.wl_pmly.cells;
   qp3d_bind_state(&P, state, nullptr);
   if (P.yw) qp3d_bind_ymodel(&P, ymodel);
-  P.eps = const_cast<float *>(epsilon);  // read only
   P.wav = const_cast<float *>(wavelet);  // read only
   P.rec = record;
 
-  // model: (vp dt)^2 and eps - del in the scratch array, eps read where it is
-  HIP_TRY(hipMemcpyAsync(P.vdt2, vp, sizeof(float) * N, hipMemcpyDeviceToDevice, s));
-  HIP_TRY(hipMemcpyAsync(P.epsd, delta, sizeof(float) * N, hipMemcpyDeviceToDevice, s));
+  // model in the scratch array (row padding zeroed; the copies from device arrays allocate
+  // nothing), then (vp dt)^2 and eps - del in place
+  HIP_TRY(hipMemsetAsync(P.vdt2, 0, 3 * fs, s));
+  if ((rc = qp3d_copy_field(&P, const_cast<float *>(vp), P.vdt2, sizeof(float), true, s)) ||
+      (rc = qp3d_copy_field(&P, const_cast<float *>(epsilon), P.eps, sizeof(float), true, s)) ||
+      (rc = qp3d_copy_field(&P, const_cast<float *>(delta), P.epsd, sizeof(float), true, s)))
+    return rc;
   qp3d_launch_model(P.vdt2, P.eps, P.epsd, dt, (long)N, s);
   HIP_TRY(hipGetLastError());
   qp3d_launch_ymodel(&P, s);

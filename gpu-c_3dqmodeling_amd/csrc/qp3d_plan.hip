@@ -294,9 +294,10 @@ int qp3d_check_config(const qp3d_config &c, const char *who)
     qp3d_set_error("%s: unknown storage mode %d", who, c.storage);
     return QP3D_ERR_ARG;
   }
-  const double cells = (double)c.nx * c.ny * c.nz;
+  const double cells = (double)c.nx * row_pitch(c.ny) * c.nz;  // rows padded to 32 cells
   if (cells >= 1073741824.0 || c.nx > 32767 || c.ny > 32767) {
-    qp3d_set_error("%s: grid of %.4g cells exceeds the 2^30-cell limit of one plan", who, cells);
+    qp3d_set_error("%s: grid of %.4g cells (rows padded to 32 cells) exceeds the 2^30-cell limit of one plan", who,
+                   cells);
     return QP3D_ERR_ARG;
   }
   return QP3D_OK;
@@ -310,9 +311,11 @@ void qp3d_plan_describe(qp3d_plan *p, const qp3d_config &c)
   p->hx = c.nx - 2 * c.pml;
   p->hy = c.ny - 2 * c.pml;
   p->hz = c.nz - 2 * c.pml;
-  p->N = (long)c.nx * c.ny * c.nz;
   p->geo.nx = c.nx; p->geo.ny = c.ny; p->geo.nz = c.nz; p->geo.pml = c.pml;
-  p->geo.pln = (unsigned)c.nx * (unsigned)c.ny;
+  p->geo.ys = row_pitch(c.ny);
+  p->geo.pln = (unsigned)c.nx * (unsigned)p->geo.ys;
+  p->N = (long)c.nz * c.nx * p->geo.ys;
+  p->Nd = (long)c.nz * c.nx * c.ny;
   // defaults measured fastest on MI355X at 576^3: 8x2 interior, 4x4 gradient max (16x1 with
   // BF16 storage), 16x2 PML x and z faces, 16x1 PML y faces
   p->tile_u = c.tile_update > 0 ? c.tile_update - 1 : 0;
@@ -429,6 +432,10 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
   p->state = state;
   if ((rc = dmalloc((void **)&model, p->model_bytes, "model"))) return fail(rc);
   qp3d_bind_state(p, state, model);
+  if (hipMemset(model, 0, p->model_bytes) != hipSuccess) {  // row padding of the model stays zero
+    qp3d_set_error("plan_create: clearing the model failed");
+    return fail(QP3D_ERR_HIP);
+  }
   if ((rc = dmalloc((void **)&p->pmltab, sizeof(float4) * 3 * c.pml, "PML table"))) return fail(rc);
   if ((rc = dmalloc((void **)&p->gmax, sizeof(unsigned) * (size_t)(c.nt > 0 ? c.nt : 1), "step maxima")))
     return fail(rc);
@@ -457,7 +464,7 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
       p->snap_steps.push_back(k);
       if (c.snap_s) {
         float *ds = nullptr;
-        if ((rc = dmalloc((void **)&ds, sizeof(float) * (size_t)p->N, "S-field snapshot"))) return fail(rc);
+        if ((rc = dmalloc((void **)&ds, sizeof(float) * (size_t)p->Nd, "S-field snapshot"))) return fail(rc);
         p->snaps_s.push_back(ds);
       }
     }
@@ -504,7 +511,7 @@ extern "C" int qp3d_plan_destroy(qp3d_plan *p)
 }
 
 // Whether ptr is device memory (arrays handed to the plan may live on the host or the device).
-static bool on_device(const void *ptr)
+bool qp3d_on_device(const void *ptr)
 {
   hipPointerAttribute_t a;
   if (hipPointerGetAttributes(&a, ptr) != hipSuccess) {
@@ -512,6 +519,79 @@ static bool on_device(const void *ptr)
     return false;
   }
   return a.type == hipMemoryTypeDevice;
+}
+static bool on_device(const void *ptr) { return qp3d_on_device(ptr); }
+
+// rows of ny elements from a src with row stride spitch to a dst with row stride dpitch
+template <class E>
+static __global__ void k_rows(E *dst, const E *src, long rows, int ny, int dpitch, int spitch)
+{
+  const long n = rows * ny;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long r = i / ny;
+    const int y = (int)(i - r * ny);
+    dst[r * dpitch + y] = src[r * spitch + y];
+  }
+}
+
+template <class E>
+static void launch_rows(void *dst, const void *src, long rows, int ny, int dpitch, int spitch, hipStream_t s)
+{
+  const long n = rows * ny;
+  hipLaunchKernelGGL(k_rows<E>, dim3((unsigned)std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, s, (E *)dst,
+                     (const E *)src, rows, ny, dpitch, spitch);
+}
+
+int qp3d_copy_field(const qp3d_plan *p, void *dense, void *field, size_t esz, bool to_field, hipStream_t s)
+{
+  const int ny = p->cfg.ny, ys = p->geo.ys;
+  const long rows = (long)p->cfg.nx * p->cfg.nz;
+  const size_t nbytes = esz * (size_t)p->Nd;
+  const bool dev = on_device(dense);
+  if (ys == ny) {  // no row padding: one plain copy
+    if (to_field)
+      HIP_TRY(hipMemcpyAsync(field, dense, nbytes, hipMemcpyDefault, s));
+    else
+      HIP_TRY(hipMemcpyAsync(dense, field, nbytes, hipMemcpyDefault, s));
+    if (!dev) HIP_TRY(hipStreamSynchronize(s));
+    return QP3D_OK;
+  }
+  auto rows_copy = [&](void *dst, const void *src, int dp, int sp) {
+    if (esz == 2) launch_rows<unsigned short>(dst, src, rows, ny, dp, sp, s);
+    else if (esz == 8) launch_rows<double>(dst, src, rows, ny, dp, sp, s);
+    else launch_rows<float>(dst, src, rows, ny, dp, sp, s);
+  };
+  if (dev) {
+    if (to_field) rows_copy(field, dense, ys, ny);
+    else rows_copy(dense, field, ny, ys);
+    HIP_TRY(hipGetLastError());
+    return QP3D_OK;
+  }
+  // host memory: through a dense device copy
+  void *tmp = nullptr;
+  hipError_t e = hipMalloc(&tmp, nbytes);
+  if (e != hipSuccess) {
+    qp3d_set_error("copying a field: hipMalloc of %zu bytes failed: %s", nbytes, hipGetErrorString(e));
+    return QP3D_ERR_ALLOC;
+  }
+  if (to_field) {
+    e = hipMemcpyAsync(tmp, dense, nbytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+      rows_copy(field, tmp, ys, ny);
+      e = hipGetLastError();
+    }
+  } else {
+    rows_copy(tmp, field, ny, ys);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(dense, tmp, nbytes, hipMemcpyDeviceToHost, s);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) {
+    qp3d_set_error("copying a field failed: %s", hipGetErrorString(e));
+    return QP3D_ERR_HIP;
+  }
+  return QP3D_OK;
 }
 
 // In place over the three model arrays, holding vp, eps and del on entry:
@@ -552,7 +632,7 @@ static __device__ __forceinline__ long ycompact_to_full(long i, Geo g, int yw, u
   const int rem = (int)(i - (long)z * ypln);
   const int xr = rem / (2 * yw), col = rem - xr * (2 * yw);
   const int y = col < yw ? H + col : g.ny - g.pml + (col - yw);
-  return (long)z * g.pln + (long)(xr + g.pml) * g.ny + y;
+  return (long)z * g.pln + (long)(xr + g.pml) * g.ys + y;
 }
 
 // Compact y-face copies of the three model arrays.
@@ -671,18 +751,20 @@ extern "C" int qp3d_plan_upload(qp3d_plan *p, const float *vp, const float *eps,
   HIP_TRY(hipStreamSynchronize(p->stream));
   const long N = p->N;
   // Every array may be host or device memory. Model: vp, eps, del copied into the plan's model
-  // arrays, then (vp*dt)^2, epsilon, epsilon-delta formed in place.
+  // arrays (rows padded to the pitch; the padding stays zero), then (vp*dt)^2, epsilon,
+  // epsilon-delta formed in place.
   // (copies on the plan's stream: a device-to-device hipMemcpy may return before it lands)
-  HIP_TRY(hipMemcpyAsync(p->vdt2, vp, sizeof(float) * (size_t)N, hipMemcpyDefault, p->stream));
-  HIP_TRY(hipMemcpyAsync(p->eps, eps, sizeof(float) * (size_t)N, hipMemcpyDefault, p->stream));
-  HIP_TRY(hipMemcpyAsync(p->epsd, del, sizeof(float) * (size_t)N, hipMemcpyDefault, p->stream));
+  int rc;
+  if ((rc = qp3d_copy_field(p, const_cast<float *>(vp), p->vdt2, sizeof(float), true, p->stream)) ||
+      (rc = qp3d_copy_field(p, const_cast<float *>(eps), p->eps, sizeof(float), true, p->stream)) ||
+      (rc = qp3d_copy_field(p, const_cast<float *>(del), p->epsd, sizeof(float), true, p->stream)))
+    return rc;
   qp3d_launch_model(p->vdt2, p->eps, p->epsd, cf.dt, N, p->stream);
   HIP_TRY(hipGetLastError());
   qp3d_launch_ymodel(p, p->stream);
   HIP_TRY(hipGetLastError());
   std::vector<float4> tab;
-  int rc = qp3d_make_tables(p, dlr, ddlr, dtb, ddtb, dfb, ddfb, c, c2, tab);
-  if (rc) return rc;
+  if ((rc = qp3d_make_tables(p, dlr, ddlr, dtb, ddtb, dfb, ddfb, c, c2, tab))) return rc;
   HIP_TRY(hipMemcpyAsync(p->pmltab, tab.data(), sizeof(float4) * tab.size(), hipMemcpyHostToDevice, p->stream));
   std::vector<float> hw;
   if ((rc = to_host(wavelet, (size_t)cf.nt, hw))) return rc;
@@ -900,7 +982,7 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on
       HIP_TRY(hipGetLastError());
       if (si < (int)p->snaps_s.size()) {
         // S of this step: from u(k) (slot cur, which this step does not write) and G(k)
-        const int blocks = (int)std::min<long>((p->N + 255) / 256, 8192);
+        const int blocks = (int)std::min<long>((p->Nd + 255) / 256, 8192);
         hipLaunchKernelGGL(k_sfield<S>, dim3(blocks), dim3(256), 0, s0, (const T *)p->u[cur],
                            (const float *)p->eps, (const float *)p->epsd, (const unsigned *)p->gmax, st, p->geo,
                            p->coef, p->snaps_s[si]);
@@ -1114,28 +1196,44 @@ extern "C" int qp3d_plan_field(const qp3d_plan *p, int which, float *out)
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(p->stream));
   }
-  if (p->esz == 4) return copy_out(p, out, (const float *)src, (size_t)p->N);
+  const size_t nd = (size_t)p->Nd;
+  int rc;
+  if (p->esz == 4) {
+    // dense copy on the plan's stream, then the FP16 scale taken out where the copy landed
+    if ((rc = qp3d_copy_field(p, out, src, sizeof(float), false, p->stream))) return rc;
+    if (!on_device(out)) {
+      unscale(p, out, nd);
+      return QP3D_OK;
+    }
+    if (p->scale_log2) {
+      hipLaunchKernelGGL(k_scale, dim3((unsigned)std::min<size_t>((nd + 255) / 256, 8192)), dim3(256), 0, p->stream,
+                         out, (long)nd, std::ldexp(1.0f, -p->scale_log2));
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return QP3D_OK;
+  }
   const bool dev = on_device(out);
-  std::vector<float> hout(dev ? (size_t)p->N : 0);
+  std::vector<float> hout(dev ? nd : 0);
   float *const dst = out;
   if (dev) out = hout.data();
   {
-    std::vector<unsigned short> h((size_t)p->N);
-    HIP_TRY(hipMemcpy(h.data(), src, sizeof(unsigned short) * (size_t)p->N, hipMemcpyDeviceToHost));
+    std::vector<unsigned short> h(nd);
+    if ((rc = qp3d_copy_field(p, h.data(), src, sizeof(unsigned short), false, p->stream))) return rc;
     if (p->cfg.storage == QP3D_STORAGE_BF16)
-      for (long i = 0; i < p->N; i++) {
+      for (size_t i = 0; i < nd; i++) {
         const unsigned bits = (unsigned)h[i] << 16;
         memcpy(&out[i], &bits, 4);
       }
     else
-      for (long i = 0; i < p->N; i++) {
+      for (size_t i = 0; i < nd; i++) {
         _Float16 v;
         memcpy(&v, &h[i], 2);
         out[i] = (float)v;
       }
   }
-  unscale(p, out, (size_t)p->N);
-  if (dev) HIP_TRY(hipMemcpy(dst, out, sizeof(float) * (size_t)p->N, hipMemcpyHostToDevice));
+  unscale(p, out, nd);
+  if (dev) HIP_TRY(hipMemcpy(dst, out, sizeof(float) * nd, hipMemcpyHostToDevice));
   return QP3D_OK;
 }
 
@@ -1163,7 +1261,7 @@ extern "C" int qp3d_plan_snapshot_s(const qp3d_plan *p, int i, int *step, float 
   HIP_TRY(hipSetDevice(p->cfg.device));
   HIP_TRY(hipStreamSynchronize(p->stream));
   if (step) *step = p->snap_steps[i];
-  HIP_TRY(hipMemcpyAsync(out, p->snaps_s[i], sizeof(float) * (size_t)p->N, hipMemcpyDefault, p->stream));
+  HIP_TRY(hipMemcpyAsync(out, p->snaps_s[i], sizeof(float) * (size_t)p->Nd, hipMemcpyDefault, p->stream));
   HIP_TRY(hipStreamSynchronize(p->stream));
   return QP3D_OK;
 }

@@ -40,7 +40,8 @@ using qp3d_impl::EV_PER_STEP;
 struct qp3d_plan {
   qp3d_config cfg{};
   int hx = 0, hy = 0, hz = 0;
-  long N = 0;
+  long N = 0;   // cells of a device field: nz * nx * ys (rows padded to geo.ys)
+  long Nd = 0;  // cells of a dense [nz][nx][ny] array (the caller's layout)
   Geo geo{};
   Coef coef{};
   int tile_u = 0, tile_g = 0, tile_p = 0;
@@ -195,6 +196,14 @@ void qp3d_bind_state(qp3d_plan *p, char *state, float *model);
 void qp3d_bind_ymodel(qp3d_plan *p, float *ym);
 // Fills the compact y-face model copies from the full-grid model (after every model change).
 void qp3d_launch_ymodel(const qp3d_plan *p, hipStream_t s);
+// Copies between a dense [nz][nx][ny] array of esz-byte elements (2, 4 or 8; host or device
+// memory) and a device field in the plan's row-pitched layout, on stream s; to_field: dense ->
+// field. Complete on return when the dense array is host memory; device to device copies are
+// only queued. Needs no allocation when the dense array is device memory or the rows are not
+// padded.
+int qp3d_copy_field(const qp3d_plan *p, void *dense, void *field, size_t esz, bool to_field, hipStream_t s);
+// dense array in device memory of the current device
+bool qp3d_on_device(const void *ptr);
 
 // Work lists (qp3d_plan.hip): a uniform list over [5,n-5)^3 for a tile shape.
 int qp3d_build_uniform_list(const qp3d_plan *p, int tile, int occ, WorkList &w);

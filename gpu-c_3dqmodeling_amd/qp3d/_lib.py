@@ -92,6 +92,7 @@ _SIGS = [
     ("qp3d_batch_autotune", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     ("qp3d_batch_step_bytes", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     ("qp3d_batch_per_shot", C.c_int, [C.c_void_p]),
+    ("qp3d_batch_share", C.c_int, [C.c_void_p]),
     ("qp3d_batch_set_per_shot", C.c_int, [C.c_void_p, C.c_int]),
 ]
 

@@ -243,7 +243,8 @@ class Batch:
         tiles = (C.c_int * 4)()
         ms = (C.c_double * 4)()
         check(self.lib.qp3d_batch_autotune(self.h, int(steps), tiles, ms))
-        return {"tiles": list(tiles), "ms": list(ms), "per_shot": bool(self.lib.qp3d_batch_per_shot(self.h))}
+        return {"tiles": list(tiles), "ms": list(ms), "per_shot": bool(self.lib.qp3d_batch_per_shot(self.h)),
+                "share": bool(self.lib.qp3d_batch_share(self.h))}
 
     def set_per_shot(self, on=True):
         """Each shot's own launches within a step (True) or one launch per stage (False)."""

@@ -69,10 +69,11 @@ int qp3d_modeling3d(int nx, int ny, int nz, int nt, int ntsnap, float dx, float 
  * It calls no hipMalloc/hipFree: the wavefields, PML state, converted model, PML table,
  * per-step maxima, flag, source box and work lists live in one fixed device array of
  * QP3D_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until the process ends
- * (about 44 bytes per grid cell: a 141^3 grid fits); larger grids are rejected with
- * QP3D_ERR_ARG. Every call on a device shares that device's array: calls that overlap on one
- * GPU (other threads, other streams) race and must be serialised by the caller. */
-#define QP3D_NOALLOC_SCRATCH_BYTES (160ull << 20)
+ * (about 48 bytes per grid cell with rows padded to 32 cells, plus 40 bytes per y-face PML
+ * cell: a 141^3 grid fits); larger grids are rejected with QP3D_ERR_ARG. Every call on a device
+ * shares that device's array: calls that overlap on one GPU (other threads, other streams) race
+ * and must be serialised by the caller. */
+#define QP3D_NOALLOC_SCRATCH_BYTES (192ull << 20)
 int qp3d_modeling3d_noalloc(int nx, int ny, int nz, int nt, int ntsnap, float dx, float dy, float dz, float dt,
                             int pml, int snapflag, int sx, int sy, int sz, const float *vp, const float *epsilon,
                             const float *delta, const float *source, const float *wavelet, float *record,
@@ -245,12 +246,16 @@ int qp3d_batch_destroy(qp3d_batch *b);
 int qp3d_batch_reset(qp3d_batch *b);
 int qp3d_batch_run(qp3d_batch *b, int k0, int k1);
 int qp3d_batch_sync(qp3d_batch *b);
-/* Picks each stage's tile shape by timing every candidate over `steps` steps, then how a step
- * goes out: one launch per stage for all shots with each shot's work items cut for the whole
- * machine (as its own plan cuts them) or for its share of the machine, or each shot's own
- * launches one after another; then resets;
- * tiles_out[4] / ms_out[4] (may be NULL): gradient max, interior, PML x/z faces, PML y faces. */
+/* Picks how a step goes out, timing `steps` steps of each candidate: one launch per stage for
+ * all shots with each shot's work items cut for the whole machine (as its own plan cuts them)
+ * or for its share of the machine, each with every stage's tile shape timed and the fastest
+ * kept, or each shot's own launches one after another; then resets;
+ * tiles_out[4] / ms_out[4] (may be NULL): the shared launches' tiles and stage times: gradient
+ * max, interior, PML x/z faces, PML y faces. */
 int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, double *ms_out);
+/* 1 when the shared launches cut each shot's work items for its share of the machine, 0 for
+ * the whole machine (the default; chosen by qp3d_batch_autotune). */
+int qp3d_batch_share(const qp3d_batch *b);
 /* 1 when the batch runs each shot's own launches (chosen by qp3d_batch_autotune), 0 when it
  * runs one launch per stage for all shots (the default); the setter forces either. */
 int qp3d_batch_per_shot(const qp3d_batch *b);
