@@ -230,15 +230,18 @@ def test_graph_replay_matches_direct_launches():
 
 # Measured on MI355X for this case (41x45x37 interior, pml 12, 200 steps) against the FP64
 # oracle: FP32 1.3e-6, FP16 3.0e-3, BF16 2.5e-2 relative L2 on the gather. The bounds below
-# are those figures with a 2x margin; they pin the storage modes, not a precision target.
+# are those figures with a 2x margin: a characterisation that catches a change of the storage
+# modes' behaviour, not a parity claim. The precision target (1e-3 against FP64) is
+# tests/test_gpu_configs.py::test_c3_16bit_storage_meets_1e3_against_fp64, an expected failure.
 LP_TOL = {"fp16": 6e-3, "bf16": 5e-2}
 
 
 @pytest.mark.parametrize("storage", ["fp16", "bf16"])
-def test_low_precision_storage_against_fp64(storage):
-    """Wavefields and PML state stored in 16 bits, FP32 arithmetic: gather error against the
-    FP64 oracle, the values held are exactly representable in the storage type, and the FP16
-    scale is taken out of everything read back (record, state, snapshots, maxima)."""
+def test_low_precision_storage_characterisation(storage):
+    """Wavefields and PML state stored in 16 bits, FP32 arithmetic (characterisation, not
+    parity): the gather error against the FP64 oracle stays at its measured size, the values
+    held are exactly representable in the storage type, and the FP16 scale is taken out of
+    everything read back (record, state, snapshots, maxima)."""
     s = qp3d.config_setup(hx=41, hy=45, nz0=37, pml=12, nt=200, seed=3, src_depth=3)
     say(f"storage {storage}")
     o64 = O.modeling3d(s, "f64")
