@@ -3,6 +3,8 @@
 // step whose work items carry the shot's index. Each shot keeps its own plan (grid, model, source, PML profiles,
 // wavefields, record); all shots advance in lockstep. The step sequence can be captured once
 // in a hipGraph, and an autotuner picks each stage's tile shape by timing the candidates.
+// Persistent mode: every step of a run of every shot in one launch (k_persist, qp3d_persist.hpp),
+// the shots advancing independently through per-shot item counters.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -10,6 +12,7 @@
 #include <vector>
 
 #include "qp3d_launch.hpp"
+#include "qp3d_persist.hpp"
 #include "qp3d_plan_impl.hpp"
 
 struct qp3d_batch {
@@ -42,6 +45,17 @@ struct qp3d_batch {
   bool timing = false;  // autotuner: events around stage `timed_stage`
   int timed_stage = 0;
   double timed_ms = 0;
+  // persistent mode: qp3d_batch_run is one k_persist launch over its steps
+  bool persist = false;
+  int pset = 0;                  // tile set (PSET_TILES)
+  int4 *pitems = nullptr;        // one step's items in queue order
+  int pips = 0;                  // items per step
+  unsigned *pnitems = nullptr;   // [2][nshots] items per shot and step: gradient, update
+  unsigned *pctr = nullptr;      // queue, error word, [2][nshots] items done
+  int pblocks = 0;               // resident workgroups launched
+  bool pshare = false;           // items cut for each shot's share of the machine
+  bool pbuilt = false;
+  double wait_us = 5e6;          // bound of one dependency wait
 };
 
 // Work list of one stage over all shots, items in batch format (stage 3 fills wl[3] and wl[4]).
@@ -95,6 +109,102 @@ static int upload_args(qp3d_batch *b)
     HIP_TRY(hipMemcpy(b->args_g[par], g.data(), sizeof(GradArgs<S>) * n, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b->args_u[par], u.data(), sizeof(UpdArgs<S>) * n, hipMemcpyHostToDevice));
   }
+  return QP3D_OK;
+}
+
+// The persistent queue of one slot (see qp3d_persist.hpp): the gradient tiles of the even shots,
+// the update tiles (interior, x/z faces, y faces, rim source) of the odd shots for the step
+// before the slot, the update tiles of the even shots, the gradient tiles of the odd shots; with
+// the set's tile shapes and each shot's items cut for the whole machine or for its share of it
+// (b->pshare).
+static int build_persist(qp3d_batch *b)
+{
+  int occ = 1;
+  with_storage(b->storage, [&](auto sp) {
+    using S = decltype(sp);
+    switch (b->pset) {
+      case 0: occ = occupancy_persist<S, 0>(); break;
+      case 1: occ = occupancy_persist<S, 1>(); break;
+      default: occ = occupancy_persist<S, 2>(); break;
+    }
+  });
+  const int n = (int)b->plans.size();
+  const int ncu = qp3d_cu_count(b->plans[0]);
+  const int *t = PSET_TILES[b->pset];
+  const int cap = std::max(1, occ * ncu / (b->pshare ? n : 1));
+  std::vector<int4> grad[2], upd[2];  // by group (shot parity)
+  std::vector<unsigned> nitems(2 * (size_t)n, 0u);
+  auto put = [](std::vector<int4> &dst, const std::vector<int4> &src, int shot, int kind, int prev) {
+    unsigned c = 0;
+    for (const int4 &w : src) {
+      if (w.z >= w.w) continue;
+      dst.push_back(make_int4(w.x, w.y, w.z | (w.w << 16), shot | (kind << 16) | (prev << 24)));
+      c++;
+    }
+    return c;
+  };
+  for (int i = 0; i < n; i++) {
+    const qp3d_plan *p = b->plans[i];
+    const int g = i & 1;  // group B (odd shots) runs its update one slot late
+    std::vector<int4> ig, ii, ip, iy;
+    qp3d_items_uniform(p, t[0], cap, ig);
+    qp3d_items_update(p, t[1], t[2], cap, cap, ii, ip, iy, t[3], cap);
+    nitems[i] = put(grad[g], ig, i, PK_GRAD, 0);
+    unsigned c = put(upd[g], ii, i, PK_INT, g);
+    c += put(upd[g], ip, i, PK_PML, g);
+    c += put(upd[g], iy, i, p->yw ? PK_PMLY : PK_PMLYF, g);
+    if (p->src_rim) {
+      upd[g].push_back(make_int4(0, 0, 0, i | (PK_RIM << 16) | (g << 24)));
+      c++;
+    }
+    nitems[n + i] = c;
+  }
+  std::vector<int4> q;
+  for (const std::vector<int4> *v : {&grad[0], &upd[1], &upd[0], &grad[1]}) q.insert(q.end(), v->begin(), v->end());
+  (void)hipFree(b->pitems);
+  b->pitems = nullptr;
+  b->pips = (int)q.size();
+  HIP_TRY(hipMalloc((void **)&b->pitems, sizeof(int4) * q.size()));
+  HIP_TRY(hipMemcpy(b->pitems, q.data(), sizeof(int4) * q.size(), hipMemcpyHostToDevice));
+  if (!b->pnitems) HIP_TRY(hipMalloc((void **)&b->pnitems, sizeof(unsigned) * 2 * n));
+  HIP_TRY(hipMemcpy(b->pnitems, nitems.data(), sizeof(unsigned) * 2 * n, hipMemcpyHostToDevice));
+  if (!b->pctr) {
+    HIP_TRY(hipMalloc((void **)&b->pctr, sizeof(unsigned) * (2 + 2 * (size_t)n)));
+    HIP_TRY(hipMemset(b->pctr, 0, sizeof(unsigned) * (2 + 2 * (size_t)n)));
+  }
+  b->pblocks = occ * ncu;
+  b->pbuilt = true;
+  return QP3D_OK;
+}
+
+// Steps [k0, k1) of every shot in one persistent launch (after the counters are zeroed).
+template <class S>
+static int persist_run_s(qp3d_batch *b, int k0, int k1)
+{
+  const int n = (int)b->plans.size();
+  hipLaunchKernelGGL(k_persist_init, dim3(1), dim3(256), 0, b->stream, b->pctr, 2 + 2 * n);
+  HIP_TRY(hipGetLastError());
+  int khz = 100000;  // wall_clock64 rate
+  (void)hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device);
+  PersistArgs<S> a;
+  for (int par = 0; par < 2; par++) {
+    a.ag[par] = (const GradArgs<S> *)b->args_g[par];
+    a.au[par] = (const UpdArgs<S> *)b->args_u[par];
+  }
+  a.items = b->pitems;
+  a.ips = b->pips;
+  a.nshots = n;
+  a.k0 = k0;
+  a.k1 = k1;
+  a.nitems = b->pnitems;
+  a.ctr = b->pctr;
+  a.wait_ticks = (long long)(b->wait_us * 1e-3 * khz);
+  switch (b->pset) {
+    case 0: launch_persist<S, 0>(b->pblocks, b->stream, a); break;
+    case 1: launch_persist<S, 1>(b->pblocks, b->stream, a); break;
+    default: launch_persist<S, 2>(b->pblocks, b->stream, a); break;
+  }
+  HIP_TRY(hipGetLastError());
   return QP3D_OK;
 }
 
@@ -171,6 +281,9 @@ static void batch_free(qp3d_batch *b)
     (void)hipFree(b->args_u[i]);
   }
   (void)hipFree(b->kdev);
+  (void)hipFree(b->pitems);
+  (void)hipFree(b->pnitems);
+  (void)hipFree(b->pctr);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   for (hipEvent_t e : b->ev)
@@ -304,6 +417,20 @@ extern "C" int qp3d_batch_run(qp3d_batch *b, int k0, int k1)
       return QP3D_ERR_STATE;
     }
   HIP_TRY(hipSetDevice(b->device));
+  if (b->persist && !b->per_shot) {
+    if (!b->pbuilt) {
+      int rc = build_persist(b);
+      if (rc) return rc;
+    }
+    if (k1 > k0) {
+      int rc = QP3D_OK;
+      with_storage(b->storage, [&](auto sp) { rc = persist_run_s<decltype(sp)>(b, k0, k1); });
+      if (rc) return rc;
+    }
+    b->steps_done = k1;
+    for (qp3d_plan *p : b->plans) p->steps_done = k1;
+    return QP3D_OK;
+  }
   const bool graph = b->use_graph && !b->timing;
   for (int k = k0; k < k1;) {
     if (graph && (k & 1) == 0 && k + GRAPH_LEN <= k1) {
@@ -332,6 +459,16 @@ extern "C" int qp3d_batch_sync(qp3d_batch *b)
   if (!b) return QP3D_ERR_ARG;
   HIP_TRY(hipSetDevice(b->device));
   HIP_TRY(hipStreamSynchronize(b->stream));
+  if (b->pctr) {
+    unsigned err = 0;
+    HIP_TRY(hipMemcpy(&err, b->pctr + 1, sizeof(unsigned), hipMemcpyDeviceToHost));
+    if (err) {
+      qp3d_set_error("batch: a wait of the persistent launch passed its bound (%.0f us); the shots' state is "
+                     "incomplete",
+                     b->wait_us);
+      return QP3D_ERR_TIMEOUT;
+    }
+  }
   for (size_t i = 0; i < b->plans.size(); i++) {
     int flag = 0;
     HIP_TRY(hipMemcpy(&flag, b->plans[i]->flag, sizeof(int), hipMemcpyDeviceToHost));
@@ -419,6 +556,17 @@ extern "C" int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, dou
     rc = time_steps(b, steps, t[2]);
     b->per_shot = false;
   }
+  // one persistent launch: every tile set with each cut
+  double tp[NPSETS][2] = {};
+  for (int set = 0; set < NPSETS && rc == QP3D_OK; set++)
+    for (int sh = 0; sh < 2 && rc == QP3D_OK; sh++) {
+      b->persist = true;
+      b->pset = set;
+      b->pshare = sh == 1;
+      b->pbuilt = false;
+      rc = time_steps(b, steps, tp[set][sh]);
+      b->persist = false;
+    }
   b->use_graph = graph;
   if (rc) return rc;
   const int best = t[2] < std::min(t[0], t[1]) ? 2 : (t[1] < t[0] ? 1 : 0);
@@ -430,6 +578,20 @@ extern "C" int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, dou
     if ((rc = build_stage(b, s))) return rc;
     if (tiles_out) tiles_out[s] = tiles[sh][s] + 1;
     if (ms_out) ms_out[s] = ms[sh][s];
+  }
+  int pbest = 0, pbsh = 0;
+  for (int set = 0; set < NPSETS; set++)
+    for (int h = 0; h < 2; h++)
+      if (tp[set][h] < tp[pbest][pbsh]) {
+        pbest = set;
+        pbsh = h;
+      }
+  b->pset = pbest;
+  b->pshare = pbsh == 1;
+  b->pbuilt = false;
+  if (tp[pbest][pbsh] < t[best]) {
+    b->persist = true;
+    b->per_shot = false;
   }
   // a graph captured before holds another mode's launches
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
@@ -444,6 +606,41 @@ extern "C" int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, dou
 extern "C" int qp3d_batch_share(const qp3d_batch *b) { return b ? (b->share ? 1 : 0) : QP3D_ERR_ARG; }
 
 extern "C" int qp3d_batch_per_shot(const qp3d_batch *b) { return b ? (b->per_shot ? 1 : 0) : QP3D_ERR_ARG; }
+
+// Persistent mode: -1 when off, else the tile set (PSET_TILES) of the one launch per run.
+extern "C" int qp3d_batch_persistent(const qp3d_batch *b) { return b ? (b->persist ? b->pset : -1) : QP3D_ERR_ARG; }
+
+// Turns persistent mode on with tile set `set` (0 .. NPSETS-1) and each shot's items cut for
+// its share of the machine (share = 1) or for the whole machine (0), or off (set = -1).
+extern "C" int qp3d_batch_set_persistent(qp3d_batch *b, int set, int share)
+{
+  if (!b || set < -1 || set >= NPSETS || share < 0 || share > 1) {
+    qp3d_set_error("batch_set_persistent: bad arguments");
+    return QP3D_ERR_ARG;
+  }
+  HIP_TRY(hipSetDevice(b->device));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->persist = set >= 0;
+  if (b->persist) {
+    b->per_shot = false;
+    b->pset = set;
+    b->pshare = share == 1;
+    b->pbuilt = false;
+  }
+  return QP3D_OK;
+}
+
+// Bound of one dependency wait of the persistent launch, in microseconds (default 5 s); a wait
+// past it makes the launch leave and qp3d_batch_sync return QP3D_ERR_TIMEOUT.
+extern "C" int qp3d_batch_set_wait_limit(qp3d_batch *b, double us)
+{
+  if (!b || !(us >= 0)) {
+    qp3d_set_error("batch_set_wait_limit: bad arguments");
+    return QP3D_ERR_ARG;
+  }
+  b->wait_us = us;
+  return QP3D_OK;
+}
 
 extern "C" int qp3d_batch_set_per_shot(qp3d_batch *b, int on)
 {
@@ -460,6 +657,7 @@ extern "C" int qp3d_batch_set_per_shot(qp3d_batch *b, int on)
     b->graph = nullptr;
     b->per_shot = on == 1;
   }
+  if (on == 1) b->persist = false;
   return QP3D_OK;
 }
 

@@ -62,6 +62,11 @@ struct Tile {
   static_assert(TX * 2 * H <= NT, "one y-halo value per lane");
 };
 
+// The double-buffered LDS plane of a tile (plane z with its 5-cell halos, LX + 1 rows: the
+// last holds a spare slot), owned by the kernel that runs the tile.
+template <class T>
+using TileLds = float (*)[T::LX + 1][T::LY];
+
 // p advanced by a byte offset (a 32-bit offset from a scalar base keeps loads in the
 // scalar-base + vector-offset form).
 template <class P>
@@ -497,9 +502,8 @@ __device__ __forceinline__ void reduce_gmax(float m, bool nonfinite, int tx, int
 // whose decision |grad u| > thr differs between ts and tt; the workgroup exits at once when
 // the band was valid.
 template <class S, class T, bool MARK = false>
-__device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int kstep)
+__device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int kstep, TileLds<T> sm)
 {
-  __shared__ float sm[2][T::LX + 1][T::LY];
   const Geo &g = a.g;
   int x0, x1, y0, y1, z0, z1;
   bool fold;
@@ -602,14 +606,16 @@ __device__ __forceinline__ int4 batch_item(const int4 *work, int &shot)
 template <class S, class T>
 __global__ __launch_bounds__(T::NT) void k_gradmax(GradArgs<S> a)
 {
-  gradmax_tile<S, T>(a, a.work[blockIdx.x], a.step.get());
+  __shared__ float sm[2][T::LX + 1][T::LY];
+  gradmax_tile<S, T>(a, a.work[blockIdx.x], a.step.get(), sm);
 }
 
 // Marking pass of a step whose predicted threshold missed (gmax_mode 2); see gradmax_tile.
 template <class S, class T>
 __global__ __launch_bounds__(T::NT) void k_spec_mark(GradArgs<S> a)
 {
-  gradmax_tile<S, T, true>(a, a.work[blockIdx.x], a.step.get());
+  __shared__ float sm[2][T::LX + 1][T::LY];
+  gradmax_tile<S, T, true>(a, a.work[blockIdx.x], a.step.get(), sm);
 }
 
 template <class S, class T>
@@ -618,7 +624,8 @@ __global__ __launch_bounds__(T::NT) void k_gradmax_batch(const GradArgs<S> *A, c
   int shot;
   const int4 w = batch_item(work, shot);
   const GradArgs<S> a = A[shot];
-  gradmax_tile<S, T>(a, w, st.get());
+  __shared__ float sm[2][T::LX + 1][T::LY];
+  gradmax_tile<S, T>(a, w, st.get(), sm);
 }
 
 
@@ -694,9 +701,8 @@ struct CellLoad {
 // SPEC (gmax_mode 2): the threshold is predicted (spec_band), cells in the band are marked in
 // a.fixmask, and the tile's maximum gradient component goes into G_k.
 template <class S, class T, int PML, bool SPEC = false>
-__device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kstep)
+__device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kstep, TileLds<T> sm)
 {
-  __shared__ float sm[2][T::LX + 1][T::LY];
   const Geo &g = a.g;
   int x0, x1, y0, y1, z0, z1;
   bool fold;
@@ -910,7 +916,8 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
 template <class S, class T, int PML, bool SPEC = false>
 __global__ __launch_bounds__(T::NT) void k_update(UpdArgs<S> a)
 {
-  update_tile<S, T, PML, SPEC>(a, a.work[blockIdx.x], a.step.get());
+  __shared__ float sm[2][T::LX + 1][T::LY];
+  update_tile<S, T, PML, SPEC>(a, a.work[blockIdx.x], a.step.get(), sm);
 }
 
 template <class S, class T, int PML>
@@ -919,7 +926,8 @@ __global__ __launch_bounds__(T::NT) void k_update_batch(const UpdArgs<S> *A, con
   int shot;
   const int4 w = batch_item(work, shot);
   const UpdArgs<S> a = A[shot];  // one copy: stores to the fields must not force reloads
-  update_tile<S, T, PML>(a, w, st.get());
+  __shared__ float sm[2][T::LX + 1][T::LY];
+  update_tile<S, T, PML>(a, w, st.get(), sm);
 }
 
 // Source cells outside [5,n-5)^3 are never updated by the stencil; the reference's u3 keeps

@@ -35,4 +35,16 @@ int occupancy_gradmax_batch(int tile);
 template <class S, int PML>
 int occupancy_update_batch(int tile);
 
+// persistent batch kernel (qp3d_persist.hpp) of tile set SET: its launch (blocks resident
+// workgroups) and its resident workgroups per CU
+template <class S> struct PersistArgs;
+template <class S, int SET>
+void launch_persist(int blocks, hipStream_t st, const PersistArgs<S> &a);
+template <class S, int SET>
+int occupancy_persist();
+// tile shapes (QP3D_TILES index) of each persistent tile set: gradient pass, interior update,
+// PML x/z faces, y faces; one block shape (64 x WX) per set
+constexpr int NPSETS = 3;
+constexpr int PSET_TILES[NPSETS][4] = {{0, 0, 3, 3}, {4, 4, 4, 4}, {1, 1, 1, 1}};
+
 }  // namespace qp3d

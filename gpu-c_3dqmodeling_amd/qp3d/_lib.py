@@ -19,6 +19,7 @@ QP3D_ERR_ALLOC = -3
 QP3D_ERR_HIP = -4
 QP3D_ERR_NONFINITE = -5
 QP3D_ERR_STATE = -6
+QP3D_ERR_TIMEOUT = -7
 
 STORAGE = {"fp32": 0, "bf16": 1, "fp16": 2}
 NTILES = 7  # launchable tile shapes (QP3D_NTILES in include/qp3d.h)
@@ -94,6 +95,9 @@ _SIGS = [
     ("qp3d_batch_per_shot", C.c_int, [C.c_void_p]),
     ("qp3d_batch_share", C.c_int, [C.c_void_p]),
     ("qp3d_batch_set_per_shot", C.c_int, [C.c_void_p, C.c_int]),
+    ("qp3d_batch_persistent", C.c_int, [C.c_void_p]),
+    ("qp3d_batch_set_persistent", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("qp3d_batch_set_wait_limit", C.c_int, [C.c_void_p, C.c_double]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]

@@ -244,7 +244,19 @@ class Batch:
         ms = (C.c_double * 4)()
         check(self.lib.qp3d_batch_autotune(self.h, int(steps), tiles, ms))
         return {"tiles": list(tiles), "ms": list(ms), "per_shot": bool(self.lib.qp3d_batch_per_shot(self.h)),
-                "share": bool(self.lib.qp3d_batch_share(self.h))}
+                "share": bool(self.lib.qp3d_batch_share(self.h)), "persistent": self.persistent()}
+
+    def persistent(self):
+        """Tile set of the one-launch-per-run persistent mode, or -1 when it is off."""
+        return int(self.lib.qp3d_batch_persistent(self.h))
+
+    def set_persistent(self, tile_set=0, share=False):
+        """Persistent mode with a tile set (0, 1, 2), or off (tile_set=-1)."""
+        check(self.lib.qp3d_batch_set_persistent(self.h, int(tile_set), int(bool(share))))
+
+    def set_wait_limit(self, us):
+        """Bound (microseconds) of one device-side wait of the persistent launch."""
+        check(self.lib.qp3d_batch_set_wait_limit(self.h, float(us)))
 
     def set_per_shot(self, on=True):
         """Each shot's own launches within a step (True) or one launch per stage (False)."""

@@ -23,6 +23,7 @@ extern "C" {
 #define QP3D_ERR_HIP (-4)       /* a HIP runtime call failed */
 #define QP3D_ERR_NONFINITE (-5) /* the wavefield became inf/NaN (CFL violation) */
 #define QP3D_ERR_STATE (-6)     /* call order violated (e.g. run before upload) */
+#define QP3D_ERR_TIMEOUT (-7)   /* a bounded device-side wait ran out (persistent batch launch) */
 
 /* Message of the last failure on the calling thread ("" if none). */
 const char *qp3d_last_error(void);
@@ -249,7 +250,8 @@ int qp3d_batch_sync(qp3d_batch *b);
 /* Picks how a step goes out, timing `steps` steps of each candidate: one launch per stage for
  * all shots with each shot's work items cut for the whole machine (as its own plan cuts them)
  * or for its share of the machine, each with every stage's tile shape timed and the fastest
- * kept, or each shot's own launches one after another; then resets;
+ * kept; each shot's own launches one after another; or one persistent launch per run, with
+ * each tile set and either cut; then resets;
  * tiles_out[4] / ms_out[4] (may be NULL): the shared launches' tiles and stage times: gradient
  * max, interior, PML x/z faces, PML y faces. */
 int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, double *ms_out);
@@ -260,6 +262,18 @@ int qp3d_batch_share(const qp3d_batch *b);
  * runs one launch per stage for all shots (the default); the setter forces either. */
 int qp3d_batch_per_shot(const qp3d_batch *b);
 int qp3d_batch_set_per_shot(qp3d_batch *b, int on);
+/* Persistent mode (the shots of the reference's serial loop, Main.c:179-247, in one launch): each
+ * qp3d_batch_run is a single launch over all its steps and shots. Resident workgroups take
+ * work items (gradient-pass and update tiles of one shot and step) from one queue; two item
+ * counters per shot order a shot's phases, so shots advance independently. Every
+ * device-side wait is bounded (qp3d_batch_set_wait_limit, default 5 s): past the bound the
+ * launch leaves and qp3d_batch_sync returns QP3D_ERR_TIMEOUT. qp3d_batch_persistent: the tile
+ * set in use (0 .. 2), -1 when off; the setter turns it on with a tile set and cut (share 1:
+ * each shot's items cut for its share of the machine) or off (set -1); qp3d_batch_autotune
+ * chooses it when it is the fastest form. Bit-identical to the other forms. */
+int qp3d_batch_persistent(const qp3d_batch *b);
+int qp3d_batch_set_persistent(qp3d_batch *b, int set, int share);
+int qp3d_batch_set_wait_limit(qp3d_batch *b, double us);
 /* bytes[0] gradient-max bytes, bytes[1] update bytes per step (all shots), bytes[2] updated
  * cells, bytes[3] interior cells. */
 int qp3d_batch_step_bytes(const qp3d_batch *b, double *bytes);
