@@ -268,10 +268,10 @@ def bench_batch(args):
     """Batched shots: an 800x800x300 model (seed 3), shots on a random surface grid (seed 3),
     each with its own window of 160-320 cells in x and y by 300 in z (pml 32), 20 Hz Ricker at
     the window centre, the full surface receiver plane of the window, 2500 steps; all shots in
-    one launch per stage and step (or each shot's own launches within a step, when the
-    autotuner finds that faster), replayed from a captured graph, tile shapes from the
-    autotuner. The same shots run one after another (each plan alone, also with its graph)
-    give the baseline."""
+    the form the autotuner finds fastest: one persistent launch per run, or one launch per stage
+    and step replayed from a captured graph (or each shot's own launches within a step), with
+    its tile shapes. The other of the first two forms is timed too. The same shots run one after
+    another (each plan alone, also with its graph) give the baseline."""
     import qp3d
     nt = 2500 if args.nt == 3000 else args.nt
     t = time.perf_counter()
@@ -287,16 +287,30 @@ def bench_batch(args):
         t = time.perf_counter()
         tune = b.autotune(steps=4)
         log(f"autotune {tune} in {time.perf_counter() - t:.1f} s")
-        b.reset()
-        b.run(0, 20)
-        b.sync()
-        t0 = time.perf_counter()
-        b.reset()
-        b.run()
-        b.sync()
-        wall = time.perf_counter() - t0
+        def timed_run():
+            b.reset()
+            b.run(0, 20)
+            b.sync()
+            t0 = time.perf_counter()
+            b.reset()
+            b.run()
+            b.sync()
+            return time.perf_counter() - t0
+
+        wall = timed_run()
         kb = b.step_bytes()
-    log(f"batched: {wall:.2f} s")
+        log(f"batched ({'persistent' if tune['persistent'] >= 0 else 'launches'}): {wall:.2f} s")
+        # the other form, forced: one persistent launch per run (the autotuner's fastest tile set
+        # and cut), or the per-stage launches
+        pset, pshare = b.persistent_choice()
+        if tune["persistent"] >= 0:
+            b.set_persistent(-1, False)
+            b.set_per_shot(tune["per_shot"])
+            wall_p, wall_l = wall, timed_run()
+        else:
+            b.set_persistent(pset, pshare)
+            wall_p, wall_l = timed_run(), wall
+        log(f"persistent launch (tile set {pset}, share {pshare}): {wall_p:.2f} s; launches: {wall_l:.2f} s")
     seq = None
     if args.sequential:
         t0 = time.perf_counter()
@@ -311,7 +325,8 @@ def bench_batch(args):
     step_bytes = kb["gradmax"] + kb["update"]
     out = {"metric": f"Gcell-updates/s ({args.shots} shots batched, {nt} steps)", "value": round(cells / wall / 1e9, 3),
            "unit": "Gcell-updates/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(wall * 1e3, 1),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "fp32" if args.storage == "fp32" else f"fp32 arithmetic, {args.storage} storage",
            "data": "synthetic (layered VTI + smooth random perturbation, seed 3)",
            "config": {"workload": f"{args.shots} shots, windows {args.win[0]}-{args.win[1]} x {args.win[0]}-{args.win[1]} "
                                   f"x {args.depth} of an 800x800x{args.depth} model, "
@@ -320,7 +335,10 @@ def bench_batch(args):
                       "autotuned_per_shot_launches": tune["per_shot"],
                       "autotuned_items_per_shot_share": tune["share"],
                       "autotune_ms_per_step": [round(x, 4) for x in tune["ms"]],
-                      "batched_s": round(wall, 3), "sequential_s": round(seq, 3) if seq else None,
+                      "autotuned_persistent_tile_set": tune["persistent"],
+                      "batched_s": round(wall, 3), "persistent_launch_s": round(wall_p, 3),
+                      "persistent_tile_set_share": [pset, pshare], "stage_launches_s": round(wall_l, 3),
+                      "sequential_s": round(seq, 3) if seq else None,
                       "speedup_vs_sequential": round(seq / wall, 3) if seq else None,
                       "hbm_gbs_algorithmic": round(step_bytes * nt / wall / 1e9, 1)}}
     print(json.dumps(out), flush=True)

@@ -610,6 +610,16 @@ extern "C" int qp3d_batch_per_shot(const qp3d_batch *b) { return b ? (b->per_sho
 // Persistent mode: -1 when off, else the tile set (PSET_TILES) of the one launch per run.
 extern "C" int qp3d_batch_persistent(const qp3d_batch *b) { return b ? (b->persist ? b->pset : -1) : QP3D_ERR_ARG; }
 
+// The persistent form the batch would use (the fastest one the last autotune timed, or the one
+// set last): its tile set and cut, whether or not persistent mode is on.
+extern "C" int qp3d_batch_persistent_choice(const qp3d_batch *b, int *set, int *share)
+{
+  if (!b || !set || !share) return QP3D_ERR_ARG;
+  *set = b->pset;
+  *share = b->pshare ? 1 : 0;
+  return QP3D_OK;
+}
+
 // Turns persistent mode on with tile set `set` (0 .. NPSETS-1) and each shot's items cut for
 // its share of the machine (share = 1) or for the whole machine (0), or off (set = -1).
 extern "C" int qp3d_batch_set_persistent(qp3d_batch *b, int set, int share)

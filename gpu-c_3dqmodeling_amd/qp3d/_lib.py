@@ -98,6 +98,7 @@ _SIGS = [
     ("qp3d_batch_persistent", C.c_int, [C.c_void_p]),
     ("qp3d_batch_set_persistent", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("qp3d_batch_set_wait_limit", C.c_int, [C.c_void_p, C.c_double]),
+    ("qp3d_batch_persistent_choice", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]

@@ -254,6 +254,12 @@ class Batch:
         """Persistent mode with a tile set (0, 1, 2), or off (tile_set=-1)."""
         check(self.lib.qp3d_batch_set_persistent(self.h, int(tile_set), int(bool(share))))
 
+    def persistent_choice(self):
+        """(tile set, share) of the fastest persistent form of the last autotune."""
+        a, b = C.c_int(), C.c_int()
+        check(self.lib.qp3d_batch_persistent_choice(self.h, C.byref(a), C.byref(b)))
+        return a.value, bool(b.value)
+
     def set_wait_limit(self, us):
         """Bound (microseconds) of one device-side wait of the persistent launch."""
         check(self.lib.qp3d_batch_set_wait_limit(self.h, float(us)))

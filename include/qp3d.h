@@ -274,6 +274,8 @@ int qp3d_batch_set_per_shot(qp3d_batch *b, int on);
 int qp3d_batch_persistent(const qp3d_batch *b);
 int qp3d_batch_set_persistent(qp3d_batch *b, int set, int share);
 int qp3d_batch_set_wait_limit(qp3d_batch *b, double us);
+/* The persistent form (tile set, cut) the fastest in the last autotune, or the one set last. */
+int qp3d_batch_persistent_choice(const qp3d_batch *b, int *set, int *share);
 /* bytes[0] gradient-max bytes, bytes[1] update bytes per step (all shots), bytes[2] updated
  * cells, bytes[3] interior cells. */
 int qp3d_batch_step_bytes(const qp3d_batch *b, double *bytes);
