@@ -372,6 +372,8 @@ def main():
     ap.add_argument("--win", type=int, nargs=2, default=[160, 320], help="batch: window size range in x and y")
     ap.add_argument("--depth", type=int, default=300, help="batch: model depth (cells)")
     ap.add_argument("--sequential", type=int, default=1, help="batch: also run the shots one after another")
+    ap.add_argument("--amp-bound", type=int, default=1,
+                    help="forward: gradient pass under the amplitude bound (1) or every tile evaluated (0)")
     ap.add_argument("--workload", default="forward", choices=["forward", "fwi", "precision", "batch", "dropin"],
                     help="forward: the 512^3 shot (default); fwi: the 256^3 adjoint gradient; "
                          "precision: FP32/BF16/FP16 storage against the FP64 oracle on a 256^3 crop; "
@@ -404,6 +406,8 @@ def main():
                      tile_grad=args.tile_grad, tile_pml=args.tile_pml, tile_pml_y=args.tile_pml_y,
                      streams=args.streams, storage=args.storage, gmax_mode=args.gmax_mode)
     log("model uploaded")
+    if not args.amp_bound:
+        plan.set_amp_bound(False)
     for i in range(args.warmup):
         t = time.perf_counter()
         plan.forward_async()
@@ -422,6 +426,7 @@ def main():
     t1 = time.perf_counter()
     wall = t1 - t0
     kt_timed = plan.kernel_times()
+
     if args.graph:
         plan.set_instrument(2)
         plan.forward_async()
@@ -431,6 +436,7 @@ def main():
     plan.forward_async()
     plan.sync()
     kt = plan.kernel_times()
+    gtiles = plan.gradient_tiles()  # evaluated, skipped over the instrumented shot
     kt["update_interior"] = kt_timed["update_interior"]
     kb = plan.step_bytes()
     plan.set_instrument(0)
@@ -493,6 +499,9 @@ def main():
                    "ms_update_pml": round(kt["update_pml"], 5), "ms_update_stage": round(kt["update_stage"], 5),
                    "gbs_gradmax": round(gbs.get("gradmax", 0), 1), "gbs_update_pml": round(gbs.get("update_pml", 0), 1),
                    "storage": args.storage, "gmax_mode": gmode, "steps_in_predicted_band": in_band,
+                   "gradient_amp_bound": bool(args.amp_bound) and gmode == 1,
+                   "gradient_tiles_evaluated_skipped": list(gtiles),
+                   "gradient_tiles_skipped_frac": round(gtiles[1] / max(1, gtiles[0] + gtiles[1]), 4),
                    "roofline_kernel": f"k_update<interior> ({kb['update_interior'] / kb['cells_interior']:.0f} B per "
                                       f"interior cell, {kb['cells_interior']:.0f} cells per launch)",
                    "cpu_baseline_single_thread": cpu1},

@@ -103,7 +103,14 @@ static int upload_args(qp3d_batch *b)
   for (int par = 0; par < 2; par++) {
     std::vector<GradArgs<S>> g(n);
     std::vector<UpdArgs<S>> u(n);
-    for (size_t i = 0; i < n; i++) make_args<S>(b->plans[i], par, Step{nullptr, 0}, g[i], u[i]);
+    for (size_t i = 0; i < n; i++) {
+      make_args<S>(b->plans[i], par, Step{nullptr, 0}, g[i], u[i]);
+      // no amplitude bound in batches: every gradient tile is evaluated
+      g[i].amp = AmpGrid{};
+      g[i].argkey = nullptr;
+      g[i].skip = 0;
+      u[i].amp = AmpGrid{};
+    }
     HIP_TRY(hipMalloc(&b->args_g[par], sizeof(GradArgs<S>) * n));
     HIP_TRY(hipMalloc(&b->args_u[par], sizeof(UpdArgs<S>) * n));
     HIP_TRY(hipMemcpy(b->args_g[par], g.data(), sizeof(GradArgs<S>) * n, hipMemcpyHostToDevice));
@@ -255,7 +262,7 @@ static int batch_step_s(qp3d_batch *b, int k, Step st)
     const int blocks = (int)std::min<long>((nb + 255) / 256, 4096);
     hipLaunchKernelGGL(k_rim_source<S>, dim3(blocks), dim3(256), 0, b->stream, (const T *)p->u[cur], (T *)p->u[prev],
                        p->geo, p->sb[0], p->sb[1], p->sb[2], p->sb[3], p->sb[4], p->sb[5], (const float *)p->sbox,
-                       (const float *)p->wav, st);
+                       (const float *)p->wav, st, AmpGrid{});
     HIP_TRY(hipGetLastError());
   }
   return QP3D_OK;
@@ -428,7 +435,10 @@ extern "C" int qp3d_batch_run(qp3d_batch *b, int k0, int k1)
       if (rc) return rc;
     }
     b->steps_done = k1;
-    for (qp3d_plan *p : b->plans) p->steps_done = k1;
+    for (qp3d_plan *p : b->plans) {
+      p->steps_done = k1;
+      p->amp_ok = false;
+    }
     return QP3D_OK;
   }
   const bool graph = b->use_graph && !b->timing;
@@ -449,7 +459,10 @@ extern "C" int qp3d_batch_run(qp3d_batch *b, int k0, int k1)
     }
   }
   b->steps_done = k1;
-  for (qp3d_plan *p : b->plans) p->steps_done = k1;
+  for (qp3d_plan *p : b->plans) {
+    p->steps_done = k1;
+    p->amp_ok = false;  // the batch kernels leave the plans' block maxima stale
+  }
   return QP3D_OK;
 }
 

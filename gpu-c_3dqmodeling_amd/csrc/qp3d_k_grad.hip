@@ -44,6 +44,19 @@ void launch_mark(int tile, int blocks, hipStream_t st, const GradArgs<S> &a)
 }
 
 template <class S>
+void launch_gbound(int tile, int blocks, hipStream_t st, const GradArgs<S> &a)
+{
+  int i = 0;
+#define QP3D_L(wx, rx)                                                                              \
+  if (tile == i++) {                                                                                \
+    hipLaunchKernelGGL((k_gbound<S, Tile<wx, rx>>), dim3(blocks), dim3(TY, wx), 0, st, a);          \
+    return;                                                                                         \
+  }
+  QP3D_TILES(QP3D_L)
+#undef QP3D_L
+}
+
+template <class S>
 int occupancy_gradmax(int tile)
 {
   int i = 0;
@@ -80,6 +93,7 @@ int occupancy_gradmax_batch(int tile)
 
 template void launch_gradmax<QP3D_STORE>(int, int, hipStream_t, const GradArgs<QP3D_STORE> &);
 template void launch_mark<QP3D_STORE>(int, int, hipStream_t, const GradArgs<QP3D_STORE> &);
+template void launch_gbound<QP3D_STORE>(int, int, hipStream_t, const GradArgs<QP3D_STORE> &);
 template int occupancy_gradmax<QP3D_STORE>(int);
 template void launch_gradmax_batch<QP3D_STORE>(int, int, const int4 *, hipStream_t, const GradArgs<QP3D_STORE> *, Step);
 template int occupancy_gradmax_batch<QP3D_STORE>(int);

@@ -156,6 +156,51 @@ struct Step {
   __device__ __forceinline__ int get() const { return (kdev ? *kdev : 0) + koff; }
 };
 
+// Amplitude bound of the gradient pass. The update kernels of step k record max |u_{k+1}| per
+// block of 8 planes x 8 rows x 32 cells (z, x, y; float bits, atomic max) in buffer (k+1) % 3 of
+// three. A gradient component at a cell is sum_m g_m (u(+m) - u(-m)), so its magnitude is at
+// most 2 sum_m |g_m| times max |u| within 5 cells: a gradient tile whose blocks (with that
+// halo) bound every component below a value of G_k already computed exactly cannot hold the
+// maximum, and the pass skips it without loading u. G_k stays exact. b == nullptr: off.
+constexpr int AB_Z = 3, AB_X = 3, AB_Y = 5;  // log2 of the block extents
+struct AmpGrid {
+  unsigned *b;   // [3][nbz][nbx][nby]
+  int nbz, nbx, nby;
+  unsigned nblk;  // nbz * nbx * nby
+  __device__ __forceinline__ unsigned *buf(int k) const { return b + (unsigned)(k % 3) * nblk; }
+};
+
+// Per-lane running maximum of |u_next| (float bits: inf and NaN order above every finite value)
+// over planes [za, zb), flushed into the block maxima of every z block those planes touch. Each
+// half-wavefront holds 32 consecutive y cells (a folded tile: one y face each), so its maximum
+// goes to the one or two y blocks its first and last lanes fall in, for every x block of the
+// tile. The (z block, x block, half, y block) targets are spread over the lanes: one atomic
+// instruction per flush (at most 4 z blocks x 3 x blocks x 4 targets = 48 lanes).
+__device__ __forceinline__ void amp_flush(const AmpGrid &ag, int kstep, int za, int zb, int x0, int x1, int ylane,
+                                          unsigned &am)
+{
+  unsigned v = am;
+#pragma unroll
+  for (int off = 16; off >= 1; off >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, off, 64));
+  const int yb = min(ylane >> AB_Y, ag.nby - 1);
+  // targets t = 0..3: half 0 first / last lane's y block, half 1 first / last
+  const int ty[4] = {__builtin_amdgcn_readlane(yb, 0), __builtin_amdgcn_readlane(yb, 31),
+                     __builtin_amdgcn_readlane(yb, 32), __builtin_amdgcn_readlane(yb, 63)};
+  const unsigned tv[2] = {(unsigned)__builtin_amdgcn_readlane((int)v, 0), (unsigned)__builtin_amdgcn_readlane((int)v, 32)};
+  const int bz0 = za >> AB_Z, nbz = ((zb - 1) >> AB_Z) - bz0 + 1;
+  const int bx0 = x0 >> AB_X, nbx = ((x1 - 1) >> AB_X) - bx0 + 1;
+  const int l = __lane_id();
+  const int t = l & 3, c = l >> 2;  // target, (z block, x block) pair
+  const int bz = bz0 + c / nbx, bx = bx0 + c % nbx;
+  const int ybt = t == 0 ? ty[0] : t == 1 ? ty[1] : t == 2 ? ty[2] : ty[3];
+  const unsigned val = t < 2 ? tv[0] : tv[1];
+  // the last-lane target only where it differs from the first-lane one
+  const bool dup = (t == 1 && ty[1] == ty[0]) || (t == 3 && ty[3] == ty[2]);
+  if (c < nbz * nbx && val != 0u && !dup)
+    atomicMax(ag.buf(kstep + 1) + ((unsigned)bz * ag.nbx + (unsigned)bx) * ag.nby + (unsigned)ybt, val);
+  am = 0u;
+}
+
 // The y faces of the PML fold into one tile per x row group when each is at most 27 cells
 // wide (pml <= 32: the two faces with their 5-cell halos fit one 64-lane wavefront).
 __host__ __device__ constexpr bool y_fold(int pml) { return pml <= 32; }
@@ -209,6 +254,7 @@ struct UpdArgs {
   float *rec;                               // record [nt][hx][hy]
   int sz;                                   // receiver plane
   const int4 *work;                         // one item per workgroup
+  AmpGrid amp;                              // block maxima of |u_next| (b == nullptr: off)
 };
 
 template <class S>
@@ -222,6 +268,11 @@ struct GradArgs {
   Step step;
   unsigned *fixmask;  // marking pass: one bit per cell whose threshold decision changes
   const float *gpred; // marking pass: predicted G per step
+  AmpGrid amp;        // block maxima of |u_k| (b == nullptr: no skipping)
+  int skip;           // 1: skip tiles whose amplitude bound is below the G_k found so far
+  unsigned long long *argkey;  // [2]: (tile maximum bits << 32 | item) of the step's largest tile
+  int nwork;          // items in `work`
+  unsigned long long *gcount;  // [2]: tiles evaluated, tiles skipped (instrumented steps; else null)
 };
 
 // A lane's fixed position in its tile and its in-plane offsets.
@@ -464,7 +515,8 @@ __device__ __forceinline__ void mark_cell(unsigned *mask, unsigned cell) { atomi
 // Workgroup maximum of m into *gmax (float bits, one atomic per workgroup); *flag |= 1 when a
 // lane saw a non-finite gradient (nonfinite, or m above the float range).
 template <class T>
-__device__ __forceinline__ void reduce_gmax(float m, bool nonfinite, int tx, int ty, unsigned *gmax, int *flag)
+__device__ __forceinline__ void reduce_gmax(float m, bool nonfinite, int tx, int ty, unsigned *gmax, int *flag,
+                                            unsigned long long *argkey = nullptr, int item = -1)
 {
   __shared__ float red[T::WX];
   __shared__ int redbad[T::WX];
@@ -489,7 +541,53 @@ __device__ __forceinline__ void reduce_gmax(float m, bool nonfinite, int tx, int
     }
     atomicMax(gmax, __float_as_uint(v));
     if (bb) atomicOr(flag, 1);
+    // the tile's maximum with its item: the next step's bound pass starts from that tile
+    if (argkey && item >= 0) atomicMax(argkey, ((unsigned long long)__float_as_uint(v) << 32) | (unsigned)item);
   }
+}
+
+// Amplitude bound of a gradient tile (see AmpGrid): true when every gradient component of the
+// tile is provably below the G_k already found (*gmax). Uniform over the workgroup (the values
+// pass through LDS).
+template <class S, class T>
+__device__ __forceinline__ bool amp_skip(const GradArgs<S> &a, int x0, int x1, int y0, int y1, int z0, int z1,
+                                         int kstep)
+{
+  __shared__ unsigned amp_red[T::WX + 1];
+  const AmpGrid &ag = a.amp;
+  const int zb0 = max(0, (z0 - H) >> AB_Z), zb1 = min(ag.nbz - 1, (z1 - 1 + H) >> AB_Z);
+  const int xb0 = max(0, (x0 - H) >> AB_X), xb1 = min(ag.nbx - 1, (x1 - 1 + H) >> AB_X);
+  const int yb0 = max(0, (y0 - H) >> AB_Y), yb1 = min(ag.nby - 1, (y1 - 1 + H) >> AB_Y);
+  const int nxb = xb1 - xb0 + 1, nyb = yb1 - yb0 + 1;
+  const int nb = (zb1 - zb0 + 1) * nxb * nyb;
+  const unsigned *bb = ag.buf(kstep);
+  const int tid = threadIdx.y * TY + threadIdx.x;
+  unsigned m = 0u;
+  for (int i = tid; i < nb; i += T::NT) {
+    const int yb = i % nyb, t = i / nyb, xb = t % nxb, zb = t / nxb;
+    m = max(m, bb[((unsigned)(zb0 + zb) * ag.nbx + (unsigned)(xb0 + xb)) * ag.nby + (unsigned)(yb0 + yb)]);
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off, 64));
+  if (threadIdx.x == 0) amp_red[threadIdx.y] = m;
+  // any value of *gmax ever stored is the maximum of exactly computed components, so a cached
+  // (possibly older) value is as valid a lower bound as the newest, and costs no trip past L2
+  if (tid == 0) amp_red[T::WX] = __hip_atomic_load(a.gmax + kstep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  __syncthreads();
+  unsigned M = amp_red[0];
+#pragma unroll
+  for (int w = 1; w < T::WX; w++) M = max(M, amp_red[w]);
+  const float Gk = __uint_as_float(amp_red[T::WX]);
+  // |sum_m g_m (u+m - u-m)| <= 2 sum_m |g_m| max|u|; the factor covers the rounding of the sum
+  float fx = 0.f, fy = 0.f, fz = 0.f;
+#pragma unroll
+  for (int k = 0; k < H; k++) {
+    fx += fabsf(a.c.g1x[k]);
+    fy += fabsf(a.c.g1y[k]);
+    fz += fabsf(a.c.g1z[k]);
+  }
+  const float fac = 2.f * fmaxf(fx, fmaxf(fy, fz)) * (1.f + 1.0e-5f);
+  return M < 0x7f800000u && __uint_as_float(M) * fac < Gk;
 }
 
 // One work item of the gradient-max pass (w: packed tile, kstep: time step). Each lane owns a
@@ -502,13 +600,21 @@ __device__ __forceinline__ void reduce_gmax(float m, bool nonfinite, int tx, int
 // whose decision |grad u| > thr differs between ts and tt; the workgroup exits at once when
 // the band was valid.
 template <class S, class T, bool MARK = false>
-__device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int kstep, TileLds<T> sm)
+__device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int kstep, TileLds<T> sm, int item = -1)
 {
   const Geo &g = a.g;
   int x0, x1, y0, y1, z0, z1;
   bool fold;
   unpack(w, x0, x1, y0, y1, z0, z1, fold);
   if (z0 >= z1) return;
+  if constexpr (!MARK) {
+    if (a.skip && a.amp.b) {
+      // a uniform decision in a scalar register
+      const int sk = __builtin_amdgcn_readfirstlane(amp_skip<S, T>(a, x0, x1, y0, y1, z0, z1, kstep) ? 1 : 0);
+      if (a.gcount && threadIdx.x == 0 && threadIdx.y == 0) atomicAdd(a.gcount + sk, 1ull);
+      if (sk) return;
+    }
+  }
   float ts = 0.f, tt = 0.f;
   if constexpr (MARK) {
     const SpecBand sb = spec_band(a.gpred, kstep);
@@ -591,7 +697,9 @@ __device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int k
     }
   };
   for (; zb < z1; zb += 2 * QN) Unroll<0, 2 * QN>::run(plane, z1 - zb);
-  if constexpr (!MARK) reduce_gmax<T>(m, !(nanacc == 0.f), L.tx, L.ty, a.gmax + kstep, a.flag);
+  if constexpr (!MARK)
+    reduce_gmax<T>(m, !(nanacc == 0.f), L.tx, L.ty, a.gmax + kstep, a.flag, a.argkey ? a.argkey + (kstep & 1) : nullptr,
+                   item);
 }
 
 // Batched launches (several grids in one launch): item .z packs z0 | z1 << 16 and .w is the
@@ -607,7 +715,36 @@ template <class S, class T>
 __global__ __launch_bounds__(T::NT) void k_gradmax(GradArgs<S> a)
 {
   __shared__ float sm[2][T::LX + 1][T::LY];
-  gradmax_tile<S, T>(a, a.work[blockIdx.x], a.step.get(), sm);
+  gradmax_tile<S, T>(a, a.work[blockIdx.x], a.step.get(), sm, (int)blockIdx.x);
+}
+
+// First part of the gradient pass with the amplitude bound: the tile that held the largest
+// gradient component at step k-1, split in z over the grid, evaluated exactly, so that the
+// skipping pass starts from a G_k close to the final one. It also clears the block maxima that
+// the updates of step k+1 fill ((k+2) % 3) and the key of this step's largest tile.
+template <class S, class T>
+__global__ __launch_bounds__(T::NT) void k_gbound(GradArgs<S> a)
+{
+  __shared__ float sm[2][T::LX + 1][T::LY];
+  const int kstep = a.step.get();
+  const unsigned tid = threadIdx.y * TY + threadIdx.x;
+  unsigned *clr = a.amp.buf(kstep + 2);
+  for (unsigned i = blockIdx.x * T::NT + tid; i < a.amp.nblk; i += gridDim.x * T::NT) clr[i] = 0u;
+  if (blockIdx.x == 0 && tid == 0) a.argkey[kstep & 1] = 0ull;
+  const unsigned long long key = a.argkey[(kstep + 1) & 1];  // written by step k-1
+  int item = (int)(key & 0xffffffffull);
+  if (item < 0 || item >= a.nwork) item = 0;
+  int4 w = a.work[item];
+  const int len = w.w - w.z;
+  const int c = (len + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int s0 = w.z + (int)blockIdx.x * c, s1 = min(w.w, s0 + c);
+  if (s0 >= s1) return;
+  w.z = s0;
+  w.w = s1;
+  GradArgs<S> b = a;
+  b.skip = 0;
+  b.argkey = nullptr;
+  gradmax_tile<S, T>(b, w, kstep, sm);
 }
 
 // Marking pass of a step whose predicted threshold missed (gmax_mode 2); see gradmax_tile.
@@ -736,6 +873,8 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
   }
   float gm = 0.f;          // SPEC: G_k over this tile's cells
   bool nonfinite = false;  // SPEC: a non-finite gradient component seen
+  const bool amp_on = a.amp.b != nullptr;  // block maxima of |u_next| for the next gradient pass
+  unsigned am = 0u;
   // source box and receiver plane tests, hoisted to the workgroup and the plane (uniform)
   const bool src_tile = a.src_on && a.sb_x0 < x1 && a.sb_x1 >= x0 &&
                         (fold || (a.sb_y0 < y1 && a.sb_y1 >= y0)) && a.sb_z0 < z1 && a.sb_z1 >= z0;
@@ -878,8 +1017,13 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
         }
         un = S::rnd(un);
         S::st(a.unext + zo, o, un);
+        if (amp_on) am = max(am, __float_as_uint(fabsf(un)));
         if (rec_plane && x >= pml && x < nx - pml && y >= pml && y < ny - pml)
           rec[(x - pml) * (ny - 2 * pml) + (y - pml)] = un;
+      }
+      if constexpr (PML != 0) {
+        // the rolled PML march flushes at the end of every z block
+        if (amp_on && ((z & ((1 << AB_Z) - 1)) == (1 << AB_Z) - 1 || !more)) amp_flush(a.amp, kstep, z, z + 1, x0, x1, y, am);
       }
       if (more) {
 #pragma unroll
@@ -895,7 +1039,11 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
     }
   };
   if constexpr (!PML) {
-    for (; zb < z1; zb += 2 * QN) Unroll<0, 2 * QN>::run(plane, z1 - zb);
+    for (; zb < z1; zb += 2 * QN) {
+      Unroll<0, 2 * QN>::run(plane, z1 - zb);
+      // once per unrolled stretch (its planes may cover up to four z blocks)
+      if (amp_on) amp_flush(a.amp, kstep, zb, min(zb + 2 * QN, z1), x0, x1, y, am);
+    }
   } else {
     // the PML kernel carries seven more state values per row; a rolled loop with a shifted
     // queue keeps it free of spills
@@ -935,9 +1083,10 @@ __global__ __launch_bounds__(T::NT) void k_update_batch(const UpdArgs<S> *A, con
 // GPU_kernel.cu:749-750), i.e. u_next = u_cur + s*w.
 template <class S>
 __global__ void k_rim_source(const typename S::T *ucur, typename S::T *unext, Geo g, int z0, int z1, int x0, int x1,
-                             int y0, int y1, const float *sbox, const float *wav, Step step)
+                             int y0, int y1, const float *sbox, const float *wav, Step step, AmpGrid amp)
 {
-  const float wk = wav[step.get()];
+  const int kstep = step.get();
+  const float wk = wav[kstep];
   const int bx = x1 - x0 + 1, by = y1 - y0 + 1, bz = z1 - z0 + 1;
   const long n = (long)bx * by * bz;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
@@ -946,8 +1095,11 @@ __global__ void k_rim_source(const typename S::T *ucur, typename S::T *unext, Ge
     const bool r5 = x >= H && x < g.nx - H && y >= H && y < g.ny - H && z >= H && z < g.nz - H;
     if (r5) continue;
     const unsigned b = 4u * ((unsigned)z * g.pln + (unsigned)x * g.ys + (unsigned)y);
-    const float v = fmaf(sbox[t], wk, S::ld(ucur, b));
-    S::st(unext, b, S::rnd(v));
+    const float v = S::rnd(fmaf(sbox[t], wk, S::ld(ucur, b)));
+    S::st(unext, b, v);
+    if (amp.b)
+      atomicMax(amp.buf(kstep + 1) + ((unsigned)(z >> AB_Z) * amp.nbx + (unsigned)(x >> AB_X)) * amp.nby + (y >> AB_Y),
+                __float_as_uint(fabsf(v)));
   }
 }
 

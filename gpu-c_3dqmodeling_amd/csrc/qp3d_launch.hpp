@@ -15,6 +15,9 @@ template <class S>
 void launch_gradmax(int tile, int blocks, hipStream_t st, const GradArgs<S> &a);
 template <class S>
 void launch_mark(int tile, int blocks, hipStream_t st, const GradArgs<S> &a);
+// first part of the gradient pass with the amplitude bound (k_gbound), `blocks` workgroups
+template <class S>
+void launch_gbound(int tile, int blocks, hipStream_t st, const GradArgs<S> &a);
 // resident workgroups per CU of k_gradmax for a tile shape
 template <class S>
 int occupancy_gradmax(int tile);

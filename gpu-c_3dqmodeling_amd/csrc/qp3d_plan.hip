@@ -112,7 +112,7 @@ int qp3d_cu_count(const qp3d_plan *p)
 
 // Uniform tiles over [5,n-5)^3 (the gradient-max pass); `capacity` is the number of
 // workgroups resident at once that the items should fill.
-void qp3d_items_uniform(const qp3d_plan *p, int tile, int capacity, std::vector<int4> &items)
+void qp3d_items_uniform(const qp3d_plan *p, int tile, int capacity, std::vector<int4> &items, int zchunk)
 {
   const qp3d_config &c = p->cfg;
   const int TX = tile_tx(tile);
@@ -120,7 +120,7 @@ void qp3d_items_uniform(const qp3d_plan *p, int tile, int capacity, std::vector<
   cut(xs, H, c.nx - H, TX);
   cut_aligned(ys, H, c.ny - H, TY);
   const int len = c.nz - 2 * H;
-  const int zc = choose_zchunk((int)(xs.size() * ys.size()), len, capacity, c.zchunk);
+  const int zc = choose_zchunk((int)(xs.size() * ys.size()), len, capacity, zchunk > 0 ? zchunk : c.zchunk);
   for (int z = H; z < c.nz - H; z += zc)
     for (const Seg &x : xs)
       for (const Seg &y : ys) items.push_back(pack(x.a, x.b, y.a, y.b, z, std::min(z + zc, c.nz - H)));
@@ -246,6 +246,13 @@ static int build_work(qp3d_plan *p)
   qp3d_plan_items(p, items);
   int rc;
   if ((rc = upload_list(p->wl_grad, items[0]))) return rc;
+  if (p->ampg.b) {
+    // the gradient pass under the amplitude bound: short items, so that a quiet stretch of a
+    // tile column is skipped on its own
+    std::vector<int4> ia;
+    qp3d_items_uniform(p, p->tile_g, 1, ia, AMP_ZCHUNK);
+    if ((rc = upload_list(p->wl_grad_amp, ia))) return rc;
+  }
   if ((rc = upload_list(p->wl_int, items[1]))) return rc;
   if ((rc = upload_list(p->wl_pml, items[2]))) return rc;
   if ((rc = upload_list(p->wl_pmly, items[3]))) return rc;
@@ -441,6 +448,22 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
     return fail(rc);
   if ((rc = dmalloc((void **)&p->flag, sizeof(int), "flag"))) return fail(rc);
   if ((rc = dmalloc((void **)&p->kdev, sizeof(int), "step index"))) return fail(rc);
+  if (!p->spec) {
+    // amplitude bound of the gradient pass: three buffers of block maxima, two step keys
+    AmpGrid &ag = p->ampg;
+    ag.nbz = (c.nz + (1 << AB_Z) - 1) >> AB_Z;
+    ag.nbx = (c.nx + (1 << AB_X) - 1) >> AB_X;
+    ag.nby = (c.ny + (1 << AB_Y) - 1) >> AB_Y;
+    ag.nblk = (unsigned)ag.nbz * ag.nbx * ag.nby;
+    if ((rc = dmalloc((void **)&ag.b, sizeof(unsigned) * 3 * (size_t)ag.nblk, "block maxima"))) return fail(rc);
+    if ((rc = dmalloc((void **)&p->argkey, 4 * sizeof(unsigned long long), "largest-tile keys"))) return fail(rc);
+    p->gcount = p->argkey + 2;
+    if (hipMemset(ag.b, 0, sizeof(unsigned) * 3 * (size_t)ag.nblk) != hipSuccess ||
+        hipMemset(p->argkey, 0, 4 * sizeof(unsigned long long)) != hipSuccess) {
+      qp3d_set_error("plan_create: clearing the block maxima failed");
+      return fail(QP3D_ERR_HIP);
+    }
+  }
   if (p->spec) {
     p->fixwords = (p->N + 31) / 32;
     if ((rc = dmalloc((void **)&p->fixmask, sizeof(unsigned) * (size_t)p->fixwords, "cell mask"))) return fail(rc);
@@ -489,11 +512,14 @@ extern "C" int qp3d_plan_destroy(qp3d_plan *p)
   (void)hipFree(p->gpred);
   (void)hipFree(p->kdev);
   (void)hipFree(p->wav);
+  (void)hipFree(p->ampg.b);
+  (void)hipFree(p->argkey);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   (void)hipFree(p->rec);
   (void)hipFree(p->sbox);
   (void)hipFree(p->wl_grad.d);
+  (void)hipFree(p->wl_grad_amp.d);
   (void)hipFree(p->wl_int.d);
   (void)hipFree(p->wl_pml.d);
   (void)hipFree(p->wl_pmly.d);
@@ -860,6 +886,11 @@ extern "C" int qp3d_plan_reset(qp3d_plan *p)
   if (p->gpred)
     HIP_TRY(hipMemsetAsync(p->gpred, 0, sizeof(float) * (size_t)(p->cfg.nt > 0 ? p->cfg.nt : 1), p->stream));
   if (p->rec) HIP_TRY(hipMemsetAsync(p->rec, 0, sizeof(float) * (size_t)p->cfg.nt * p->hx * p->hy, p->stream));
+  if (p->ampg.b) {
+    HIP_TRY(hipMemsetAsync(p->ampg.b, 0, sizeof(unsigned) * 3 * (size_t)p->ampg.nblk, p->stream));
+    HIP_TRY(hipMemsetAsync(p->argkey, 0, 4 * sizeof(unsigned long long), p->stream));
+  }
+  p->amp_ok = true;  // u = 0 everywhere, and so are the block maxima
   p->nsnap_taken = 0;
   p->steps_done = 0;
   return QP3D_OK;
@@ -894,9 +925,13 @@ static int harvest_events(qp3d_plan *p)
 // Enqueues time step k. `st` is the step index the kernels use (k itself, or a device index
 // plus an offset when the launches are being captured; k then only sets the slot parity).
 // A bare step (graph capture, recomputation) leaves out the snapshot and the instrumentation
-// events.
+// events. amp: the gradient pass skips tiles under the amplitude bound (when the block maxima
+// cover the current u) and the updates refresh the block maxima; off for the steps that other
+// objects drive (FWI recomputation, batches, the allocation-free entry point).
+constexpr int GBOUND_BLOCKS = 64;  // workgroups of the bound pass (k_gbound)
+
 template <class S>
-static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on)
+static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on, bool amp)
 {
   // on: launch everything on this stream (and without the plan's second stream) instead of on
   // the plan's own streams
@@ -921,10 +956,28 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on
   GradArgs<S> ga;
   UpdArgs<S> ua;
   make_args<S>(p, k & 1, st, ga, ua);
+  const bool use_amp = amp && p->amp_on && p->ampg.b && !p->spec;
+  if (!use_amp) {
+    ga.amp = AmpGrid{};
+    ga.argkey = nullptr;
+    ga.gcount = nullptr;
+    ua.amp = AmpGrid{};
+  }
   if (!p->spec) {
-    launch_gradmax<S>(p->tile_g, p->wl_grad.n, s0, ga);
+    int nwork = p->wl_grad.n;
+    if (use_amp) {
+      ga.work = p->wl_grad_amp.d;
+      ga.nwork = nwork = p->wl_grad_amp.n;
+      ga.skip = p->amp_ok ? 1 : 0;
+      // tile counts only on instrumented steps: one counter for every workgroup is a hot spot
+      if (!p->instrument || bare) ga.gcount = nullptr;
+      launch_gbound<S>(p->tile_g, GBOUND_BLOCKS, s0, ga);
+      HIP_TRY(hipGetLastError());
+    }
+    launch_gradmax<S>(p->tile_g, nwork, s0, ga);
     HIP_TRY(hipGetLastError());
   }
+  p->amp_ok = use_amp;  // u(k+1) is covered by the block maxima the updates write
   if (e) HIP_TRY(hipEventRecord(e[1], s0));
   if (two) {
     HIP_TRY(hipEventRecord(p->ev_grad, s0));
@@ -971,7 +1024,7 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on
     const int blocks = (int)std::min<long>((nb + 255) / 256, 4096);
     hipLaunchKernelGGL(k_rim_source<S>, dim3(blocks), dim3(256), 0, s0, (const T *)p->u[cur],
                        (T *)p->u[prev], p->geo, p->sb[0], p->sb[1], p->sb[2], p->sb[3], p->sb[4], p->sb[5],
-                       (const float *)p->sbox, (const float *)p->wav, st);
+                       (const float *)p->sbox, (const float *)p->wav, st, ua.amp);
     HIP_TRY(hipGetLastError());
   }
   if (!bare && cf.snapflag == 1 && k % cf.ntsnap == 0) {
@@ -994,16 +1047,19 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on
   return QP3D_OK;
 }
 
-static int launch_step(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on = nullptr)
+static int launch_step(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on, bool amp)
 {
   int rc = QP3D_OK;
-  with_storage(p->cfg.storage, [&](auto sp) { rc = launch_step_s<decltype(sp)>(p, k, st, bare, on); });
+  with_storage(p->cfg.storage, [&](auto sp) { rc = launch_step_s<decltype(sp)>(p, k, st, bare, on, amp); });
   return rc;
 }
 
-int qp3d_forward_step(qp3d_plan *p, int k) { return launch_step(p, k, Step{nullptr, k}, true); }
+int qp3d_forward_step(qp3d_plan *p, int k) { return launch_step(p, k, Step{nullptr, k}, true, nullptr, false); }
 
-int qp3d_forward_step_on(qp3d_plan *p, int k, Step st, hipStream_t on) { return launch_step(p, k, st, true, on); }
+int qp3d_forward_step_on(qp3d_plan *p, int k, Step st, hipStream_t on)
+{
+  return launch_step(p, k, st, true, on, false);
+}
 
 static bool snapshot_in(const qp3d_plan *p, int k0, int k1)
 {
@@ -1017,7 +1073,9 @@ static int capture_steps(qp3d_plan *p)
 {
   HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
   int rc = QP3D_OK;
-  for (int i = 0; i < GRAPH_LEN && rc == QP3D_OK; i++) rc = launch_step(p, i, Step{p->kdev, i}, true);
+  // every replay starts with the block maxima covering u (qp3d_plan_run checks amp_ok)
+  p->amp_ok = true;
+  for (int i = 0; i < GRAPH_LEN && rc == QP3D_OK; i++) rc = launch_step(p, i, Step{p->kdev, i}, true, nullptr, true);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(p->stream, &g);
   if (rc == QP3D_OK && e != hipSuccess) {
@@ -1056,7 +1114,9 @@ extern "C" int qp3d_plan_run(qp3d_plan *p, int k0, int k1)
   HIP_TRY(hipSetDevice(p->cfg.device));
   const bool graph = p->cfg.use_graph && !p->instrument;
   for (int k = k0; k < k1;) {
-    if (graph && (k & 1) == 0 && k + GRAPH_LEN <= k1 && !snapshot_in(p, k, k + GRAPH_LEN)) {
+    // a replay assumes the block maxima cover u (a direct step renews them first when not)
+    if (graph && (k & 1) == 0 && k + GRAPH_LEN <= k1 && !snapshot_in(p, k, k + GRAPH_LEN) &&
+        (p->amp_ok || !p->ampg.b)) {
       if (!p->gexec) {
         int rc = capture_steps(p);
         if (rc) return rc;
@@ -1066,7 +1126,7 @@ extern "C" int qp3d_plan_run(qp3d_plan *p, int k0, int k1)
       HIP_TRY(hipGraphLaunch(p->gexec, p->stream));
       k += GRAPH_LEN;
     } else {
-      int rc = launch_step(p, k, Step{nullptr, k}, false);
+      int rc = launch_step(p, k, Step{nullptr, k}, false, nullptr, true);
       if (rc) return rc;
       k++;
     }
@@ -1338,6 +1398,40 @@ extern "C" int qp3d_plan_gmax_mode(const qp3d_plan *p)
     return QP3D_ERR_ARG;
   }
   return p->spec ? 2 : 1;
+}
+
+// Amplitude bound of the gradient pass on (the default) or off; results are the same either way.
+extern "C" int qp3d_plan_set_amp_bound(qp3d_plan *p, int on)
+{
+  if (!p || on < 0 || on > 1) {
+    qp3d_set_error("plan_set_amp_bound: bad arguments");
+    return QP3D_ERR_ARG;
+  }
+  if (p->amp_on != (on == 1)) {
+    HIP_TRY(hipSetDevice(p->cfg.device));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->gexec = nullptr;
+    p->graph = nullptr;
+    p->amp_on = on == 1;
+    p->amp_ok = false;  // the block maxima were not kept up while off
+  }
+  return QP3D_OK;
+}
+
+// Gradient-pass tiles evaluated and skipped under the amplitude bound since the last reset
+// (out[0], out[1]), counted on instrumented steps (qp3d_plan_set_instrument); both 0 when the
+// bound is unavailable (gmax_mode 2).
+extern "C" int qp3d_plan_gradient_tiles(const qp3d_plan *p, unsigned long long *out)
+{
+  if (!p || !out) return QP3D_ERR_ARG;
+  out[0] = out[1] = 0;
+  if (!p->gcount) return QP3D_OK;
+  HIP_TRY(hipSetDevice(p->cfg.device));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  HIP_TRY(hipMemcpy(out, p->gcount, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  return QP3D_OK;
 }
 
 extern "C" int qp3d_plan_set_instrument(qp3d_plan *p, int on)

@@ -23,6 +23,7 @@ using namespace qp3d;
 namespace qp3d_impl {
 
 constexpr int GRAPH_LEN = 10;   // time steps per captured graph
+constexpr int AMP_ZCHUNK = 32;  // z planes per item of the gradient pass under the amplitude bound
 constexpr int EV_PER_STEP = 5;  // instrumentation events per step
 
 struct WorkList {
@@ -36,6 +37,7 @@ struct WorkList {
 using qp3d_impl::WorkList;
 using qp3d_impl::GRAPH_LEN;
 using qp3d_impl::EV_PER_STEP;
+using qp3d_impl::AMP_ZCHUNK;
 
 struct qp3d_plan {
   qp3d_config cfg{};
@@ -50,6 +52,7 @@ struct qp3d_plan {
   hipEvent_t ev_grad = nullptr, ev_pml = nullptr;         // cross-stream ordering, no timing
   hipEvent_t ev_in = nullptr, ev_out = nullptr;           // ordering with a caller's stream
   WorkList wl_grad, wl_int, wl_pml, wl_pmly;  // wl_pml: x and z faces; wl_pmly: y faces
+  WorkList wl_grad_amp;                       // gradient pass under the amplitude bound
   int tile_py = 1;
   long n_int_cells = 0, n_pml_cells = 0;
   hipStream_t stream = nullptr;
@@ -77,6 +80,14 @@ struct qp3d_plan {
   float4 *pmltab = nullptr;
   unsigned *gmax = nullptr;  // [nt]
   int *flag = nullptr;
+  // amplitude bound of the gradient pass (AmpGrid): block maxima of |u| in three buffers, the
+  // key of each step's largest tile in two; amp_ok: every cell of the current u is covered by
+  // its buffer (false after steps run without the bound, e.g. FWI recomputation)
+  AmpGrid ampg{};
+  unsigned long long *argkey = nullptr;
+  unsigned long long *gcount = nullptr;  // [2]: gradient tiles evaluated, skipped since reset
+  bool amp_ok = true;
+  bool amp_on = true;  // qp3d_plan_set_amp_bound
   bool spec = false;             // gmax_mode 2 (a batch turns it off for its plans)
   int in_batch = 0;              // 1 while a live batch drives the plan
   unsigned *fixmask = nullptr;   // gmax_mode 2: one bit per cell, all clear between steps
@@ -155,6 +166,11 @@ inline void make_args(const qp3d_plan *p, int prev, Step st, GradArgs<S> &ga, Up
   ga.step = st;
   ga.fixmask = p->fixmask;
   ga.gpred = p->gpred;
+  ga.amp = p->ampg;  // the caller turns skipping on (ga.skip) and may turn the bound off
+  ga.skip = 0;
+  ga.argkey = p->argkey;
+  ga.nwork = p->wl_grad.n;
+  ga.gcount = p->gcount;
   ua.g = p->geo;
   ua.c = p->coef;
   ua.ucur = (const T *)p->u[cur];
@@ -185,6 +201,7 @@ inline void make_args(const qp3d_plan *p, int prev, Step st, GradArgs<S> &ga, Up
   ua.rec = p->rec;
   ua.sz = p->cfg.sz;
   ua.work = nullptr;
+  ua.amp = p->ampg;
 }
 
 // Sizes of the state and model blocks (esz and the geometry set): fstride, ystride,
@@ -208,7 +225,7 @@ bool qp3d_on_device(const void *ptr);
 // Work lists (qp3d_plan.hip): a uniform list over [5,n-5)^3 for a tile shape.
 int qp3d_build_uniform_list(const qp3d_plan *p, int tile, int occ, WorkList &w);
 // The same items in host memory (logical order) for `capacity` resident workgroups.
-void qp3d_items_uniform(const qp3d_plan *p, int tile, int capacity, std::vector<int4> &items);
+void qp3d_items_uniform(const qp3d_plan *p, int tile, int capacity, std::vector<int4> &items, int zchunk = 0);
 // Interior update items (tile_i, cap_i workgroups) into it, PML items of the x and z faces
 // (tile_p, cap_p) into ip, of the y faces (tile_py, cap_py: a launch of their own, folded tiles
 // when y_fold(pml)) into ipy.

@@ -74,6 +74,16 @@ class Plan:
         check(self.lib.qp3d_plan_record(self.h, fptr(out.reshape(-1))))
         return out
 
+    def set_amp_bound(self, on=True):
+        """Amplitude bound of the gradient pass on or off (results are the same)."""
+        check(self.lib.qp3d_plan_set_amp_bound(self.h, int(bool(on))))
+
+    def gradient_tiles(self):
+        """(evaluated, skipped) gradient-pass tiles since the last reset."""
+        out = (C.c_ulonglong * 2)()
+        check(self.lib.qp3d_plan_gradient_tiles(self.h, out))
+        return int(out[0]), int(out[1])
+
     def field(self, which=0):
         s = self.s
         out = np.zeros((s.nz, s.nx, s.ny), np.float32)

@@ -199,6 +199,15 @@ int qp3d_plan_step_predictions(const qp3d_plan *p, float *out);
 /* How the plan's steps obtain G (qp3d_config.gmax_mode after defaults: 1 or 2; 1 while the
  * plan belongs to a batch), or a QP3D_* error. */
 int qp3d_plan_gmax_mode(const qp3d_plan *p);
+/* Amplitude bound of the gradient pass (gmax_mode 1; no reference counterpart): the update
+ * kernels record max |u| per block of 8 x 8 x 64 cells, a small pass evaluates exactly the tile
+ * that held the previous step's largest gradient component, and the pass skips every tile whose
+ * bound 2 sum|c1|/d max|u| is below the G found so far, without reading it. G, and so every
+ * result, is the same as with the bound off. On by default; the setter turns it off (0) or on
+ * (1). qp3d_plan_gradient_tiles: tiles evaluated (out[0]) and skipped (out[1]) since the
+ * last reset, counted on instrumented steps (qp3d_plan_set_instrument). */
+int qp3d_plan_set_amp_bound(qp3d_plan *p, int on);
+int qp3d_plan_gradient_tiles(const qp3d_plan *p, unsigned long long *out);
 /* 1: bracket every stencil launch with HIP events on the plan's stream; 2: only the interior
  * update (two events per step instead of four); 0: none. Resets the accumulated times. */
 int qp3d_plan_set_instrument(qp3d_plan *p, int on);
