@@ -197,7 +197,7 @@ extern "C" int qp3d_fwi_forward(qp3d_fwi *f)
     if (k >= klast)
       HIP_TRY(hipMemcpyAsync(f->ubuf + (size_t)(k - klast) * N, p->u[(k & 1) ^ 1], sizeof(float) * N,
                              hipMemcpyDeviceToDevice, p->stream));
-    if ((rc = qp3d_forward_step(p, k))) return rc;
+    if ((rc = qp3d_forward_step_fwi(p, k, false))) return rc;
   }
   f->ubuf_seg = f->nseg - 1;
   f->gen = p->upload_gen;
@@ -289,7 +289,7 @@ extern "C" int qp3d_fwi_gradient(qp3d_fwi *f, const float *resid, double *g_vp, 
         const int cur = (k & 1) ^ 1;  // slot of u_k in the forward state
         HIP_TRY(hipMemcpyAsync(f->ubuf + (size_t)(k - k0) * N, p->u[cur], sizeof(float) * N,
                                hipMemcpyDeviceToDevice, p->stream));
-        if ((rc = qp3d_forward_step(p, k))) return rc;
+        if ((rc = qp3d_forward_step_fwi(p, k, true))) return rc;  // G_k from the forward run
       }
       f->ubuf_seg = s;
     }

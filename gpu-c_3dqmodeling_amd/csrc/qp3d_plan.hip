@@ -931,7 +931,7 @@ static int harvest_events(qp3d_plan *p)
 constexpr int GBOUND_BLOCKS = 64;  // workgroups of the bound pass (k_gbound)
 
 template <class S>
-static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on, bool amp)
+static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on, bool amp, bool gknown = false)
 {
   // on: launch everything on this stream (and without the plan's second stream) instead of on
   // the plan's own streams
@@ -963,7 +963,7 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on
     ga.gcount = nullptr;
     ua.amp = AmpGrid{};
   }
-  if (!p->spec) {
+  if (!p->spec && !gknown) {
     int nwork = p->wl_grad.n;
     if (use_amp) {
       ga.work = p->wl_grad_amp.d;
@@ -1047,11 +1047,21 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on
   return QP3D_OK;
 }
 
-static int launch_step(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on, bool amp)
+static int launch_step(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on, bool amp, bool gknown = false)
 {
   int rc = QP3D_OK;
-  with_storage(p->cfg.storage, [&](auto sp) { rc = launch_step_s<decltype(sp)>(p, k, st, bare, on, amp); });
+  with_storage(p->cfg.storage,
+               [&](auto sp) { rc = launch_step_s<decltype(sp)>(p, k, st, bare, on, amp, gknown); });
   return rc;
+}
+
+// FWI: the first forward run with the amplitude bound, or a recomputed step whose G_k the first
+// run already left in gmax[k] (no gradient pass; the update reads it)
+int qp3d_forward_step_fwi(qp3d_plan *p, int k, bool recompute)
+{
+  if (recompute && p->spec) return launch_step(p, k, Step{nullptr, k}, true, nullptr, false);
+  return recompute ? launch_step(p, k, Step{nullptr, k}, true, nullptr, false, true)
+                   : launch_step(p, k, Step{nullptr, k}, true, nullptr, true);
 }
 
 int qp3d_forward_step(qp3d_plan *p, int k) { return launch_step(p, k, Step{nullptr, k}, true, nullptr, false); }

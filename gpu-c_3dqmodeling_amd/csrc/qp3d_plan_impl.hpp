@@ -236,6 +236,9 @@ int qp3d_upload_items(WorkList &w, const std::vector<int4> &logical, bool batch)
 int qp3d_cu_count(const qp3d_plan *p);
 // Enqueues forward time step k with direct launches (no capture, no snapshot, no events).
 int qp3d_forward_step(qp3d_plan *p, int k);
+// FWI steps: recompute = false, the forward run with the amplitude bound; true, a recomputed
+// step that takes G_k from gmax[k] (left there by the forward run) instead of a gradient pass.
+int qp3d_forward_step_fwi(qp3d_plan *p, int k, bool recompute);
 // The same on stream `on` (one stream), with the step index the kernels read given by `st`.
 int qp3d_forward_step_on(qp3d_plan *p, int k, Step st, hipStream_t on);
 // Validates a configuration (no device access); `who` prefixes the error message.
