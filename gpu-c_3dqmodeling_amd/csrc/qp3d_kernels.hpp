@@ -179,14 +179,20 @@ struct AmpGrid {
 __device__ __forceinline__ void amp_flush(const AmpGrid &ag, int kstep, int za, int zb, int x0, int x1, int ylane,
                                           unsigned &am)
 {
-  unsigned v = am;
-#pragma unroll
-  for (int off = 16; off >= 1; off >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, off, 64));
+  // half-wavefront maxima with DPP: prefix maxima within each 16-lane row (row_shr 1, 2, 4, 8),
+  // then row 0's maximum into row 1 and row 2's into row 3 (row_bcast:15), so lanes 31 and 63
+  // hold the maxima of lanes 0..31 and 32..63 (lanes without a source read 0, the identity)
+  int v = (int)am;
+  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));
+  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));
+  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));
+  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));
+  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));
   const int yb = min(ylane >> AB_Y, ag.nby - 1);
   // targets t = 0..3: half 0 first / last lane's y block, half 1 first / last
   const int ty[4] = {__builtin_amdgcn_readlane(yb, 0), __builtin_amdgcn_readlane(yb, 31),
                      __builtin_amdgcn_readlane(yb, 32), __builtin_amdgcn_readlane(yb, 63)};
-  const unsigned tv[2] = {(unsigned)__builtin_amdgcn_readlane((int)v, 0), (unsigned)__builtin_amdgcn_readlane((int)v, 32)};
+  const unsigned tv[2] = {(unsigned)__builtin_amdgcn_readlane(v, 31), (unsigned)__builtin_amdgcn_readlane(v, 63)};
   const int bz0 = za >> AB_Z, nbz = ((zb - 1) >> AB_Z) - bz0 + 1;
   const int bx0 = x0 >> AB_X, nbx = ((x1 - 1) >> AB_X) - bx0 + 1;
   const int l = __lane_id();
@@ -1017,7 +1023,7 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
         }
         un = S::rnd(un);
         S::st(a.unext + zo, o, un);
-        if (amp_on) am = max(am, __float_as_uint(fabsf(un)));
+        am = max(am, __float_as_uint(fabsf(un)));  // unconditional: a branch per cell costs more
         if (rec_plane && x >= pml && x < nx - pml && y >= pml && y < ny - pml)
           rec[(x - pml) * (ny - 2 * pml) + (y - pml)] = un;
       }
