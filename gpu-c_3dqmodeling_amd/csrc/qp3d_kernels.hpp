@@ -279,6 +279,8 @@ struct GradArgs {
   unsigned long long *argkey;  // [2]: (tile maximum bits << 32 | item) of the step's largest tile
   int nwork;          // items in `work`
   unsigned long long *gcount;  // [2]: tiles evaluated, tiles skipped (instrumented steps; else null)
+  unsigned *gseed;    // [2] per step parity: the exact maximum the bound pass found (float bits)
+  unsigned *gseed_out;  // the bound pass: where its tiles' maximum goes (else null)
 };
 
 // A lane's fixed position in its tile and its in-plane offsets.
@@ -522,7 +524,8 @@ __device__ __forceinline__ void mark_cell(unsigned *mask, unsigned cell) { atomi
 // lane saw a non-finite gradient (nonfinite, or m above the float range).
 template <class T>
 __device__ __forceinline__ void reduce_gmax(float m, bool nonfinite, int tx, int ty, unsigned *gmax, int *flag,
-                                            unsigned long long *argkey = nullptr, int item = -1)
+                                            unsigned long long *argkey = nullptr, int item = -1,
+                                            unsigned *seed = nullptr)
 {
   __shared__ float red[T::WX];
   __shared__ int redbad[T::WX];
@@ -549,17 +552,18 @@ __device__ __forceinline__ void reduce_gmax(float m, bool nonfinite, int tx, int
     if (bb) atomicOr(flag, 1);
     // the tile's maximum with its item: the next step's bound pass starts from that tile
     if (argkey && item >= 0) atomicMax(argkey, ((unsigned long long)__float_as_uint(v) << 32) | (unsigned)item);
+    if (seed) atomicMax(seed, __float_as_uint(v));
   }
 }
 
 // Amplitude bound of a gradient tile (see AmpGrid): true when every gradient component of the
-// tile is provably below the G_k already found (*gmax). Uniform over the workgroup (the values
-// pass through LDS).
+// tile is provably below the exact maximum the bound pass found for this step (the seed, fixed
+// during the pass). Every wavefront decides on its own from the same loads, so the decision is
+// the same in all of them without LDS or a barrier.
 template <class S, class T>
 __device__ __forceinline__ bool amp_skip(const GradArgs<S> &a, int x0, int x1, int y0, int y1, int z0, int z1,
                                          int kstep)
 {
-  __shared__ unsigned amp_red[T::WX + 1];
   const AmpGrid &ag = a.amp;
   const int zb0 = max(0, (z0 - H) >> AB_Z), zb1 = min(ag.nbz - 1, (z1 - 1 + H) >> AB_Z);
   const int xb0 = max(0, (x0 - H) >> AB_X), xb1 = min(ag.nbx - 1, (x1 - 1 + H) >> AB_X);
@@ -567,23 +571,21 @@ __device__ __forceinline__ bool amp_skip(const GradArgs<S> &a, int x0, int x1, i
   const int nxb = xb1 - xb0 + 1, nyb = yb1 - yb0 + 1;
   const int nb = (zb1 - zb0 + 1) * nxb * nyb;
   const unsigned *bb = ag.buf(kstep);
-  const int tid = threadIdx.y * TY + threadIdx.x;
   unsigned m = 0u;
-  for (int i = tid; i < nb; i += T::NT) {
+  for (int i = __lane_id(); i < nb; i += TY) {
     const int yb = i % nyb, t = i / nyb, xb = t % nxb, zb = t / nxb;
     m = max(m, bb[((unsigned)(zb0 + zb) * ag.nbx + (unsigned)(xb0 + xb)) * ag.nby + (unsigned)(yb0 + yb)]);
   }
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off, 64));
-  if (threadIdx.x == 0) amp_red[threadIdx.y] = m;
-  // any value of *gmax ever stored is the maximum of exactly computed components, so a cached
-  // (possibly older) value is as valid a lower bound as the newest, and costs no trip past L2
-  if (tid == 0) amp_red[T::WX] = __hip_atomic_load(a.gmax + kstep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-  __syncthreads();
-  unsigned M = amp_red[0];
-#pragma unroll
-  for (int w = 1; w < T::WX; w++) M = max(M, amp_red[w]);
-  const float Gk = __uint_as_float(amp_red[T::WX]);
+  // wavefront maximum into lane 63 (row prefix maxima, then rows 0 -> 1, 2 -> 3, 1 -> 2 and 3)
+  int v = (int)m;
+  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));
+  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));
+  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));
+  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));
+  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));
+  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));
+  const unsigned M = (unsigned)__builtin_amdgcn_readlane(v, 63);
+  const float Gk = __uint_as_float(a.gseed[kstep & 1]);
   // |sum_m g_m (u+m - u-m)| <= 2 sum_m |g_m| max|u|; the factor covers the rounding of the sum
   float fx = 0.f, fy = 0.f, fz = 0.f;
 #pragma unroll
@@ -705,7 +707,7 @@ __device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int k
   for (; zb < z1; zb += 2 * QN) Unroll<0, 2 * QN>::run(plane, z1 - zb);
   if constexpr (!MARK)
     reduce_gmax<T>(m, !(nanacc == 0.f), L.tx, L.ty, a.gmax + kstep, a.flag, a.argkey ? a.argkey + (kstep & 1) : nullptr,
-                   item);
+                   item, a.gseed_out);
 }
 
 // Batched launches (several grids in one launch): item .z packs z0 | z1 << 16 and .w is the
@@ -736,7 +738,10 @@ __global__ __launch_bounds__(T::NT) void k_gbound(GradArgs<S> a)
   const unsigned tid = threadIdx.y * TY + threadIdx.x;
   unsigned *clr = a.amp.buf(kstep + 2);
   for (unsigned i = blockIdx.x * T::NT + tid; i < a.amp.nblk; i += gridDim.x * T::NT) clr[i] = 0u;
-  if (blockIdx.x == 0 && tid == 0) a.argkey[kstep & 1] = 0ull;
+  if (blockIdx.x == 0 && tid == 0) {
+    a.argkey[kstep & 1] = 0ull;
+    a.gseed[(kstep + 1) & 1] = 0u;  // the next step's seed; this step's was cleared by step k-1
+  }
   const unsigned long long key = a.argkey[(kstep + 1) & 1];  // written by step k-1
   int item = (int)(key & 0xffffffffull);
   if (item < 0 || item >= a.nwork) item = 0;
@@ -750,6 +755,7 @@ __global__ __launch_bounds__(T::NT) void k_gbound(GradArgs<S> a)
   GradArgs<S> b = a;
   b.skip = 0;
   b.argkey = nullptr;
+  b.gseed_out = a.gseed + (kstep & 1);
   gradmax_tile<S, T>(b, w, kstep, sm);
 }
 

@@ -456,10 +456,11 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
     ag.nby = (c.ny + (1 << AB_Y) - 1) >> AB_Y;
     ag.nblk = (unsigned)ag.nbz * ag.nbx * ag.nby;
     if ((rc = dmalloc((void **)&ag.b, sizeof(unsigned) * 3 * (size_t)ag.nblk, "block maxima"))) return fail(rc);
-    if ((rc = dmalloc((void **)&p->argkey, 4 * sizeof(unsigned long long), "largest-tile keys"))) return fail(rc);
+    // keys of the largest tile [2], tile counters [2], bound-pass maxima (two words)
+    if ((rc = dmalloc((void **)&p->argkey, 5 * sizeof(unsigned long long), "largest-tile keys"))) return fail(rc);
     p->gcount = p->argkey + 2;
     if (hipMemset(ag.b, 0, sizeof(unsigned) * 3 * (size_t)ag.nblk) != hipSuccess ||
-        hipMemset(p->argkey, 0, 4 * sizeof(unsigned long long)) != hipSuccess) {
+        hipMemset(p->argkey, 0, 5 * sizeof(unsigned long long)) != hipSuccess) {
       qp3d_set_error("plan_create: clearing the block maxima failed");
       return fail(QP3D_ERR_HIP);
     }
@@ -888,7 +889,7 @@ extern "C" int qp3d_plan_reset(qp3d_plan *p)
   if (p->rec) HIP_TRY(hipMemsetAsync(p->rec, 0, sizeof(float) * (size_t)p->cfg.nt * p->hx * p->hy, p->stream));
   if (p->ampg.b) {
     HIP_TRY(hipMemsetAsync(p->ampg.b, 0, sizeof(unsigned) * 3 * (size_t)p->ampg.nblk, p->stream));
-    HIP_TRY(hipMemsetAsync(p->argkey, 0, 4 * sizeof(unsigned long long), p->stream));
+    HIP_TRY(hipMemsetAsync(p->argkey, 0, 5 * sizeof(unsigned long long), p->stream));
   }
   p->amp_ok = true;  // u = 0 everywhere, and so are the block maxima
   p->nsnap_taken = 0;

@@ -171,6 +171,8 @@ inline void make_args(const qp3d_plan *p, int prev, Step st, GradArgs<S> &ga, Up
   ga.argkey = p->argkey;
   ga.nwork = p->wl_grad.n;
   ga.gcount = p->gcount;
+  ga.gseed = p->argkey ? (unsigned *)(p->argkey + 4) : nullptr;
+  ga.gseed_out = nullptr;
   ua.g = p->geo;
   ua.c = p->coef;
   ua.ucur = (const T *)p->u[cur];
