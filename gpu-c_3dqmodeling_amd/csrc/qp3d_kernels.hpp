@@ -167,6 +167,7 @@ struct AmpGrid {
   unsigned *b;   // [3][nbz][nbx][nby]
   int nbz, nbx, nby;
   unsigned nblk;  // nbz * nbx * nby
+  unsigned char *q;  // [nbz][nbx][nby] quiet map of the current step (k_quiet)
   __device__ __forceinline__ unsigned *buf(int k) const { return b + (unsigned)(k % 3) * nblk; }
 };
 
@@ -710,6 +711,32 @@ __device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int k
                    item, a.gseed_out);
 }
 
+// Quiet map of step k (after its gradient pass): a block is quiet when the amplitude bound over
+// its 3 x 3 x 3 block neighbourhood (every cell within 5 of it) keeps |grad u| = sqrt(ux^2 + uy^2
+// + uz^2) of each of its cells below the scalar operator's threshold rd(0.01 G_k): there S = 1
+// without reading eps - del. The bound on each component is the gradient pass's, with the
+// update's first derivatives (sum c1 (u+m - u-m), then 1/d) and a margin for their rounding and
+// the hardware square root.
+static __global__ void k_quiet(AmpGrid ag, Coef c, const unsigned *gmax, Step st)
+{
+  const int kstep = st.get();
+  const unsigned *bb = ag.buf(kstep);
+  const float thr = s_threshold(__uint_as_float(gmax[kstep]));
+  float s1 = 0.f;
+#pragma unroll
+  for (int k = 0; k < H; k++) s1 += fabsf(c.c1[k]);
+  const float fac = 2.f * s1 * fmaxf(c.ix, fmaxf(c.iy, c.iz)) * 1.7320509f * (1.f + 1.0e-4f);
+  for (unsigned b = blockIdx.x * blockDim.x + threadIdx.x; b < ag.nblk; b += gridDim.x * blockDim.x) {
+    const int by = (int)(b % ag.nby), t = (int)(b / ag.nby), bx = t % ag.nbx, bz = t / ag.nbx;
+    unsigned m = 0u;
+    for (int z = max(0, bz - 1); z <= min(ag.nbz - 1, bz + 1); z++)
+      for (int x = max(0, bx - 1); x <= min(ag.nbx - 1, bx + 1); x++)
+        for (int y = max(0, by - 1); y <= min(ag.nby - 1, by + 1); y++)
+          m = max(m, bb[((unsigned)z * ag.nbx + x) * ag.nby + y]);
+    ag.q[b] = (m < 0x7f800000u && __uint_as_float(m) * fac < thr) ? 1 : 0;
+  }
+}
+
 // Batched launches (several grids in one launch): item .z packs z0 | z1 << 16 and .w is the
 // index of the grid's argument block.
 __device__ __forceinline__ int4 batch_item(const int4 *work, int &shot)
@@ -822,7 +849,8 @@ struct CellLoad {
   float v2[T::RX], e[T::RX], ed[T::RX];
   float st[T::RX][NS];  // interior: st[0] = u_prev; PML: w1p w1c pp pc w3p w3c w21
 
-  __device__ __forceinline__ void issue(const CellSrc<S> &c, const unsigned (&offs)[T::RX], int z)
+  // ed_on false (uniform): every cell of the plane keeps S = 1, which does not read eps - del
+  __device__ __forceinline__ void issue(const CellSrc<S> &c, const unsigned (&offs)[T::RX], int z, bool ed_on = true)
   {
     const size_t zo = (size_t)z * c.pln;
     const float *pv = c.v2 + zo, *pe = c.e + zo, *pd = c.ed + zo;
@@ -830,7 +858,13 @@ struct CellLoad {
     for (int r = 0; r < T::RX; r++) {
       v2[r] = ld_model(pv, offs[r]);
       e[r] = ld_model(pe, offs[r]);
-      ed[r] = ld_model(pd, offs[r]);
+    }
+    if (ed_on) {
+#pragma unroll
+      for (int r = 0; r < T::RX; r++) ed[r] = ld_model(pd, offs[r]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < T::RX; r++) ed[r] = 0.f;
     }
     if constexpr (!PML) {
       const typename S::T *pu = c.uprev + zo;
@@ -923,8 +957,36 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
   // per-cell loads, double-buffered: the interior march (unrolled by 22) addresses the buffer
   // of plane `it` as CL[it & 1] at compile time; the rolled PML march copies the next into
   // the current
+  // Quiet z blocks of this tile (when the quiet map is on): every cell there has |grad u| below
+  // the threshold, so S = 1 and eps - del is not loaded. Bit i of qm: z block (z0 >> AB_Z) + i;
+  // lanes take one z block each and AND the map over the tile's x and y blocks (a folded tile:
+  // over both of its y strips).
+  unsigned long long qm[2] = {0ull, 0ull};
+  const int zq0 = z0 >> AB_Z;
+  if (a.amp.q) {
+    const int nzb = ((z1 - 1) >> AB_Z) - zq0 + 1;
+    const int xb0 = x0 >> AB_X, xb1 = (x1 - 1) >> AB_X;
+    const int ya0 = y0 >> AB_Y, ya1 = (y1 - 1) >> AB_Y;
+    const int yh = g.ny - g.pml;  // a folded tile's second strip starts here
+    const int yc0 = fold ? yh >> AB_Y : ya0, yc1 = fold ? (yh + (y1 - y0) - 1) >> AB_Y : ya1;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int i = __lane_id() + TY * h;
+      bool q = i < nzb;
+      const unsigned row = (unsigned)(zq0 + i) * a.amp.nbx;
+      for (int xb = xb0; q && xb <= xb1; xb++) {
+        for (int yb = ya0; q && yb <= ya1; yb++) q = a.amp.q[(row + (unsigned)xb) * a.amp.nby + (unsigned)yb] != 0;
+        for (int yb = yc0; q && yb <= yc1; yb++) q = a.amp.q[(row + (unsigned)xb) * a.amp.nby + (unsigned)yb] != 0;
+      }
+      qm[h] = __ballot(q);
+    }
+  }
+  auto ed_needed = [&](int z) {
+    const int i = (z >> AB_Z) - zq0;
+    return !(((i < 64 ? qm[0] >> i : qm[1] >> (i - 64)) & 1ull) != 0ull);
+  };
   CellLoad<S, T, PML> CL[2];
-  CL[0].issue(cs, coff, z0);
+  CL[0].issue(cs, coff, z0, ed_needed(z0));
   float centre[T::RX];
 #pragma unroll
   for (int r = 0; r < T::RX; r++) {
@@ -949,7 +1011,7 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
       CellLoad<S, T, PML> &Cn = CL[PML ? 1 : ((it + 1) & 1)];
       if (more) {
         P.issue(a.ucur, g, L, z + 1);
-        Cn.issue(cs, coff, z + 1);
+        Cn.issue(cs, coff, z + 1, ed_needed(z + 1));
       }
       // interior: unrolled by 22, the buffer of plane `it` is it & 1 at compile time
       float (*s)[T::LY] = sm[PML ? b : (it & 1)];

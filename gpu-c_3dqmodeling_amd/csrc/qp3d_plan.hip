@@ -456,6 +456,7 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
     ag.nby = (c.ny + (1 << AB_Y) - 1) >> AB_Y;
     ag.nblk = (unsigned)ag.nbz * ag.nbx * ag.nby;
     if ((rc = dmalloc((void **)&ag.b, sizeof(unsigned) * 3 * (size_t)ag.nblk, "block maxima"))) return fail(rc);
+    if ((rc = dmalloc((void **)&ag.q, (size_t)ag.nblk, "quiet map"))) return fail(rc);
     // keys of the largest tile [2], tile counters [2], bound-pass maxima (two words)
     if ((rc = dmalloc((void **)&p->argkey, 5 * sizeof(unsigned long long), "largest-tile keys"))) return fail(rc);
     p->gcount = p->argkey + 2;
@@ -514,6 +515,7 @@ extern "C" int qp3d_plan_destroy(qp3d_plan *p)
   (void)hipFree(p->kdev);
   (void)hipFree(p->wav);
   (void)hipFree(p->ampg.b);
+  (void)hipFree(p->ampg.q);
   (void)hipFree(p->argkey);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
@@ -977,6 +979,13 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on
     }
     launch_gradmax<S>(p->tile_g, nwork, s0, ga);
     HIP_TRY(hipGetLastError());
+    if (use_amp && ga.skip) {
+      // G_k is complete: mark the blocks where S = 1 for the interior update
+      hipLaunchKernelGGL(k_quiet, dim3((p->ampg.nblk + 255) / 256), dim3(256), 0, s0, p->ampg, p->coef,
+                         (const unsigned *)p->gmax, st);
+      HIP_TRY(hipGetLastError());
+      ua.amp.q = p->ampg.q;
+    }
   }
   p->amp_ok = use_amp;  // u(k+1) is covered by the block maxima the updates write
   if (e) HIP_TRY(hipEventRecord(e[1], s0));

@@ -204,6 +204,7 @@ inline void make_args(const qp3d_plan *p, int prev, Step st, GradArgs<S> &ga, Up
   ua.sz = p->cfg.sz;
   ua.work = nullptr;
   ua.amp = p->ampg;
+  ua.amp.q = nullptr;  // the step turns the quiet map on after k_quiet has filled it
 }
 
 // Sizes of the state and model blocks (esz and the geometry set): fstride, ystride,
