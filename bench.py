@@ -9,11 +9,14 @@ shots replay the captured 10-step hipGraph (--graph 0: direct launches).
 
 metric: Gcell-updates/s = updated cells (nx-10)(ny-10)(nz-10) x time steps / wall time.
 roofline: the dominant launch, the interior update kernel: its algorithmic bytes per launch
-(24 B per interior cell: u_cur, u_prev, three model fields, u_next) over its mean launch time,
-from HIP events on the plan's stream around every interior-update launch of one instrumented
-shot run right after the timed shots (replayed graphs carry no per-launch events).
-roofline_step: the whole time step: algorithmic bytes of all its launches (28 B per interior
-cell, 68 B per PML cell) over the timed shots' wall time per time step.
+(24 B per interior cell updated: u_cur, u_prev, three model fields, u_next; 20 B where the
+quiet map spares the eps - del load; 0 B for cells left out at rest), averaged over one
+instrumented shot, over its mean launch time, from HIP events on the plan's stream around every
+interior-update launch of a shot run right after the timed shots (replayed graphs carry no
+per-launch events). config.roofline_all_cells counts every cell at 24 B.
+roofline_step: the whole time step: algorithmic bytes of all its launches (the same per-cell
+counts; 64 B per PML cell; 4 B per cell of the gradient tiles evaluated) over the timed shots'
+wall time per time step.
 cpu_baseline: the CPU oracle (the reference's arithmetic, bit-exact with its CPU kernels)
 on the host's CPU share, on a 20-step window that starts from the GPU's state at step 300;
 config.cpu_baseline_single_thread: the same on one thread (the reference's CPU path is
@@ -362,6 +365,7 @@ def main():
     ap.add_argument("--tile-pml", type=int, default=0)
     ap.add_argument("--tile-pml-y", type=int, default=0)
     ap.add_argument("--streams", type=int, default=0, help="2: interior and PML updates concurrent; 1: serial")
+    ap.add_argument("--zchunk", type=int, default=0, help="z planes per update item (0: the plan's choice)")
     ap.add_argument("--gmax-mode", type=int, default=0,
                     help="1: gradient pass before the update; 2: predicted threshold with fix-up; 0: default")
     ap.add_argument("--storage", default="fp32", choices=["fp32", "bf16", "fp16"],
@@ -404,7 +408,7 @@ def main():
     ntsnap = 500
     plan = qp3d.Plan(s, snapflag=1, ntsnap=ntsnap, use_graph=bool(args.graph), tile_update=args.tile_update,
                      tile_grad=args.tile_grad, tile_pml=args.tile_pml, tile_pml_y=args.tile_pml_y,
-                     streams=args.streams, storage=args.storage, gmax_mode=args.gmax_mode)
+                     streams=args.streams, storage=args.storage, gmax_mode=args.gmax_mode, zchunk=args.zchunk)
     log("model uploaded")
     if not args.amp_bound:
         plan.set_amp_bound(False)
@@ -437,6 +441,7 @@ def main():
     plan.sync()
     kt = plan.kernel_times()
     gtiles = plan.gradient_tiles()  # evaluated, skipped over the instrumented shot
+    ucells = plan.update_cells()  # per update kind: cells, at rest, without eps - del (same shot)
     kt["update_interior"] = kt_timed["update_interior"]
     kb = plan.step_bytes()
     plan.set_instrument(0)
@@ -449,10 +454,31 @@ def main():
 
     cells = s.n_upd * s.nt * args.steps
     value = cells / wall / 1e9
-    achieved = gbs["update_interior"]
-    step_bytes = kb["gradmax"] + kb["update_interior"] + kb["update_pml"]
     ms_time_step = wall / args.steps / s.nt * 1e3
-    step_gbs = step_bytes / (ms_time_step * 1e-3) / 1e9
+    # Bytes the launches had to move: cells at rest are left out (0 B), quiet cells skip the
+    # 4-byte eps - del load, the gradient pass reads only the tiles it evaluates. Without the
+    # amplitude bound (or with gmax_mode 2) the counters stay 0 and every cell counts in full.
+    b_int = kb["update_interior"] / kb["cells_interior"]
+    b_pml = kb["update_pml"] / kb["cells_pml"] if kb["cells_pml"] else 0.0
+    int_c = ucells[0]
+    pml_c = ucells[1] + ucells[2]
+    counted = int_c[0] > 0
+    if counted:
+        moved_int = ((int_c[0] - int_c[1]) * b_int - int_c[2] * 4.0) / s.nt
+        moved_pml = ((pml_c[0] - pml_c[1]) * b_pml - pml_c[2] * 4.0) / s.nt
+        geval = gtiles[0] / max(1, gtiles[0] + gtiles[1])
+    else:
+        moved_int, moved_pml, geval = kb["update_interior"], kb["update_pml"], 1.0
+    moved_grad = kb["gradmax"] * geval
+    achieved = moved_int / (kt["update_interior"] * 1e-3) / 1e9
+    achieved_all = gbs["update_interior"]
+    step_bytes = kb["gradmax"] + kb["update_interior"] + kb["update_pml"]
+    step_moved = moved_grad + moved_int + moved_pml
+    step_gbs = step_moved / (ms_time_step * 1e-3) / 1e9
+    step_gbs_all = step_bytes / (ms_time_step * 1e-3) / 1e9
+    upd_total = int(int_c[0] + pml_c[0])
+    rest_frac = float(int_c[1] + pml_c[1]) / max(1, upd_total)
+    quiet_frac = float(int_c[2] + pml_c[2]) / max(1, upd_total)
     traffic = step_traffic = None
     if os.path.exists(TRAFFIC_FILE):
         try:
@@ -502,14 +528,22 @@ def main():
                    "gradient_amp_bound": bool(args.amp_bound) and gmode == 1,
                    "gradient_tiles_evaluated_skipped": list(gtiles),
                    "gradient_tiles_skipped_frac": round(gtiles[1] / max(1, gtiles[0] + gtiles[1]), 4),
-                   "roofline_kernel": f"k_update<interior> ({kb['update_interior'] / kb['cells_interior']:.0f} B per "
-                                      f"interior cell, {kb['cells_interior']:.0f} cells per launch)",
+                   "update_cells_at_rest_frac": round(rest_frac, 4) if counted else None,
+                   "update_cells_quiet_frac": round(quiet_frac, 4) if counted else None,
+                   "update_cells_at_rest_frac_interior": round(int_c[1] / max(1, int_c[0]), 4) if counted else None,
+                   "roofline_kernel": f"k_update<interior> ({b_int:.0f} B per interior cell updated, "
+                                      f"{b_int - 4:.0f} B without eps - del, 0 B at rest; "
+                                      f"{moved_int:.0f} B per launch over the shot, "
+                                      f"{kb['cells_interior']:.0f} cells per launch)",
+                   "roofline_all_cells": {"achieved": round(achieved_all, 1), "frac": round(achieved_all / HBM_PEAK_GBS, 4),
+                                          "step_achieved": round(step_gbs_all, 1),
+                                          "step_frac": round(step_gbs_all / HBM_PEAK_GBS, 4), "step_bytes": step_bytes},
                    "cpu_baseline_single_thread": cpu1},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "roofline_step": {"bound": "hbm", "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(step_gbs / HBM_PEAK_GBS, 4), "traffic": step_traffic,
-                          "bytes": step_bytes},
+                          "bytes": round(step_moved)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

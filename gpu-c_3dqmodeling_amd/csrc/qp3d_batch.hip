@@ -109,6 +109,8 @@ static int upload_args(qp3d_batch *b)
       g[i].amp = AmpGrid{};
       g[i].argkey = nullptr;
       g[i].skip = 0;
+      g[i].gcount = nullptr;
+      g[i].gseed = g[i].gseed_out = nullptr;
       u[i].amp = AmpGrid{};
     }
     HIP_TRY(hipMalloc(&b->args_g[par], sizeof(GradArgs<S>) * n));

@@ -162,13 +162,31 @@ struct Step {
 // most 2 sum_m |g_m| times max |u| within 5 cells: a gradient tile whose blocks (with that
 // halo) bound every component below a value of G_k already computed exactly cannot hold the
 // maximum, and the pass skips it without loading u. G_k stays exact. b == nullptr: off.
+//
+// Counters of instrumented steps: CNT_SLOTS slots of 16 words, one 128-byte line each, a
+// workgroup adding into slot blockIdx.x % CNT_SLOTS (one address for every workgroup of a launch
+// serialises its atomics): words 0, 1 gradient tiles evaluated, skipped; 2 + 3 * kind + j update
+// cells in items (j = 0), left out at rest (1), without eps - del (2).
+constexpr int CNT_SLOTS = 32, CNT_STRIDE = 16;
+__device__ __forceinline__ unsigned long long *cnt_slot(unsigned long long *c)
+{
+  return c + (blockIdx.x % CNT_SLOTS) * CNT_STRIDE;
+}
 constexpr int AB_Z = 3, AB_X = 3, AB_Y = 5;  // log2 of the block extents
 struct AmpGrid {
   unsigned *b;   // [3][nbz][nbx][nby]
   int nbz, nbx, nby;
   unsigned nblk;  // nbz * nbx * nby
-  unsigned char *q;  // [nbz][nbx][nby] quiet map of the current step (k_quiet)
+  // [nbx][nby][nbz] map of the current step (k_quiet): bit 0 quiet (S = 1 in every cell), bit 1
+  // touched (u has been nonzero within the block's neighbourhood at some step since the reset;
+  // a block without it is at rest). z fastest: a wavefront reading one z block per lane loads one
+  // or two cache lines.
+  unsigned char *q;
   __device__ __forceinline__ unsigned *buf(int k) const { return b + (unsigned)(k % 3) * nblk; }
+  __device__ __forceinline__ unsigned qi(int bz, int bx, int by) const
+  {
+    return ((unsigned)bx * (unsigned)nby + (unsigned)by) * (unsigned)nbz + (unsigned)bz;
+  }
 };
 
 // Per-lane running maximum of |u_next| (float bits: inf and NaN order above every finite value)
@@ -262,6 +280,10 @@ struct UpdArgs {
   int sz;                                   // receiver plane
   const int4 *work;                         // one item per workgroup
   AmpGrid amp;                              // block maxima of |u_next| (b == nullptr: off)
+  unsigned *rest_off;  // set when this step runs without the map: the launch clears the word
+                       // that tells k_quiet every step since the reset kept the map
+  unsigned long long *ucount;  // counter slots + 2 (cnt_slot), instrumented steps: per update kind,
+                               // cells in items, left out at rest, updated without eps - del
 };
 
 template <class S>
@@ -279,7 +301,7 @@ struct GradArgs {
   int skip;           // 1: skip tiles whose amplitude bound is below the G_k found so far
   unsigned long long *argkey;  // [2]: (tile maximum bits << 32 | item) of the step's largest tile
   int nwork;          // items in `work`
-  unsigned long long *gcount;  // [2]: tiles evaluated, tiles skipped (instrumented steps; else null)
+  unsigned long long *gcount;  // counter slots (cnt_slot): tiles evaluated, skipped (instrumented steps)
   unsigned *gseed;    // [2] per step parity: the exact maximum the bound pass found (float bits)
   unsigned *gseed_out;  // the bound pass: where its tiles' maximum goes (else null)
 };
@@ -620,7 +642,7 @@ __device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int k
     if (a.skip && a.amp.b) {
       // a uniform decision in a scalar register
       const int sk = __builtin_amdgcn_readfirstlane(amp_skip<S, T>(a, x0, x1, y0, y1, z0, z1, kstep) ? 1 : 0);
-      if (a.gcount && threadIdx.x == 0 && threadIdx.y == 0) atomicAdd(a.gcount + sk, 1ull);
+      if (a.gcount && threadIdx.x == 0 && threadIdx.y == 0) atomicAdd(cnt_slot(a.gcount) + sk, 1ull);
       if (sk) return;
     }
   }
@@ -717,11 +739,24 @@ __device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int k
 // without reading eps - del. The bound on each component is the gradient pass's, with the
 // update's first derivatives (sum c1 (u+m - u-m), then 1/d) and a margin for their rounding and
 // the hardware square root.
-static __global__ void k_quiet(AmpGrid ag, Coef c, const unsigned *gmax, Step st)
+//
+// The same pass keeps the touched bit: set once the neighbourhood's maximum of |u_k| is nonzero
+// or the block meets the source box, and never cleared until the reset. A block without it (at
+// rest) has u = 0 within 5 cells at every step so far, so u_{k-1}, u_k and the PML state
+// (driven only by derivatives of u at the cell) are 0 there and the step would write zeros over
+// zeros: the updates skip it. The history is whole only while every step since the reset ran
+// this pass (*hist nonzero; an update launched without the map clears it): otherwise every
+// block counts as touched.
+struct SrcBox {
+  int on, z0, z1, x0, x1, y0, y1;  // inclusive bounds
+};
+
+static __global__ void k_quiet(AmpGrid ag, Coef c, const unsigned *gmax, Step st, const unsigned *hist, SrcBox sb)
 {
   const int kstep = st.get();
   const unsigned *bb = ag.buf(kstep);
   const float thr = s_threshold(__uint_as_float(gmax[kstep]));
+  const bool whole = *hist != 0u;
   float s1 = 0.f;
 #pragma unroll
   for (int k = 0; k < H; k++) s1 += fabsf(c.c1[k]);
@@ -733,7 +768,12 @@ static __global__ void k_quiet(AmpGrid ag, Coef c, const unsigned *gmax, Step st
       for (int x = max(0, bx - 1); x <= min(ag.nbx - 1, bx + 1); x++)
         for (int y = max(0, by - 1); y <= min(ag.nby - 1, by + 1); y++)
           m = max(m, bb[((unsigned)z * ag.nbx + x) * ag.nby + y]);
-    ag.q[b] = (m < 0x7f800000u && __uint_as_float(m) * fac < thr) ? 1 : 0;
+    const bool src = sb.on && sb.z0 < ((bz + 1) << AB_Z) && sb.z1 >= (bz << AB_Z) && sb.x0 < ((bx + 1) << AB_X) &&
+                     sb.x1 >= (bx << AB_X) && sb.y0 < ((by + 1) << AB_Y) && sb.y1 >= (by << AB_Y);
+    const unsigned qb = ag.qi(bz, bx, by);
+    const bool touched = !whole || (ag.q[qb] & 2) != 0 || m != 0u || src;
+    const bool quiet = m < 0x7f800000u && __uint_as_float(m) * fac < thr;
+    ag.q[qb] = (unsigned char)((quiet ? 1 : 0) | (touched ? 2 : 0));
   }
 }
 
@@ -890,6 +930,7 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
   int x0, x1, y0, y1, z0, z1;
   bool fold;
   unpack(w, x0, x1, y0, y1, z0, z1, fold);
+  if (a.rest_off && blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) *a.rest_off = 0u;
   if (z0 >= z1) return;
   Lane<T> L(g, x0, x1, y0, y1, fold);
   // per-cell arrays and the lane's offsets into them (kind 2: compact y-face arrays; the host
@@ -950,6 +991,80 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
     }
   }
 
+  // The map of this step (when on), for the tile's z blocks: bit i of qm / rm is z block
+  // (z0 >> AB_Z) + i (i < 128; blocks past that count as neither). Lanes take one z block each
+  // and AND the map over the tile's x and y blocks (a folded tile: over both of its y strips).
+  // Quiet (qm): every cell has |grad u| below the threshold, so S = 1 and eps - del is not
+  // loaded. At rest (rm): the step would write zeros over zeros, so the planes of such blocks at
+  // either end of the item are left out (and the whole item when every block is at rest).
+  unsigned long long qm[2] = {0ull, 0ull}, rm[2] = {0ull, 0ull};
+  const int zq0 = z0 >> AB_Z, za = z0, zc = z1;
+  const int nzb = ((z1 - 1) >> AB_Z) - zq0 + 1;
+  if (a.amp.q) {
+    const int xb0 = x0 >> AB_X, xb1 = (x1 - 1) >> AB_X;
+    const int ya0 = y0 >> AB_Y, ya1 = (y1 - 1) >> AB_Y;
+    const int yh = g.ny - g.pml;  // a folded tile's second strip starts here
+    const int yc0 = fold ? yh >> AB_Y : ya0, yc1 = fold ? (yh + (y1 - y0) - 1) >> AB_Y : ya1;
+    // every block of the tile in one batch of loads (indices clamped to the tile's last block:
+    // repeats do not change an AND), so an item waits for the map once
+    constexpr int NXB = T::TX / (1 << AB_X) + 1, NYB = TY / (1 << AB_Y) + 1;
+    const int ns = fold ? 2 : 1;  // y strips (uniform)
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      if (h == 1 && nzb <= TY) break;  // uniform
+      const int i = __lane_id() + TY * h;
+      const int zb = min(zq0 + i, a.amp.nbz - 1);
+      unsigned v = 3u;  // AND of (bit 0 quiet, bit 1 not touched)
+#pragma unroll
+      for (int st = 0; st < 2; st++) {
+        if (st >= ns) break;
+        const int yb0 = st ? yc0 : ya0, yb1 = st ? yc1 : ya1;
+#pragma unroll
+        for (int xi = 0; xi < NXB; xi++)
+#pragma unroll
+          for (int yi = 0; yi < NYB; yi++) {
+            const unsigned m = a.amp.q[a.amp.qi(zb, min(xb0 + xi, xb1), min(yb0 + yi, yb1))];
+            v &= (m & 1u) | (~m & 2u);
+          }
+      }
+      qm[h] = __ballot(i < nzb && (v & 1u) != 0u);
+      rm[h] = __ballot(i < nzb && (v & 2u) != 0u);
+    }
+    if (rm[0] | rm[1]) {
+      // z blocks not at rest (blocks past the 128th always are)
+      const unsigned long long v0 = nzb >= 64 ? ~0ull : (1ull << nzb) - 1ull;
+      const unsigned long long v1 = nzb >= 128 ? ~0ull : nzb <= 64 ? 0ull : (1ull << (nzb - 64)) - 1ull;
+      const unsigned long long n0 = ~rm[0] & v0, n1 = ~rm[1] & v1;
+      if (!(n0 | n1) && nzb <= 128) {
+        z1 = z0;  // the whole item is at rest
+      } else {
+        const int first = n0 ? __builtin_ctzll(n0) : n1 ? 64 + __builtin_ctzll(n1) : 128;
+        z0 = max(z0, (zq0 + first) << AB_Z);
+        if (nzb <= 128) {
+          const int last = n1 ? 127 - __builtin_clzll(n1) : 63 - __builtin_clzll(n0);
+          z1 = min(z1, (zq0 + last + 1) << AB_Z);
+        }
+      }
+    }
+  }
+  if (a.ucount && threadIdx.x == 0 && threadIdx.y == 0) {
+    // instrumented steps, one thread per workgroup: cells of the item, cells left out at rest,
+    // cells without eps - del (a folded tile holds two y strips)
+    const unsigned cpp = (unsigned)(x1 - x0) * (unsigned)(y1 - y0) * (fold ? 2u : 1u);
+    int nq = 0;  // planes of [z0, z1) in quiet blocks
+    for (int i = 0; i < min(nzb, 128); i++)
+      if (((i < 64 ? qm[0] >> i : qm[1] >> (i - 64)) & 1ull) != 0ull)
+        nq += max(0, min(z1, (zq0 + i + 1) << AB_Z) - max(z0, (zq0 + i) << AB_Z));
+    unsigned long long *const uc = cnt_slot(a.ucount) + 3 * PML;
+    atomicAdd(uc, (unsigned long long)cpp * (unsigned)(zc - za));
+    atomicAdd(uc + 1, (unsigned long long)cpp * (unsigned)((zc - za) - (z1 - z0)));
+    atomicAdd(uc + 2, (unsigned long long)cpp * (unsigned)nq);
+  }
+  if (z0 >= z1) return;  // uniform over the workgroup
+  auto ed_needed = [&](int z) {
+    const int i = (z >> AB_Z) - zq0;
+    return i >= 128 || !(((i < 64 ? qm[0] >> i : qm[1] >> (i - 64)) & 1ull) != 0ull);
+  };
   float q[T::RX][QN];
   prime_queue<S, T>(q, a.ucur, g, L, z0);
   PlaneLoad<S, T> P;
@@ -957,34 +1072,6 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
   // per-cell loads, double-buffered: the interior march (unrolled by 22) addresses the buffer
   // of plane `it` as CL[it & 1] at compile time; the rolled PML march copies the next into
   // the current
-  // Quiet z blocks of this tile (when the quiet map is on): every cell there has |grad u| below
-  // the threshold, so S = 1 and eps - del is not loaded. Bit i of qm: z block (z0 >> AB_Z) + i;
-  // lanes take one z block each and AND the map over the tile's x and y blocks (a folded tile:
-  // over both of its y strips).
-  unsigned long long qm[2] = {0ull, 0ull};
-  const int zq0 = z0 >> AB_Z;
-  if (a.amp.q) {
-    const int nzb = ((z1 - 1) >> AB_Z) - zq0 + 1;
-    const int xb0 = x0 >> AB_X, xb1 = (x1 - 1) >> AB_X;
-    const int ya0 = y0 >> AB_Y, ya1 = (y1 - 1) >> AB_Y;
-    const int yh = g.ny - g.pml;  // a folded tile's second strip starts here
-    const int yc0 = fold ? yh >> AB_Y : ya0, yc1 = fold ? (yh + (y1 - y0) - 1) >> AB_Y : ya1;
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int i = __lane_id() + TY * h;
-      bool q = i < nzb;
-      const unsigned row = (unsigned)(zq0 + i) * a.amp.nbx;
-      for (int xb = xb0; q && xb <= xb1; xb++) {
-        for (int yb = ya0; q && yb <= ya1; yb++) q = a.amp.q[(row + (unsigned)xb) * a.amp.nby + (unsigned)yb] != 0;
-        for (int yb = yc0; q && yb <= yc1; yb++) q = a.amp.q[(row + (unsigned)xb) * a.amp.nby + (unsigned)yb] != 0;
-      }
-      qm[h] = __ballot(q);
-    }
-  }
-  auto ed_needed = [&](int z) {
-    const int i = (z >> AB_Z) - zq0;
-    return !(((i < 64 ? qm[0] >> i : qm[1] >> (i - 64)) & 1ull) != 0ull);
-  };
   CellLoad<S, T, PML> CL[2];
   CL[0].issue(cs, coff, z0, ed_needed(z0));
   float centre[T::RX];

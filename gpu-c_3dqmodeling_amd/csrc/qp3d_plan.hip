@@ -457,11 +457,16 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
     ag.nblk = (unsigned)ag.nbz * ag.nbx * ag.nby;
     if ((rc = dmalloc((void **)&ag.b, sizeof(unsigned) * 3 * (size_t)ag.nblk, "block maxima"))) return fail(rc);
     if ((rc = dmalloc((void **)&ag.q, (size_t)ag.nblk, "quiet map"))) return fail(rc);
-    // keys of the largest tile [2], tile counters [2], bound-pass maxima (two words)
-    if ((rc = dmalloc((void **)&p->argkey, 5 * sizeof(unsigned long long), "largest-tile keys"))) return fail(rc);
-    p->gcount = p->argkey + 2;
+    // keys of the largest tile [2], two spare, bound-pass maxima (two words), the rest history
+    // word; the counters of instrumented steps
+    if ((rc = dmalloc((void **)&p->argkey, ARG_WORDS * sizeof(unsigned long long), "largest-tile keys")))
+      return fail(rc);
+    if ((rc = dmalloc((void **)&p->cnt, CNT_BYTES, "counters"))) return fail(rc);
+    p->rflag = (unsigned *)(p->argkey + 5);
     if (hipMemset(ag.b, 0, sizeof(unsigned) * 3 * (size_t)ag.nblk) != hipSuccess ||
-        hipMemset(p->argkey, 0, 5 * sizeof(unsigned long long)) != hipSuccess) {
+        hipMemset(ag.q, 0, (size_t)ag.nblk) != hipSuccess ||
+        hipMemset(p->argkey, 0, ARG_WORDS * sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(p->cnt, 0, CNT_BYTES) != hipSuccess || hipMemset(p->rflag, 1, sizeof(unsigned)) != hipSuccess) {
       qp3d_set_error("plan_create: clearing the block maxima failed");
       return fail(QP3D_ERR_HIP);
     }
@@ -517,6 +522,7 @@ extern "C" int qp3d_plan_destroy(qp3d_plan *p)
   (void)hipFree(p->ampg.b);
   (void)hipFree(p->ampg.q);
   (void)hipFree(p->argkey);
+  (void)hipFree(p->cnt);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   (void)hipFree(p->rec);
@@ -891,7 +897,11 @@ extern "C" int qp3d_plan_reset(qp3d_plan *p)
   if (p->rec) HIP_TRY(hipMemsetAsync(p->rec, 0, sizeof(float) * (size_t)p->cfg.nt * p->hx * p->hy, p->stream));
   if (p->ampg.b) {
     HIP_TRY(hipMemsetAsync(p->ampg.b, 0, sizeof(unsigned) * 3 * (size_t)p->ampg.nblk, p->stream));
-    HIP_TRY(hipMemsetAsync(p->argkey, 0, 5 * sizeof(unsigned long long), p->stream));
+    HIP_TRY(hipMemsetAsync(p->argkey, 0, ARG_WORDS * sizeof(unsigned long long), p->stream));
+    HIP_TRY(hipMemsetAsync(p->cnt, 0, CNT_BYTES, p->stream));
+    // every block at rest, and a history from here
+    HIP_TRY(hipMemsetAsync(p->ampg.q, 0, (size_t)p->ampg.nblk, p->stream));
+    HIP_TRY(hipMemsetAsync(p->rflag, 1, sizeof(unsigned), p->stream));
   }
   p->amp_ok = true;  // u = 0 everywhere, and so are the block maxima
   p->nsnap_taken = 0;
@@ -917,6 +927,8 @@ static int harvest_events(qp3d_plan *p)
     }
     p->t_grad += a;
     p->t_int += b;
+    if (p->t_int_step.size() < (size_t)p->cfg.nt) p->t_int_step.assign((size_t)p->cfg.nt, 0.f);
+    p->t_int_step[(size_t)k] = b;
     p->t_pml += c;
     if (p->instrument == 1) p->t_upd += p->two_streams ? std::max(b, c) : b + c;
     p->n_launch += 1;
@@ -972,21 +984,25 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on
       ga.work = p->wl_grad_amp.d;
       ga.nwork = nwork = p->wl_grad_amp.n;
       ga.skip = p->amp_ok ? 1 : 0;
-      // tile counts only on instrumented steps: one counter for every workgroup is a hot spot
-      if (!p->instrument || bare) ga.gcount = nullptr;
+      // tile counts only on fully instrumented steps
+      if (p->instrument != 1 || bare) ga.gcount = nullptr;
       launch_gbound<S>(p->tile_g, GBOUND_BLOCKS, s0, ga);
       HIP_TRY(hipGetLastError());
     }
     launch_gradmax<S>(p->tile_g, nwork, s0, ga);
     HIP_TRY(hipGetLastError());
     if (use_amp && ga.skip) {
-      // G_k is complete: mark the blocks where S = 1 for the interior update
+      // G_k is complete: mark the blocks where S = 1, and those at rest, for the updates
+      const SrcBox sb{p->src_on ? 1 : 0, p->sb[0], p->sb[1], p->sb[2], p->sb[3], p->sb[4], p->sb[5]};
       hipLaunchKernelGGL(k_quiet, dim3((p->ampg.nblk + 255) / 256), dim3(256), 0, s0, p->ampg, p->coef,
-                         (const unsigned *)p->gmax, st);
+                         (const unsigned *)p->gmax, st, (const unsigned *)p->rflag, sb);
       HIP_TRY(hipGetLastError());
       ua.amp.q = p->ampg.q;
     }
   }
+  // a step without the map breaks the rest history until the next reset
+  if (!ua.amp.q) ua.rest_off = p->rflag;
+  if (ua.amp.q && p->instrument == 1 && !bare) ua.ucount = p->cnt + 2;  // all events: counters too
   p->amp_ok = use_amp;  // u(k+1) is covered by the block maxima the updates write
   if (e) HIP_TRY(hipEventRecord(e[1], s0));
   if (two) {
@@ -1443,14 +1459,39 @@ extern "C" int qp3d_plan_set_amp_bound(qp3d_plan *p, int on)
 // Gradient-pass tiles evaluated and skipped under the amplitude bound since the last reset
 // (out[0], out[1]), counted on instrumented steps (qp3d_plan_set_instrument); both 0 when the
 // bound is unavailable (gmax_mode 2).
+// The counter slots after the plan's work so far (zeros when the plan keeps none).
+static int read_counters(const qp3d_plan *p, unsigned long long *c)
+{
+  std::memset(c, 0, CNT_BYTES);
+  if (!p->cnt) return QP3D_OK;
+  HIP_TRY(hipSetDevice(p->cfg.device));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  if (p->two_streams) HIP_TRY(hipStreamSynchronize(p->stream2));
+  HIP_TRY(hipMemcpy(c, p->cnt, CNT_BYTES, hipMemcpyDeviceToHost));
+  return QP3D_OK;
+}
+
 extern "C" int qp3d_plan_gradient_tiles(const qp3d_plan *p, unsigned long long *out)
 {
   if (!p || !out) return QP3D_ERR_ARG;
   out[0] = out[1] = 0;
-  if (!p->gcount) return QP3D_OK;
-  HIP_TRY(hipSetDevice(p->cfg.device));
-  HIP_TRY(hipStreamSynchronize(p->stream));
-  HIP_TRY(hipMemcpy(out, p->gcount, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  unsigned long long c[CNT_SLOTS * CNT_STRIDE];
+  int rc = read_counters(p, c);
+  if (rc) return rc;
+  for (int s = 0; s < CNT_SLOTS; s++)
+    for (int j = 0; j < 2; j++) out[j] += c[s * CNT_STRIDE + j];
+  return QP3D_OK;
+}
+
+extern "C" int qp3d_plan_update_cells(const qp3d_plan *p, unsigned long long *out)
+{
+  if (!p || !out) return QP3D_ERR_ARG;
+  for (int i = 0; i < 9; i++) out[i] = 0;
+  unsigned long long c[CNT_SLOTS * CNT_STRIDE];
+  int rc = read_counters(p, c);
+  if (rc) return rc;
+  for (int s = 0; s < CNT_SLOTS; s++)
+    for (int j = 0; j < 9; j++) out[j] += c[s * CNT_STRIDE + 2 + j];
   return QP3D_OK;
 }
 
@@ -1465,6 +1506,13 @@ extern "C" int qp3d_plan_set_instrument(qp3d_plan *p, int on)
   p->t_grad = p->t_int = p->t_pml = p->t_upd = 0;
   p->n_launch = 0;
   p->ev_pending.clear();
+  return QP3D_OK;
+}
+
+extern "C" int qp3d_plan_step_times(const qp3d_plan *p, float *ms)
+{
+  if (!p || !ms) return QP3D_ERR_ARG;
+  for (int k = 0; k < p->cfg.nt; k++) ms[k] = (size_t)k < p->t_int_step.size() ? p->t_int_step[(size_t)k] : 0.f;
   return QP3D_OK;
 }
 

@@ -25,6 +25,10 @@ namespace qp3d_impl {
 constexpr int GRAPH_LEN = 10;   // time steps per captured graph
 constexpr int AMP_ZCHUNK = 32;  // z planes per item of the gradient pass under the amplitude bound
 constexpr int EV_PER_STEP = 5;  // instrumentation events per step
+// words of the plan's small device block: largest-tile keys [2], two spare, bound-pass maxima
+// (two 32-bit words), the rest-history word
+constexpr int ARG_WORDS = 6;
+constexpr size_t CNT_BYTES = sizeof(unsigned long long) * qp3d::CNT_SLOTS * qp3d::CNT_STRIDE;
 
 struct WorkList {
   int4 *d = nullptr;
@@ -38,6 +42,8 @@ using qp3d_impl::WorkList;
 using qp3d_impl::GRAPH_LEN;
 using qp3d_impl::EV_PER_STEP;
 using qp3d_impl::AMP_ZCHUNK;
+using qp3d_impl::ARG_WORDS;
+using qp3d_impl::CNT_BYTES;
 
 struct qp3d_plan {
   qp3d_config cfg{};
@@ -85,7 +91,10 @@ struct qp3d_plan {
   // its buffer (false after steps run without the bound, e.g. FWI recomputation)
   AmpGrid ampg{};
   unsigned long long *argkey = nullptr;
-  unsigned long long *gcount = nullptr;  // [2]: gradient tiles evaluated, skipped since reset
+  unsigned *rflag = nullptr;  // nonzero while every step since the reset ran k_quiet (rest history)
+  // counters of instrumented steps since the reset, CNT_SLOTS x CNT_STRIDE words (cnt_slot):
+  // gradient tiles evaluated and skipped, update cells per kind
+  unsigned long long *cnt = nullptr;
   bool amp_ok = true;
   bool amp_on = true;  // qp3d_plan_set_amp_bound
   bool spec = false;             // gmax_mode 2 (a batch turns it off for its plans)
@@ -119,6 +128,7 @@ struct qp3d_plan {
                                 // after PML, after the fix-up (gmax_mode 2)
   std::vector<int> ev_pending;  // steps with recorded events not yet harvested
   double t_grad = 0, t_int = 0, t_pml = 0, t_upd = 0;
+  std::vector<float> t_int_step;  // [nt] interior-update time of each instrumented step (ms)
   long n_launch = 0;
 };
 
@@ -170,7 +180,7 @@ inline void make_args(const qp3d_plan *p, int prev, Step st, GradArgs<S> &ga, Up
   ga.skip = 0;
   ga.argkey = p->argkey;
   ga.nwork = p->wl_grad.n;
-  ga.gcount = p->gcount;
+  ga.gcount = p->cnt;
   ga.gseed = p->argkey ? (unsigned *)(p->argkey + 4) : nullptr;
   ga.gseed_out = nullptr;
   ua.g = p->geo;
@@ -205,6 +215,8 @@ inline void make_args(const qp3d_plan *p, int prev, Step st, GradArgs<S> &ga, Up
   ua.work = nullptr;
   ua.amp = p->ampg;
   ua.amp.q = nullptr;  // the step turns the quiet map on after k_quiet has filled it
+  ua.rest_off = nullptr;
+  ua.ucount = nullptr;
 }
 
 // Sizes of the state and model blocks (esz and the geometry set): fstride, ystride,

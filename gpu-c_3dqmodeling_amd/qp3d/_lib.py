@@ -76,6 +76,8 @@ _SIGS = [
     ("qp3d_plan_step_predictions", C.c_int, [C.c_void_p, _F]),
     ("qp3d_plan_set_amp_bound", C.c_int, [C.c_void_p, C.c_int]),
     ("qp3d_plan_gradient_tiles", C.c_int, [C.c_void_p, C.POINTER(C.c_ulonglong)]),
+    ("qp3d_plan_update_cells", C.c_int, [C.c_void_p, C.POINTER(C.c_ulonglong)]),
+    ("qp3d_plan_step_times", C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     ("qp3d_plan_set_instrument", C.c_int, [C.c_void_p, C.c_int]),
     ("qp3d_plan_gmax_mode", C.c_int, [C.c_void_p]),
     ("qp3d_work_items", C.c_int, [C.POINTER(Config), C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int]),

@@ -84,6 +84,14 @@ class Plan:
         check(self.lib.qp3d_plan_gradient_tiles(self.h, out))
         return int(out[0]), int(out[1])
 
+    def update_cells(self):
+        """Per update kind (interior, PML over the full grid, folded y faces): (cells of its
+        items, left out at rest, updated without eps - del) since the last reset, counted on
+        instrumented steps; a 3 x 3 integer array."""
+        out = (C.c_ulonglong * 9)()
+        check(self.lib.qp3d_plan_update_cells(self.h, out))
+        return np.array(out[:], dtype=np.int64).reshape(3, 3)
+
     def field(self, which=0):
         s = self.s
         out = np.zeros((s.nz, s.nx, s.ny), np.float32)
@@ -141,6 +149,12 @@ class Plan:
         check(self.lib.qp3d_plan_kernel_times(self.h, ms))
         return {"gradmax": ms[0], "update_interior": ms[1], "update_pml": ms[2], "update_stage": ms[3],
                 "steps": int(ms[4])}
+
+    def step_times(self):
+        """Interior-update time (ms) of each step from the last instrumented run (0: not measured)."""
+        out = np.zeros(self.s.nt, np.float32)
+        check(self.lib.qp3d_plan_step_times(self.h, fptr(out)))
+        return out
 
     def step_bytes(self):
         """Algorithmic bytes per step of each launch and the cells of the two update launches."""

@@ -200,14 +200,21 @@ int qp3d_plan_step_predictions(const qp3d_plan *p, float *out);
  * plan belongs to a batch), or a QP3D_* error. */
 int qp3d_plan_gmax_mode(const qp3d_plan *p);
 /* Amplitude bound of the gradient pass (gmax_mode 1; no reference counterpart): the update
- * kernels record max |u| per block of 8 x 8 x 64 cells, a small pass evaluates exactly the tile
- * that held the previous step's largest gradient component, and the pass skips every tile whose
- * bound 2 sum|c1|/d max|u| is below the G found so far, without reading it. G, and so every
- * result, is the same as with the bound off. On by default; the setter turns it off (0) or on
- * (1). qp3d_plan_gradient_tiles: tiles evaluated (out[0]) and skipped (out[1]) since the
- * last reset, counted on instrumented steps (qp3d_plan_set_instrument). */
+ * kernels record max |u| per block of 8 x 8 x 32 cells (z, x, y), a small pass evaluates exactly
+ * the tile that held the previous step's largest gradient component, and the pass skips every
+ * tile whose bound 2 sum|c1|/d max|u| is below the G found so far, without reading it. From the
+ * same maxima, after the gradient pass, the updates learn the blocks where S = 1 (they skip the
+ * eps - del load there) and the blocks still at rest (u = 0 within 5 cells at every step since
+ * the reset, and no source: they skip them). G, and so every result, is the same as with the
+ * bound off. On by default; the setter turns it off (0) or on (1).
+ * qp3d_plan_gradient_tiles: tiles evaluated (out[0]) and skipped (out[1]) since the last reset;
+ * qp3d_plan_update_cells: out[9], per update kind (interior tiles, PML tiles over the full-grid
+ * state, folded y-face tiles) the cells of its items, of them left out at rest, and updated
+ * without eps - del, since the last reset. Both counted on instrumented steps
+ * (qp3d_plan_set_instrument). */
 int qp3d_plan_set_amp_bound(qp3d_plan *p, int on);
 int qp3d_plan_gradient_tiles(const qp3d_plan *p, unsigned long long *out);
+int qp3d_plan_update_cells(const qp3d_plan *p, unsigned long long *out);
 /* 1: bracket every stencil launch with HIP events on the plan's stream; 2: only the interior
  * update (two events per step instead of four); 0: none. Resets the accumulated times. */
 int qp3d_plan_set_instrument(qp3d_plan *p, int on);
@@ -217,6 +224,9 @@ int qp3d_plan_set_instrument(qp3d_plan *p, int on);
  * maximum to the end of both updates; they overlap when streams == 2); ms[4] = steps counted.
  * With instrumentation mode 2 only ms[1] and ms[4] are measured (the others read 0). */
 int qp3d_plan_kernel_times(qp3d_plan *p, double *ms);
+/* Interior-update time (ms) of each step, ms[nt], as last measured on an instrumented step of
+ * this plan (0 for steps never measured); harvested by qp3d_plan_sync. */
+int qp3d_plan_step_times(const qp3d_plan *p, float *ms);
 /* Algorithmic HBM bytes of one time step (interior cells 28 B, PML cells 68 B in FP32):
  * bytes[0] gradient-max launch (4 B per updated cell), bytes[1] interior update launch (24 B per
  * cell), bytes[2] PML update launch (64 B per PML cell, 24 B per interior cell it carries);

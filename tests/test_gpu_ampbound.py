@@ -2,7 +2,11 @@
 (from the block maxima the updates record) is below the G already found exactly, so G_k and
 every result must be the same with the bound on and off, bit for bit: direct launches and the
 captured graph, 16-bit storage, a source on the frozen rim, and a plan whose block maxima went
-stale while a batch drove it. The bound must also actually skip tiles on a shot from rest."""
+stale while a batch drove it. The bound must also actually skip tiles on a shot from rest.
+
+The same maxima drive the quiet map (S = 1 proved: no eps - del load) and the rest map (blocks
+the wave has not reached, left out of the updates): both are covered by the same comparisons,
+and a shot whose rest history breaks halfway (steps run without the map) must stay exact."""
 import sys
 
 import numpy as np
@@ -111,3 +115,55 @@ def test_bound_after_a_batch_drove_the_plan():
     finally:
         for p in plans:
             p.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_rest_map_leaves_out_cells_and_changes_nothing(graph):
+    s = qp3d.config_setup(hx=101, hy=101, nz0=101, pml=20, nt=300, seed=0, src_depth=3)
+    with qp3d.Plan(s, use_graph=graph) as p:
+        p.set_amp_bound(False)
+        ref = run(p)
+        p.set_amp_bound(True)
+        p.set_instrument(1)
+        say("rest map on, instrumented")
+        same(run(p), ref)
+        uc = p.update_cells()
+        p.set_instrument(0)
+        say(f"update cells per kind (cells, at rest, quiet): {uc.tolist()}")
+        # every cell of [5, n-5)^3 is in exactly one update item per step
+        assert int(uc[:, 0].sum()) == s.nt * (s.nx - 10) * (s.ny - 10) * (s.nz - 10)
+        assert uc[0, 1] > 0.05 * uc[0, 0] and uc[1, 1] + uc[2, 1] > 0.05 * (uc[1, 0] + uc[2, 0])
+        assert uc[0, 2] > 0
+        assert np.all(uc[:, 1] + uc[:, 2] <= uc[:, 0])
+        # the history breaks for ten steps (bound off), then the map counts every block as touched
+        p.reset()
+        p.run(0, 31)
+        p.set_amp_bound(False)
+        p.run(31, 41)
+        p.set_amp_bound(True)
+        p.run(41, s.nt)
+        p.sync()
+        say(f"rest history broken at step 31, bound back on at 41, graph={graph}")
+        same(state(p), ref)
+
+
+def test_rest_map_with_a_source_deep_in_the_grid():
+    """A source far from every face: most blocks stay at rest for many steps, and the first to
+    leave rest sit next to the source box."""
+    s = qp3d.config_setup(hx=61, hy=67, nz0=59, pml=12, nt=160, seed=3, src_depth=0)
+    src = np.zeros_like(s.source)
+    src[s.nz // 2, s.nx // 2 + 3, s.ny // 2 - 2] = 1.0
+    src[s.nz // 2 + 9, s.nx // 2 - 8, s.ny // 2 + 7] = -0.7  # another block
+    s.source = src
+    with qp3d.Plan(s) as p:
+        p.set_amp_bound(False)
+        ref = run(p)
+        p.set_amp_bound(True)
+        p.set_instrument(1)
+        same(run(p, split=23), ref)
+        uc = p.update_cells()
+        p.set_instrument(0)
+        say(f"deep source: update cells per kind {uc.tolist()}")
+        # small grid: the touched region (whole 8 x 8 x 32 blocks and their neighbours) covers
+        # it within some tens of steps
+        assert uc[:, 1].sum() > 0.005 * uc[:, 0].sum()
