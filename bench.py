@@ -436,6 +436,10 @@ def main():
         plan.forward_async()
         plan.sync()
         kt_timed = plan.kernel_times()
+    # interior-update time over the shot, in twelve stretches of steps (from rest to the end)
+    st_int = plan.step_times()
+    nb = max(1, s.nt // 12)
+    int_bins = [round(float(st_int[i:i + nb].mean()), 4) for i in range(0, nb * 12, nb)] if s.nt >= 12 else None
     plan.set_instrument(1)
     plan.forward_async()
     plan.sync()
@@ -531,6 +535,7 @@ def main():
                    "update_cells_at_rest_frac": round(rest_frac, 4) if counted else None,
                    "update_cells_quiet_frac": round(quiet_frac, 4) if counted else None,
                    "update_cells_at_rest_frac_interior": round(int_c[1] / max(1, int_c[0]), 4) if counted else None,
+                   "ms_update_interior_by_twelfths": int_bins,
                    "roofline_kernel": f"k_update<interior> ({b_int:.0f} B per interior cell updated, "
                                       f"{b_int - 4:.0f} B without eps - del, 0 B at rest; "
                                       f"{moved_int:.0f} B per launch over the shot, "
