@@ -73,8 +73,8 @@ extern "C" int qp3d_fwi_create(qp3d_plan *p, int interval, int ckpt_storage, qp3
   }
   *out = nullptr;
   const qp3d_config &c = p->cfg;
-  if (c.storage != QP3D_STORAGE_FP32 || p->two_streams) {
-    qp3d_set_error("fwi_create: the gradient needs a plan with FP32 storage and one stream");
+  if (c.storage != QP3D_STORAGE_FP32) {
+    qp3d_set_error("fwi_create: the gradient needs a plan with FP32 storage");
     return QP3D_ERR_ARG;
   }
   if (c.nt < 1 || interval < 1 || (ckpt_storage != QP3D_STORAGE_FP32 && ckpt_storage != QP3D_STORAGE_BF16)) {

@@ -330,8 +330,9 @@ void qp3d_plan_describe(qp3d_plan *p, const qp3d_config &c)
   p->tile_p = c.tile_pml > 0 ? c.tile_pml - 1 : 5;
   p->tile_py = c.tile_pml_y > 0 ? c.tile_pml_y - 1 : 1;  // 16x1
   p->spec = c.gmax_mode == 2;
-  // a captured loop runs on one stream, and so does a step whose fix-up follows all updates
-  p->two_streams = c.streams == 2 && !c.use_graph && !p->spec;
+  // two streams by default; a step whose fix-up follows all updates runs on one stream; a
+  // captured loop forks the PML updates onto the second stream and joins them back in the graph
+  p->two_streams = (c.streams == 2 || c.streams == 0) && !p->spec;
   p->esz = c.storage == QP3D_STORAGE_FP32 ? 4 : 2;
   // folded y faces keep their model and PML state in compact arrays (YCompact)
   p->yw = y_fold(c.pml) ? c.pml - H : 0;
@@ -964,8 +965,10 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on
     }
     e = &p->ev[EV_PER_STEP * (size_t)k];
   }
-  // u(k) is complete only when the previous step's PML update has finished too
-  if (two) HIP_TRY(hipStreamWaitEvent(s0, p->ev_pml, 0));
+  // u(k) is complete only when the previous step's PML update has finished too (a graph's
+  // first step: the launch of the graph waits for it instead)
+  if (two && !p->cap_first) HIP_TRY(hipStreamWaitEvent(s0, p->ev_pml, 0));
+  p->cap_first = false;
   const bool all = p->instrument == 1;  // 2: events around the interior update only
   if (e && all) HIP_TRY(hipEventRecord(e[0], s0));
   GradArgs<S> ga;
@@ -1083,11 +1086,12 @@ static int launch_step(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on, 
 
 // FWI: the first forward run with the amplitude bound, or a recomputed step whose G_k the first
 // run already left in gmax[k] (no gradient pass; the update reads it)
+// (on the plan's own stream only: the checkpoints and the adjoint follow on it)
 int qp3d_forward_step_fwi(qp3d_plan *p, int k, bool recompute)
 {
-  if (recompute && p->spec) return launch_step(p, k, Step{nullptr, k}, true, nullptr, false);
-  return recompute ? launch_step(p, k, Step{nullptr, k}, true, nullptr, false, true)
-                   : launch_step(p, k, Step{nullptr, k}, true, nullptr, true);
+  if (recompute && p->spec) return launch_step(p, k, Step{nullptr, k}, true, p->stream, false);
+  return recompute ? launch_step(p, k, Step{nullptr, k}, true, p->stream, false, true)
+                   : launch_step(p, k, Step{nullptr, k}, true, p->stream, true);
 }
 
 int qp3d_forward_step(qp3d_plan *p, int k) { return launch_step(p, k, Step{nullptr, k}, true, nullptr, false); }
@@ -1111,7 +1115,19 @@ static int capture_steps(qp3d_plan *p)
   int rc = QP3D_OK;
   // every replay starts with the block maxima covering u (qp3d_plan_run checks amp_ok)
   p->amp_ok = true;
+  // two streams: each step forks the PML updates onto the second stream after the gradient
+  // pass (ev_grad) and the next step joins them (ev_pml), both recorded inside the capture;
+  // the last step's PML updates join the capturing stream before the capture ends
+  p->cap_first = true;
   for (int i = 0; i < GRAPH_LEN && rc == QP3D_OK; i++) rc = launch_step(p, i, Step{p->kdev, i}, true, nullptr, true);
+  p->cap_first = false;
+  if (rc == QP3D_OK && p->two_streams) {
+    const hipError_t ej = hipStreamWaitEvent(p->stream, p->ev_pml, 0);
+    if (ej != hipSuccess) {
+      qp3d_set_error("plan_run: joining the second stream in the capture failed: %s", hipGetErrorString(ej));
+      rc = QP3D_ERR_HIP;
+    }
+  }
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(p->stream, &g);
   if (rc == QP3D_OK && e != hipSuccess) {
@@ -1159,6 +1175,8 @@ extern "C" int qp3d_plan_run(qp3d_plan *p, int k0, int k1)
       }
       hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, p->stream, p->kdev, k);
       HIP_TRY(hipGetLastError());
+      // the replay's first step reads u(k), which the last direct step's PML update finishes
+      if (p->two_streams) HIP_TRY(hipStreamWaitEvent(p->stream, p->ev_pml, 0));
       HIP_TRY(hipGraphLaunch(p->gexec, p->stream));
       k += GRAPH_LEN;
     } else {

@@ -54,6 +54,7 @@ struct qp3d_plan {
   Coef coef{};
   int tile_u = 0, tile_g = 0, tile_p = 0;
   bool two_streams = false;
+  bool cap_first = false;  // capturing the first step of a graph: u(k) needs no wait on ev_pml
   hipStream_t stream2 = nullptr;                          // PML updates when two_streams
   hipEvent_t ev_grad = nullptr, ev_pml = nullptr;         // cross-stream ordering, no timing
   hipEvent_t ev_in = nullptr, ev_out = nullptr;           // ordering with a caller's stream

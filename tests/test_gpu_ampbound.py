@@ -167,3 +167,16 @@ def test_rest_map_with_a_source_deep_in_the_grid():
         # small grid: the touched region (whole 8 x 8 x 32 blocks and their neighbours) covers
         # it within some tens of steps
         assert uc[:, 1].sum() > 0.005 * uc[:, 0].sum()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_two_streams_equal_one(graph):
+    """PML updates on a second stream, concurrent with the interior update (in a captured graph:
+    forked and joined inside it), with the bound and both maps on: the same shot bit for bit."""
+    s = qp3d.config_setup(hx=101, hy=101, nz0=101, pml=20, nt=300, seed=0, src_depth=3)
+    with qp3d.Plan(s) as p:
+        ref = run(p)
+    with qp3d.Plan(s, use_graph=graph, streams=2) as p:
+        say(f"two streams, graph={graph}")
+        same(run(p, split=37), ref)
+        same(run(p), ref)
