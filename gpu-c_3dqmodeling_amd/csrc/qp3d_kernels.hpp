@@ -173,8 +173,19 @@ __device__ __forceinline__ unsigned long long *cnt_slot(unsigned long long *c)
   return c + (blockIdx.x % CNT_SLOTS) * CNT_STRIDE;
 }
 constexpr int AB_Z = 3, AB_X = 3, AB_Y = 5;  // log2 of the block extents
+//
+// The interior updates record in the same buffer slot two more maxima per block: of the
+// increment |u_{k+1} - u_k| (d) and of |grad u_k| as the update computed it for S (c; it bounds
+// every component). Since grad u_{k+1} = grad u_k + grad(u_{k+1} - u_k), a component of u_{k+1}
+// at a cell is at most c of its block plus 2 sum_m |g_m| times the increment's maximum within 5
+// cells: the increment is a small part of u (omega dt ~ 0.16), so this bound is several times
+// tighter than the one from max |u|, and the pass uses the smaller of the two. The PML kernels,
+// at their register limit, do not track them: a bound over blocks that meet the PML (pml_block)
+// is the one from max |u|.
 struct AmpGrid {
-  unsigned *b;   // [3][nbz][nbx][nby]
+  // [3][3][nbz][nbx][nby]: max |u|, then max |u_{k+1} - u_k| and max |grad u_k| (both in slot
+  // k + 1), three rotating slots each (one pointer: fewer scalar registers in the updates)
+  unsigned *b;
   int nbz, nbx, nby;
   unsigned nblk;  // nbz * nbx * nby
   // [nbx][nby][nbz] map of the current step (k_quiet): bit 0 quiet (S = 1 in every cell), bit 1
@@ -183,6 +194,14 @@ struct AmpGrid {
   // or two cache lines.
   unsigned char *q;
   __device__ __forceinline__ unsigned *buf(int k) const { return b + (unsigned)(k % 3) * nblk; }
+  __device__ __forceinline__ unsigned *dbuf(int k) const { return b + (unsigned)(3 + k % 3) * nblk; }
+  __device__ __forceinline__ unsigned *cbuf(int k) const { return b + (unsigned)(6 + k % 3) * nblk; }
+  // whether block (bz, bx, by) holds a PML cell (d and c not recorded there)
+  __device__ __forceinline__ static bool pml_block(const Geo &g, int bz, int bx, int by)
+  {
+    return (bz << AB_Z) < g.pml || ((bz + 1) << AB_Z) > g.nz - g.pml || (bx << AB_X) < g.pml ||
+           ((bx + 1) << AB_X) > g.nx - g.pml || (by << AB_Y) < g.pml || ((by + 1) << AB_Y) > g.ny - g.pml;
+  }
   __device__ __forceinline__ unsigned qi(int bz, int bx, int by) const
   {
     return ((unsigned)bx * (unsigned)nby + (unsigned)by) * (unsigned)nbz + (unsigned)bz;
@@ -195,35 +214,100 @@ struct AmpGrid {
 // goes to the one or two y blocks its first and last lanes fall in, for every x block of the
 // tile. The (z block, x block, half, y block) targets are spread over the lanes: one atomic
 // instruction per flush (at most 4 z blocks x 3 x blocks x 4 targets = 48 lanes).
-__device__ __forceinline__ void amp_flush(const AmpGrid &ag, int kstep, int za, int zb, int x0, int x1, int ylane,
-                                          unsigned &am)
+__device__ __forceinline__ unsigned half_wave_max(unsigned am)
 {
-  // half-wavefront maxima with DPP: prefix maxima within each 16-lane row (row_shr 1, 2, 4, 8),
-  // then row 0's maximum into row 1 and row 2's into row 3 (row_bcast:15), so lanes 31 and 63
-  // hold the maxima of lanes 0..31 and 32..63 (lanes without a source read 0, the identity)
   int v = (int)am;
   v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));
   v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));
   v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));
   v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));
   v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));
+  return (unsigned)v;
+}
+
+__device__ __forceinline__ void amp_flush(const AmpGrid &ag, int kstep, int za, int zb, int x0, int x1, int ylane,
+                                          unsigned &am, unsigned &ad, unsigned &ac)
+{
+  // half-wavefront maxima with DPP: prefix maxima within each 16-lane row (row_shr 1, 2, 4, 8),
+  // then row 0's maximum into row 1 and row 2's into row 3 (row_bcast:15), so lanes 31 and 63
+  // hold the maxima of lanes 0..31 and 32..63 (lanes without a source read 0, the identity)
+  const unsigned vm = half_wave_max(am), vd = half_wave_max(ad), vc = half_wave_max(ac);
   const int yb = min(ylane >> AB_Y, ag.nby - 1);
   // targets t = 0..3: half 0 first / last lane's y block, half 1 first / last
   const int ty[4] = {__builtin_amdgcn_readlane(yb, 0), __builtin_amdgcn_readlane(yb, 31),
                      __builtin_amdgcn_readlane(yb, 32), __builtin_amdgcn_readlane(yb, 63)};
-  const unsigned tv[2] = {(unsigned)__builtin_amdgcn_readlane(v, 31), (unsigned)__builtin_amdgcn_readlane(v, 63)};
   const int bz0 = za >> AB_Z, nbz = ((zb - 1) >> AB_Z) - bz0 + 1;
   const int bx0 = x0 >> AB_X, nbx = ((x1 - 1) >> AB_X) - bx0 + 1;
   const int l = __lane_id();
   const int t = l & 3, c = l >> 2;  // target, (z block, x block) pair
   const int bz = bz0 + c / nbx, bx = bx0 + c % nbx;
   const int ybt = t == 0 ? ty[0] : t == 1 ? ty[1] : t == 2 ? ty[2] : ty[3];
-  const unsigned val = t < 2 ? tv[0] : tv[1];
+  const unsigned wm = t < 2 ? (unsigned)__builtin_amdgcn_readlane((int)vm, 31) : (unsigned)__builtin_amdgcn_readlane((int)vm, 63);
+  const unsigned wd = t < 2 ? (unsigned)__builtin_amdgcn_readlane((int)vd, 31) : (unsigned)__builtin_amdgcn_readlane((int)vd, 63);
+  const unsigned wc = t < 2 ? (unsigned)__builtin_amdgcn_readlane((int)vc, 31) : (unsigned)__builtin_amdgcn_readlane((int)vc, 63);
   // the last-lane target only where it differs from the first-lane one
   const bool dup = (t == 1 && ty[1] == ty[0]) || (t == 3 && ty[3] == ty[2]);
-  if (c < nbz * nbx && val != 0u && !dup)
-    atomicMax(ag.buf(kstep + 1) + ((unsigned)bz * ag.nbx + (unsigned)bx) * ag.nby + (unsigned)ybt, val);
+  if (c < nbz * nbx && !dup) {
+    const unsigned o = (unsigned)(kstep + 1) % 3u * ag.nblk + ((unsigned)bz * ag.nbx + (unsigned)bx) * ag.nby + (unsigned)ybt;
+    if (wm != 0u) atomicMax(ag.b + o, wm);
+    if (wd != 0u) atomicMax(ag.b + 3 * ag.nblk + o, wd);
+    if (wc != 0u) atomicMax(ag.b + 6 * ag.nblk + o, wc);
+  }
   am = 0u;
+  ad = 0u;
+  ac = 0u;
+}
+
+// Block maxima of the interior update through LDS: every wavefront adds its half-wavefront
+// maxima of the three values into one workgroup set per y block of the tile (two lanes, LDS
+// atomics), and wavefront 0 moves a set into the global block maxima of every (z block, x block,
+// y block) of the stretch and the tile once a barrier has passed, then clears it. One set of
+// global atomics per workgroup and stretch instead of one per wavefront.
+__device__ __forceinline__ void amp_lds_add(unsigned (&sv)[3][4], int y0, int ylane, int nby, unsigned &am,
+                                            unsigned &ad, unsigned &ac)
+{
+  const unsigned vm = half_wave_max(am), vd = half_wave_max(ad), vc = half_wave_max(ac);
+  const int yb = min(ylane >> AB_Y, nby - 1) - (y0 >> AB_Y);
+  const int f0 = __builtin_amdgcn_readlane(yb, 0), l0 = __builtin_amdgcn_readlane(yb, 31);
+  const int f1 = __builtin_amdgcn_readlane(yb, 32), l1 = __builtin_amdgcn_readlane(yb, 63);
+  const int l = __lane_id();
+  if (l == 31 || l == 63) {
+    const int yf = l == 31 ? f0 : f1, yl = l == 31 ? l0 : l1;
+    atomicMax(&sv[0][yf], vm);
+    atomicMax(&sv[1][yf], vd);
+    atomicMax(&sv[2][yf], vc);
+    if (yl != yf) {
+      atomicMax(&sv[0][yl], vm);
+      atomicMax(&sv[1][yl], vd);
+      atomicMax(&sv[2][yl], vc);
+    }
+  }
+  am = 0u;
+  ad = 0u;
+  ac = 0u;
+}
+
+// Wavefront 0: planes [za, zb) of the tile's x range [x0, x1) and the y blocks of its 64 lanes
+// from y0.
+__device__ __forceinline__ void amp_lds_flush(const AmpGrid &ag, int kstep, unsigned (&sv)[3][4], int za, int zb,
+                                              int x0, int x1, int y0)
+{
+  const int yb0 = y0 >> AB_Y, nyb = min((y0 + TY - 1) >> AB_Y, ag.nby - 1) - yb0 + 1;
+  const int bz0 = za >> AB_Z, nbz = ((zb - 1) >> AB_Z) - bz0 + 1;
+  const int bx0 = x0 >> AB_X, nbx = ((x1 - 1) >> AB_X) - bx0 + 1;
+  const int l = __lane_id();
+  const int yi = l % nyb, t = l / nyb, xi = t % nbx, zi = t / nbx;
+  const bool on = l < nbz * nbx * nyb;
+  const int yr = on ? yi : 0;
+  const unsigned wm = sv[0][yr], wd = sv[1][yr], wc = sv[2][yr];
+  if (on) {
+    const unsigned o = (unsigned)(kstep + 1) % 3u * ag.nblk +
+                       ((unsigned)(bz0 + zi) * ag.nbx + (unsigned)(bx0 + xi)) * ag.nby + (unsigned)(yb0 + yi);
+    if (wm != 0u) atomicMax(ag.b + o, wm);
+    if (wd != 0u) atomicMax(ag.b + 3 * ag.nblk + o, wd);
+    if (wc != 0u) atomicMax(ag.b + 6 * ag.nblk + o, wc);
+  }
+  if (l < 12) sv[l >> 2][l & 3] = 0u;  // after every lane's read above (one wavefront, in order)
 }
 
 // The y faces of the PML fold into one tile per x row group when each is at most 27 cells
@@ -579,6 +663,37 @@ __device__ __forceinline__ void reduce_gmax(float m, bool nonfinite, int tx, int
   }
 }
 
+__device__ __forceinline__ unsigned wave_max(unsigned m)
+{
+  // wavefront maximum into lane 63 (row prefix maxima, then rows 0 -> 1, 2 -> 3, 1 -> 2 and 3)
+  int v = (int)m;
+  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));
+  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));
+  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));
+  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));
+  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));
+  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));
+  return (unsigned)__builtin_amdgcn_readlane(v, 63);
+}
+
+// The smaller of the two bounds on a gradient component of u_k over cells whose blocks hold at
+// most C (the previous step's components) and whose 5-cell neighbourhoods hold at most M (|u_k|)
+// and D (|u_k - u_{k-1}|); fac = 2 sum_m |g_m|. The margins cover the rounding of both gradient
+// sums and of the increment (each within about 1e-6 of the magnitudes they add). Non-finite
+// maxima (float bits at or above inf's) give no bound.
+__device__ __forceinline__ float grad_bound(unsigned M, unsigned D, unsigned C, float fac)
+{
+  if (M >= 0x7f800000u) return __builtin_inff();
+  const float m = __uint_as_float(M);
+  const float old = m * fac * (1.f + 1.0e-5f);
+  // D and C are passed as inf bits for blocks that meet the PML (whose kernels do not record
+  // them); a non-finite u already shows in M
+  if (D >= 0x7f800000u || C >= 0x7f800000u) return old;
+  const float d = __uint_as_float(D), c = __uint_as_float(C);
+  const float inc = fmaf(d, fac, c) * (1.f + 1.0e-5f) + 1.0e-5f * fac * (m + d);
+  return fminf(old, inc);
+}
+
 // Amplitude bound of a gradient tile (see AmpGrid): true when every gradient component of the
 // tile is provably below the exact maximum the bound pass found for this step (the seed, fixed
 // during the pass). Every wavefront decides on its own from the same loads, so the decision is
@@ -591,25 +706,23 @@ __device__ __forceinline__ bool amp_skip(const GradArgs<S> &a, int x0, int x1, i
   const int zb0 = max(0, (z0 - H) >> AB_Z), zb1 = min(ag.nbz - 1, (z1 - 1 + H) >> AB_Z);
   const int xb0 = max(0, (x0 - H) >> AB_X), xb1 = min(ag.nbx - 1, (x1 - 1 + H) >> AB_X);
   const int yb0 = max(0, (y0 - H) >> AB_Y), yb1 = min(ag.nby - 1, (y1 - 1 + H) >> AB_Y);
+  // the tile's own blocks (for the previous components), inside the halo range above
+  const int zo0 = z0 >> AB_Z, zo1 = (z1 - 1) >> AB_Z, xo0 = x0 >> AB_X, xo1 = (x1 - 1) >> AB_X;
+  const int yo0 = y0 >> AB_Y, yo1 = (y1 - 1) >> AB_Y;
   const int nxb = xb1 - xb0 + 1, nyb = yb1 - yb0 + 1;
   const int nb = (zb1 - zb0 + 1) * nxb * nyb;
-  const unsigned *bb = ag.buf(kstep);
-  unsigned m = 0u;
+  const unsigned *bb = ag.buf(kstep), *db = ag.dbuf(kstep), *cb = ag.cbuf(kstep);
+  unsigned m = 0u, d = 0u, c = 0u;
   for (int i = __lane_id(); i < nb; i += TY) {
-    const int yb = i % nyb, t = i / nyb, xb = t % nxb, zb = t / nxb;
-    m = max(m, bb[((unsigned)(zb0 + zb) * ag.nbx + (unsigned)(xb0 + xb)) * ag.nby + (unsigned)(yb0 + yb)]);
+    const int yb = yb0 + i % nyb, t = i / nyb, xb = xb0 + t % nxb, zb = zb0 + t / nxb;
+    const unsigned o = ((unsigned)zb * ag.nbx + (unsigned)xb) * ag.nby + (unsigned)yb;
+    const bool pb = AmpGrid::pml_block(a.g, zb, xb, yb);
+    m = max(m, bb[o]);
+    d = max(d, pb ? 0x7f800000u : db[o]);
+    if (zb >= zo0 && zb <= zo1 && xb >= xo0 && xb <= xo1 && yb >= yo0 && yb <= yo1) c = max(c, pb ? 0x7f800000u : cb[o]);
   }
-  // wavefront maximum into lane 63 (row prefix maxima, then rows 0 -> 1, 2 -> 3, 1 -> 2 and 3)
-  int v = (int)m;
-  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));
-  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));
-  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));
-  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));
-  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));
-  v = max((unsigned)v, (unsigned)__builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));
-  const unsigned M = (unsigned)__builtin_amdgcn_readlane(v, 63);
+  const unsigned M = wave_max(m), D = wave_max(d), Cm = wave_max(c);
   const float Gk = __uint_as_float(a.gseed[kstep & 1]);
-  // |sum_m g_m (u+m - u-m)| <= 2 sum_m |g_m| max|u|; the factor covers the rounding of the sum
   float fx = 0.f, fy = 0.f, fz = 0.f;
 #pragma unroll
   for (int k = 0; k < H; k++) {
@@ -617,8 +730,7 @@ __device__ __forceinline__ bool amp_skip(const GradArgs<S> &a, int x0, int x1, i
     fy += fabsf(a.c.g1y[k]);
     fz += fabsf(a.c.g1z[k]);
   }
-  const float fac = 2.f * fmaxf(fx, fmaxf(fy, fz)) * (1.f + 1.0e-5f);
-  return M < 0x7f800000u && __uint_as_float(M) * fac < Gk;
+  return grad_bound(M, D, Cm, 2.f * fmaxf(fx, fmaxf(fy, fz))) < Gk;
 }
 
 // One work item of the gradient-max pass (w: packed tile, kstep: time step). Each lane owns a
@@ -736,9 +848,10 @@ __device__ __forceinline__ void gradmax_tile(const GradArgs<S> &a, int4 w, int k
 // Quiet map of step k (after its gradient pass): a block is quiet when the amplitude bound over
 // its 3 x 3 x 3 block neighbourhood (every cell within 5 of it) keeps |grad u| = sqrt(ux^2 + uy^2
 // + uz^2) of each of its cells below the scalar operator's threshold rd(0.01 G_k): there S = 1
-// without reading eps - del. The bound on each component is the gradient pass's, with the
-// update's first derivatives (sum c1 (u+m - u-m), then 1/d) and a margin for their rounding and
-// the hardware square root.
+// without reading eps - del. The bound on each component is the gradient pass's (grad_bound:
+// the smaller of the one from max |u| and the one from the previous step's components plus the
+// increment), with the update's first derivatives (sum c1 (u+m - u-m), then 1/d) and a margin
+// for their rounding and the hardware square root.
 //
 // The same pass keeps the touched bit: set once the neighbourhood's maximum of |u_k| is nonzero
 // or the block meets the source box, and never cleared until the reset. A block without it (at
@@ -751,7 +864,7 @@ struct SrcBox {
   int on, z0, z1, x0, x1, y0, y1;  // inclusive bounds
 };
 
-static __global__ void k_quiet(AmpGrid ag, Coef c, const unsigned *gmax, Step st, const unsigned *hist, SrcBox sb)
+static __global__ void k_quiet(AmpGrid ag, Geo g, Coef c, const unsigned *gmax, Step st, const unsigned *hist, SrcBox sb)
 {
   const int kstep = st.get();
   const unsigned *bb = ag.buf(kstep);
@@ -760,19 +873,26 @@ static __global__ void k_quiet(AmpGrid ag, Coef c, const unsigned *gmax, Step st
   float s1 = 0.f;
 #pragma unroll
   for (int k = 0; k < H; k++) s1 += fabsf(c.c1[k]);
-  const float fac = 2.f * s1 * fmaxf(c.ix, fmaxf(c.iy, c.iz)) * 1.7320509f * (1.f + 1.0e-4f);
+  const float fac = 2.f * s1 * fmaxf(c.ix, fmaxf(c.iy, c.iz));
+  const unsigned *db = ag.dbuf(kstep), *cb = ag.cbuf(kstep);
   for (unsigned b = blockIdx.x * blockDim.x + threadIdx.x; b < ag.nblk; b += gridDim.x * blockDim.x) {
     const int by = (int)(b % ag.nby), t = (int)(b / ag.nby), bx = t % ag.nbx, bz = t / ag.nbx;
-    unsigned m = 0u;
+    unsigned m = 0u, d = 0u;
     for (int z = max(0, bz - 1); z <= min(ag.nbz - 1, bz + 1); z++)
       for (int x = max(0, bx - 1); x <= min(ag.nbx - 1, bx + 1); x++)
-        for (int y = max(0, by - 1); y <= min(ag.nby - 1, by + 1); y++)
-          m = max(m, bb[((unsigned)z * ag.nbx + x) * ag.nby + y]);
+        for (int y = max(0, by - 1); y <= min(ag.nby - 1, by + 1); y++) {
+          const unsigned o = ((unsigned)z * ag.nbx + x) * ag.nby + y;
+          m = max(m, bb[o]);
+          d = max(d, AmpGrid::pml_block(g, z, x, y) ? 0x7f800000u : db[o]);
+        }
+    const unsigned cown = AmpGrid::pml_block(g, bz, bx, by) ? 0x7f800000u : cb[b];
+    // |grad u| <= sqrt(3) times the largest component (1e-4 for the hardware square root)
+    const float gb = grad_bound(m, d, cown, fac) * 1.7320509f * (1.f + 1.0e-4f);
     const bool src = sb.on && sb.z0 < ((bz + 1) << AB_Z) && sb.z1 >= (bz << AB_Z) && sb.x0 < ((bx + 1) << AB_X) &&
                      sb.x1 >= (bx << AB_X) && sb.y0 < ((by + 1) << AB_Y) && sb.y1 >= (by << AB_Y);
     const unsigned qb = ag.qi(bz, bx, by);
     const bool touched = !whole || (ag.q[qb] & 2) != 0 || m != 0u || src;
-    const bool quiet = m < 0x7f800000u && __uint_as_float(m) * fac < thr;
+    const bool quiet = gb < thr;
     ag.q[qb] = (unsigned char)((quiet ? 1 : 0) | (touched ? 2 : 0));
   }
 }
@@ -803,8 +923,12 @@ __global__ __launch_bounds__(T::NT) void k_gbound(GradArgs<S> a)
   __shared__ float sm[2][T::LX + 1][T::LY];
   const int kstep = a.step.get();
   const unsigned tid = threadIdx.y * TY + threadIdx.x;
-  unsigned *clr = a.amp.buf(kstep + 2);
-  for (unsigned i = blockIdx.x * T::NT + tid; i < a.amp.nblk; i += gridDim.x * T::NT) clr[i] = 0u;
+  unsigned *clr = a.amp.buf(kstep + 2), *clrd = a.amp.dbuf(kstep + 2), *clrc = a.amp.cbuf(kstep + 2);
+  for (unsigned i = blockIdx.x * T::NT + tid; i < a.amp.nblk; i += gridDim.x * T::NT) {
+    clr[i] = 0u;
+    clrd[i] = 0u;
+    clrc[i] = 0u;
+  }
   if (blockIdx.x == 0 && tid == 0) {
     a.argkey[kstep & 1] = 0ull;
     a.gseed[(kstep + 1) & 1] = 0u;  // the next step's seed; this step's was cleared by step k-1
@@ -961,7 +1085,9 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
   float gm = 0.f;          // SPEC: G_k over this tile's cells
   bool nonfinite = false;  // SPEC: a non-finite gradient component seen
   const bool amp_on = a.amp.b != nullptr;  // block maxima of |u_next| for the next gradient pass
-  unsigned am = 0u;
+  // running maxima (float bits) of |u_next|, and on interior tiles of |u_next - u_cur| and of
+  // |grad u_cur|
+  unsigned am = 0u, ad = 0u, ac = 0u;
   // source box and receiver plane tests, hoisted to the workgroup and the plane (uniform)
   const bool src_tile = a.src_on && a.sb_x0 < x1 && a.sb_x1 >= x0 &&
                         (fold || (a.sb_y0 < y1 && a.sb_y1 >= y0)) && a.sb_z0 < z1 && a.sb_z1 >= z0;
@@ -1081,6 +1207,13 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
     centre[r] = q[r][H];
   }
   P.stage(sm[0], centre, L);
+  // interior: the workgroup's two LDS sets of block maxima (amp_lds_add), cleared before the
+  // march's first barrier
+  __shared__ unsigned amx[2][3][4];
+  const bool w0 = __builtin_amdgcn_readfirstlane(threadIdx.y) == 0;
+  if constexpr (!PML) {
+    if (amp_on && w0 && __lane_id() < 24) (&amx[0][0][0])[__lane_id()] = 0u;
+  }
   __syncthreads();
   int b = 0;
   int zb = z0;
@@ -1178,13 +1311,20 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
         }
         un = S::rnd(un);
         S::st(a.unext + zo, o, un);
-        am = max(am, __float_as_uint(fabsf(un)));  // unconditional: a branch per cell costs more
+        // unconditional: a branch per cell costs more
+        am = max(am, __float_as_uint(fabsf(un)));
+        if constexpr (!PML) {
+          // |grad u| (mode, computed for S) bounds every component; integer maxima of float
+          // bits need no NaN canonicalisation (mode is never negative)
+          ad = max(ad, __float_as_uint(fabsf(un - c0)));
+          ac = max(ac, __float_as_uint(mode));
+        }
         if (rec_plane && x >= pml && x < nx - pml && y >= pml && y < ny - pml)
           rec[(x - pml) * (ny - 2 * pml) + (y - pml)] = un;
       }
       if constexpr (PML != 0) {
         // the rolled PML march flushes at the end of every z block
-        if (amp_on && ((z & ((1 << AB_Z) - 1)) == (1 << AB_Z) - 1 || !more)) amp_flush(a.amp, kstep, z, z + 1, x0, x1, y, am);
+        if (amp_on && ((z & ((1 << AB_Z) - 1)) == (1 << AB_Z) - 1 || !more)) amp_flush(a.amp, kstep, z, z + 1, x0, x1, y, am, ad, ac);
       }
       if (more) {
 #pragma unroll
@@ -1200,10 +1340,23 @@ __device__ __forceinline__ void update_tile(const UpdArgs<S> &a, int4 w, int kst
     }
   };
   if constexpr (!PML) {
+    // block maxima once per unrolled stretch (its planes may cover up to four z blocks), through
+    // the LDS sets (amp_lds_add): a set is moved out at the end of the next stretch, whose planes
+    // hold a barrier (or get one here), and cleared before it is filled again two stretches on
+    int set = 0;  // this stretch's set; the previous stretch (full length) used the other
     for (; zb < z1; zb += 2 * QN) {
       Unroll<0, 2 * QN>::run(plane, z1 - zb);
-      // once per unrolled stretch (its planes may cover up to four z blocks)
-      if (amp_on) amp_flush(a.amp, kstep, zb, min(zb + 2 * QN, z1), x0, x1, y, am);
+      if (amp_on) {
+        if (z1 - zb < 2) __syncthreads();  // a stretch of one plane passed no barrier
+        if (zb > z0 && w0) amp_lds_flush(a.amp, kstep, amx[set ^ 1], zb - 2 * QN, zb, x0, x1, y0);
+        amp_lds_add(amx[set], y0, y, a.amp.nby, am, ad, ac);
+        set ^= 1;
+      }
+    }
+    if (amp_on) {
+      // the last stretch: from the last multiple of the stretch length past z0 to z1
+      __syncthreads();
+      if (w0) amp_lds_flush(a.amp, kstep, amx[set ^ 1], zb - 2 * QN, z1, x0, x1, y0);
     }
   } else {
     // the PML kernel carries seven more state values per row; a rolled loop with a shifted
@@ -1256,11 +1409,15 @@ __global__ void k_rim_source(const typename S::T *ucur, typename S::T *unext, Ge
     const bool r5 = x >= H && x < g.nx - H && y >= H && y < g.ny - H && z >= H && z < g.nz - H;
     if (r5) continue;
     const unsigned b = 4u * ((unsigned)z * g.pln + (unsigned)x * g.ys + (unsigned)y);
-    const float v = S::rnd(fmaf(sbox[t], wk, S::ld(ucur, b)));
+    const float uc = S::ld(ucur, b);
+    const float v = S::rnd(fmaf(sbox[t], wk, uc));
     S::st(unext, b, v);
-    if (amp.b)
-      atomicMax(amp.buf(kstep + 1) + ((unsigned)(z >> AB_Z) * amp.nbx + (unsigned)(x >> AB_X)) * amp.nby + (y >> AB_Y),
-                __float_as_uint(fabsf(v)));
+    if (amp.b) {
+      // the rim has no gradient of its own (it is outside [5, n-5)^3), only its increment
+      const unsigned o = ((unsigned)(z >> AB_Z) * amp.nbx + (unsigned)(x >> AB_X)) * amp.nby + (y >> AB_Y);
+      atomicMax(amp.buf(kstep + 1) + o, __float_as_uint(fabsf(v)));
+      atomicMax(amp.dbuf(kstep + 1) + o, __float_as_uint(fabsf(v - uc)));
+    }
   }
 }
 

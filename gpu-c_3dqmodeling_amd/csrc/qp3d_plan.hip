@@ -456,7 +456,8 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
     ag.nbx = (c.nx + (1 << AB_X) - 1) >> AB_X;
     ag.nby = (c.ny + (1 << AB_Y) - 1) >> AB_Y;
     ag.nblk = (unsigned)ag.nbz * ag.nbx * ag.nby;
-    if ((rc = dmalloc((void **)&ag.b, sizeof(unsigned) * 3 * (size_t)ag.nblk, "block maxima"))) return fail(rc);
+    // three rotating slots each of max |u|, max |u_{k+1} - u_k|, max gradient component of u_k
+    if ((rc = dmalloc((void **)&ag.b, sizeof(unsigned) * 9 * (size_t)ag.nblk, "block maxima"))) return fail(rc);
     if ((rc = dmalloc((void **)&ag.q, (size_t)ag.nblk, "quiet map"))) return fail(rc);
     // keys of the largest tile [2], two spare, bound-pass maxima (two words), the rest history
     // word; the counters of instrumented steps
@@ -464,7 +465,7 @@ extern "C" int qp3d_plan_create(const qp3d_config *cfg, qp3d_plan **out)
       return fail(rc);
     if ((rc = dmalloc((void **)&p->cnt, CNT_BYTES, "counters"))) return fail(rc);
     p->rflag = (unsigned *)(p->argkey + 5);
-    if (hipMemset(ag.b, 0, sizeof(unsigned) * 3 * (size_t)ag.nblk) != hipSuccess ||
+    if (hipMemset(ag.b, 0, sizeof(unsigned) * 9 * (size_t)ag.nblk) != hipSuccess ||
         hipMemset(ag.q, 0, (size_t)ag.nblk) != hipSuccess ||
         hipMemset(p->argkey, 0, ARG_WORDS * sizeof(unsigned long long)) != hipSuccess ||
         hipMemset(p->cnt, 0, CNT_BYTES) != hipSuccess || hipMemset(p->rflag, 1, sizeof(unsigned)) != hipSuccess) {
@@ -897,7 +898,7 @@ extern "C" int qp3d_plan_reset(qp3d_plan *p)
     HIP_TRY(hipMemsetAsync(p->gpred, 0, sizeof(float) * (size_t)(p->cfg.nt > 0 ? p->cfg.nt : 1), p->stream));
   if (p->rec) HIP_TRY(hipMemsetAsync(p->rec, 0, sizeof(float) * (size_t)p->cfg.nt * p->hx * p->hy, p->stream));
   if (p->ampg.b) {
-    HIP_TRY(hipMemsetAsync(p->ampg.b, 0, sizeof(unsigned) * 3 * (size_t)p->ampg.nblk, p->stream));
+    HIP_TRY(hipMemsetAsync(p->ampg.b, 0, sizeof(unsigned) * 9 * (size_t)p->ampg.nblk, p->stream));
     HIP_TRY(hipMemsetAsync(p->argkey, 0, ARG_WORDS * sizeof(unsigned long long), p->stream));
     HIP_TRY(hipMemsetAsync(p->cnt, 0, CNT_BYTES, p->stream));
     // every block at rest, and a history from here
@@ -997,7 +998,7 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on
     if (use_amp && ga.skip) {
       // G_k is complete: mark the blocks where S = 1, and those at rest, for the updates
       const SrcBox sb{p->src_on ? 1 : 0, p->sb[0], p->sb[1], p->sb[2], p->sb[3], p->sb[4], p->sb[5]};
-      hipLaunchKernelGGL(k_quiet, dim3((p->ampg.nblk + 255) / 256), dim3(256), 0, s0, p->ampg, p->coef,
+      hipLaunchKernelGGL(k_quiet, dim3((p->ampg.nblk + 255) / 256), dim3(256), 0, s0, p->ampg, p->geo, p->coef,
                          (const unsigned *)p->gmax, st, (const unsigned *)p->rflag, sb);
       HIP_TRY(hipGetLastError());
       ua.amp.q = p->ampg.q;
