@@ -426,6 +426,8 @@ extern "C" int qp3d_batch_run(qp3d_batch *b, int k0, int k1)
       return QP3D_ERR_STATE;
     }
   HIP_TRY(hipSetDevice(b->device));
+  // the batch kernels leave the plans' block maxima stale (also when a run stops on an error)
+  for (qp3d_plan *p : b->plans) p->amp_ok = false;
   if (b->persist && !b->per_shot) {
     if (!b->pbuilt) {
       int rc = build_persist(b);
@@ -437,10 +439,7 @@ extern "C" int qp3d_batch_run(qp3d_batch *b, int k0, int k1)
       if (rc) return rc;
     }
     b->steps_done = k1;
-    for (qp3d_plan *p : b->plans) {
-      p->steps_done = k1;
-      p->amp_ok = false;
-    }
+    for (qp3d_plan *p : b->plans) p->steps_done = k1;
     return QP3D_OK;
   }
   const bool graph = b->use_graph && !b->timing;
@@ -461,10 +460,7 @@ extern "C" int qp3d_batch_run(qp3d_batch *b, int k0, int k1)
     }
   }
   b->steps_done = k1;
-  for (qp3d_plan *p : b->plans) {
-    p->steps_done = k1;
-    p->amp_ok = false;  // the batch kernels leave the plans' block maxima stale
-  }
+  for (qp3d_plan *p : b->plans) p->steps_done = k1;
   return QP3D_OK;
 }
 

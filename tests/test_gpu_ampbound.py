@@ -91,9 +91,12 @@ def test_bound_with_a_source_on_the_frozen_rim():
         same(run(p), ref)
 
 
-def test_bound_after_a_batch_drove_the_plan():
-    """A batch leaves the plan's block maxima stale: the plan's next run from the batch's state
-    must evaluate every tile of its first step (no skipping) and match a run without the bound."""
+@pytest.mark.parametrize("persistent", [False, True])
+def test_bound_after_a_batch_drove_the_plan(persistent):
+    """A batch (launches per stage, or the persistent launch) leaves the plan's block maxima
+    stale and its rest history broken: the plan's next run from the batch's state must evaluate
+    every tile of its first step, count every block as touched, and match a run without the
+    bound."""
     model = qp3d.genmodel(70, 64, 30, 3)
     setups = [qp3d.window_setup(model, x0, y0, hx, hy, pml=10, nt=60, fm=20.0, src_depth=2)
               for x0, y0, hx, hy in [(0, 0, 31, 27), (20, 9, 44, 40)]]
@@ -104,6 +107,8 @@ def test_bound_after_a_batch_drove_the_plan():
             for p in plans:
                 p.set_amp_bound(bound)
             with qp3d.Batch(plans) as b:
+                if persistent:
+                    b.set_persistent(0, False)
                 b.reset()
                 b.run(0, 30)
                 b.sync()
