@@ -37,6 +37,14 @@ struct qp3d_batch {
   // launched with wl[3] (update kind 1 against kind 2)
   WorkList wl[5];
   void *args_g[2] = {}, *args_u[2] = {};  // per-parity argument blocks of every shot
+  // the same for the persistent launch, without the amplitude bound (no bound pass there)
+  void *args_gp[2] = {}, *args_up[2] = {};
+  // amplitude bound of the gradient pass: the stage launches record each shot's block maxima
+  // and k_gbound_batch seeds each shot's G; amp_ok: the maxima cover every shot's u (false after
+  // persistent or per-shot steps, which do not record them)
+  bool amp_any = false;
+  bool amp_ok = true;
+  int *first = nullptr;  // [nshots] an item of each shot in the gradient list (the bound pass's fallback)
   int *kdev = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -57,6 +65,8 @@ struct qp3d_batch {
   bool pbuilt = false;
   double wait_us = 5e6;          // bound of one dependency wait
 };
+
+constexpr int GBOUND_BLOCKS_BATCH = 32;  // workgroups of the bound pass per shot
 
 // Work list of one stage over all shots, items in batch format (stage 3 fills wl[3] and wl[4]).
 static int build_stage(qp3d_batch *b, int stage)
@@ -79,8 +89,8 @@ static int build_stage(qp3d_batch *b, int stage)
     const int o = stage == 3 ? occ[p->yw ? 2 : 1] : occ[0];
     const int cap = std::max(1, o * qp3d_cu_count(p) / (b->share ? n : 1));
     std::vector<int4> it, ip, ipy;
-    if (stage == 0)
-      qp3d_items_uniform(p, b->tile[0], cap, it);
+    if (stage == 0)  // 32-plane items where the amplitude bound can skip them
+      qp3d_items_uniform(p, b->tile[0], cap, it, p->ampg.b && p->amp_on ? AMP_ZCHUNK : 0);
     else
       qp3d_items_update(p, b->tile[1], b->tile[2], cap, cap, it, ip, ipy, b->tile[3], cap);
     const std::vector<int4> &v = stage == 3 ? ipy : stage == 2 ? ip : it;
@@ -91,6 +101,23 @@ static int build_stage(qp3d_batch *b, int stage)
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->gexec = nullptr;
   b->graph = nullptr;
+  if (stage == 0) {
+    // each shot's first item in the uploaded (XCD) order: empty padding items name shot 0
+    std::vector<int4> ord;
+    (void)qp3d_order_items(all, true, ord);
+    std::vector<int> first(n, 0);
+    std::vector<char> seen(n, 0);
+    for (size_t j = 0; j < ord.size(); j++) {
+      const int4 w = ord[j];
+      const int sh = w.w;
+      if (sh >= 0 && sh < n && !seen[sh] && (w.z & 0xffff) < (int)((unsigned)w.z >> 16)) {
+        seen[sh] = 1;
+        first[sh] = (int)j;
+      }
+    }
+    if (!b->first) HIP_TRY(hipMalloc((void **)&b->first, sizeof(int) * (size_t)n));
+    HIP_TRY(hipMemcpy(b->first, first.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice));
+  }
   int rc = qp3d_upload_items(b->wl[stage], all, true);
   if (!rc && stage == 3) rc = qp3d_upload_items(b->wl[4], all1, true);
   return rc;
@@ -100,23 +127,44 @@ template <class S>
 static int upload_args(qp3d_batch *b)
 {
   const size_t n = b->plans.size();
+  b->amp_any = false;
   for (int par = 0; par < 2; par++) {
-    std::vector<GradArgs<S>> g(n);
-    std::vector<UpdArgs<S>> u(n);
+    std::vector<GradArgs<S>> g(n), gp(n);
+    std::vector<UpdArgs<S>> u(n), up(n);
     for (size_t i = 0; i < n; i++) {
-      make_args<S>(b->plans[i], par, Step{nullptr, 0}, g[i], u[i]);
-      // no amplitude bound in batches: every gradient tile is evaluated
-      g[i].amp = AmpGrid{};
-      g[i].argkey = nullptr;
-      g[i].skip = 0;
+      qp3d_plan *p = b->plans[i];
+      make_args<S>(p, par, Step{nullptr, 0}, g[i], u[i]);
       g[i].gcount = nullptr;
-      g[i].gseed = g[i].gseed_out = nullptr;
-      u[i].amp = AmpGrid{};
+      g[i].gseed_out = nullptr;
+      u[i].amp.q = nullptr;  // no quiet or rest map in batches
+      const bool amp = p->ampg.b && p->amp_on;
+      if (amp) {
+        // the shot's block maxima, key and seed: tiles under the bound are skipped
+        g[i].skip = 1;
+        b->amp_any = true;
+      } else {
+        g[i].amp = AmpGrid{};
+        g[i].argkey = nullptr;
+        g[i].skip = 0;
+        g[i].gseed = nullptr;
+        u[i].amp = AmpGrid{};
+      }
+      gp[i] = g[i];
+      up[i] = u[i];
+      gp[i].amp = AmpGrid{};
+      gp[i].argkey = nullptr;
+      gp[i].skip = 0;
+      gp[i].gseed = nullptr;
+      up[i].amp = AmpGrid{};
     }
     HIP_TRY(hipMalloc(&b->args_g[par], sizeof(GradArgs<S>) * n));
     HIP_TRY(hipMalloc(&b->args_u[par], sizeof(UpdArgs<S>) * n));
+    HIP_TRY(hipMalloc(&b->args_gp[par], sizeof(GradArgs<S>) * n));
+    HIP_TRY(hipMalloc(&b->args_up[par], sizeof(UpdArgs<S>) * n));
     HIP_TRY(hipMemcpy(b->args_g[par], g.data(), sizeof(GradArgs<S>) * n, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b->args_u[par], u.data(), sizeof(UpdArgs<S>) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->args_gp[par], gp.data(), sizeof(GradArgs<S>) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->args_up[par], up.data(), sizeof(UpdArgs<S>) * n, hipMemcpyHostToDevice));
   }
   return QP3D_OK;
 }
@@ -197,8 +245,8 @@ static int persist_run_s(qp3d_batch *b, int k0, int k1)
   (void)hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device);
   PersistArgs<S> a;
   for (int par = 0; par < 2; par++) {
-    a.ag[par] = (const GradArgs<S> *)b->args_g[par];
-    a.au[par] = (const UpdArgs<S> *)b->args_u[par];
+    a.ag[par] = (const GradArgs<S> *)b->args_gp[par];
+    a.au[par] = (const UpdArgs<S> *)b->args_up[par];
   }
   a.items = b->pitems;
   a.ips = b->pips;
@@ -223,6 +271,7 @@ static int batch_step_s(qp3d_batch *b, int k, Step st)
   using T = typename S::T;
   const int par = k & 1;
   if (b->per_shot) {
+    b->amp_ok = false;  // the plans' own steps here do not record the block maxima
     for (qp3d_plan *p : b->plans) {
       const int rc = qp3d_forward_step_on(p, k, st, b->stream);
       if (rc) return rc;
@@ -234,9 +283,17 @@ static int batch_step_s(qp3d_batch *b, int k, Step st)
     if (tm) HIP_TRY(hipEventRecord(b->ev[0], b->stream));
     const UpdArgs<S> *A = (const UpdArgs<S> *)b->args_u[par];
     if (b->wl[s].n) {
-      if (s == 0)
-        launch_gradmax_batch<S>(b->tile[0], b->wl[0].n, b->wl[0].d, b->stream, (const GradArgs<S> *)b->args_g[par], st);
-      else if (s == 1)
+      if (s == 0) {
+        const GradArgs<S> *G = (const GradArgs<S> *)b->args_g[par];
+        // the bound pass of every shot (seed of G_k; clears the maxima step k+1 fills), then the
+        // pass itself, skipping only while every shot's maxima cover its u
+        if (b->amp_any) {
+          launch_gbound_batch<S>(b->tile[0], GBOUND_BLOCKS_BATCH, (int)b->plans.size(), b->wl[0].d, b->wl[0].n,
+                                 b->first, b->stream, G, st);
+          HIP_TRY(hipGetLastError());
+        }
+        launch_gradmax_batch<S>(b->tile[0], b->wl[0].n, b->wl[0].d, b->stream, G, st, b->amp_ok ? 1 : 0);
+      } else if (s == 1)
         launch_update_batch<S, 0>(b->tile[1], b->wl[1].n, b->wl[1].d, b->stream, A, st);
       else if (s == 2)
         launch_update_batch<S, 1>(b->tile[2], b->wl[2].n, b->wl[2].d, b->stream, A, st);
@@ -264,9 +321,10 @@ static int batch_step_s(qp3d_batch *b, int k, Step st)
     const int blocks = (int)std::min<long>((nb + 255) / 256, 4096);
     hipLaunchKernelGGL(k_rim_source<S>, dim3(blocks), dim3(256), 0, b->stream, (const T *)p->u[cur], (T *)p->u[prev],
                        p->geo, p->sb[0], p->sb[1], p->sb[2], p->sb[3], p->sb[4], p->sb[5], (const float *)p->sbox,
-                       (const float *)p->wav, st, AmpGrid{});
+                       (const float *)p->wav, st, b->amp_any && p->amp_on ? p->ampg : AmpGrid{});
     HIP_TRY(hipGetLastError());
   }
+  b->amp_ok = true;  // u(k+1) of every shot is covered by the maxima these launches recorded
   return QP3D_OK;
 }
 
@@ -288,7 +346,10 @@ static void batch_free(qp3d_batch *b)
   for (int i = 0; i < 2; i++) {
     (void)hipFree(b->args_g[i]);
     (void)hipFree(b->args_u[i]);
+    (void)hipFree(b->args_gp[i]);
+    (void)hipFree(b->args_up[i]);
   }
+  (void)hipFree(b->first);
   (void)hipFree(b->kdev);
   (void)hipFree(b->pitems);
   (void)hipFree(b->pnitems);
@@ -384,9 +445,14 @@ extern "C" int qp3d_batch_reset(qp3d_batch *b)
     HIP_TRY(hipMemsetAsync(p->gmax, 0, sizeof(unsigned) * (size_t)b->nt, b->stream));
     HIP_TRY(hipMemsetAsync(p->flag, 0, sizeof(int), b->stream));
     if (p->rec) HIP_TRY(hipMemsetAsync(p->rec, 0, sizeof(float) * (size_t)b->nt * p->hx * p->hy, b->stream));
+    if (p->ampg.b) {
+      HIP_TRY(hipMemsetAsync(p->ampg.b, 0, sizeof(unsigned) * 9 * (size_t)p->ampg.nblk, b->stream));
+      HIP_TRY(hipMemsetAsync(p->argkey, 0, ARG_WORDS * sizeof(unsigned long long), b->stream));
+    }
     p->steps_done = 0;
   }
   b->steps_done = 0;
+  b->amp_ok = true;  // u = 0 everywhere, and so are the block maxima
   return QP3D_OK;
 }
 
@@ -394,6 +460,8 @@ static int batch_capture(qp3d_batch *b)
 {
   HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   int rc = QP3D_OK;
+  // every replay starts with the block maxima covering u (qp3d_batch_run checks amp_ok)
+  b->amp_ok = true;
   for (int i = 0; i < GRAPH_LEN && rc == QP3D_OK; i++) rc = batch_step(b, i, Step{b->kdev, i});
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(b->stream, &g);
@@ -434,6 +502,7 @@ extern "C" int qp3d_batch_run(qp3d_batch *b, int k0, int k1)
       if (rc) return rc;
     }
     if (k1 > k0) {
+      b->amp_ok = false;  // the persistent launch does not record the block maxima
       int rc = QP3D_OK;
       with_storage(b->storage, [&](auto sp) { rc = persist_run_s<decltype(sp)>(b, k0, k1); });
       if (rc) return rc;
@@ -444,7 +513,7 @@ extern "C" int qp3d_batch_run(qp3d_batch *b, int k0, int k1)
   }
   const bool graph = b->use_graph && !b->timing;
   for (int k = k0; k < k1;) {
-    if (graph && (k & 1) == 0 && k + GRAPH_LEN <= k1) {
+    if (graph && (k & 1) == 0 && k + GRAPH_LEN <= k1 && (b->amp_ok || !b->amp_any)) {
       if (!b->gexec) {
         int rc = batch_capture(b);
         if (rc) return rc;

@@ -68,13 +68,29 @@ int occupancy_gradmax(int tile)
 }
 
 template <class S>
-void launch_gradmax_batch(int tile, int blocks, const int4 *work, hipStream_t st, const GradArgs<S> *A, Step s)
+void launch_gradmax_batch(int tile, int blocks, const int4 *work, hipStream_t st, const GradArgs<S> *A, Step s,
+                          int skip)
 {
   int i = 0;
-#define QP3D_L(wx, rx)                                                                                 \
-  if (tile == i++) {                                                                                   \
-    hipLaunchKernelGGL((k_gradmax_batch<S, Tile<wx, rx>>), dim3(blocks), dim3(TY, wx), 0, st, A, work, s); \
-    return;                                                                                            \
+#define QP3D_L(wx, rx)                                                                                       \
+  if (tile == i++) {                                                                                         \
+    hipLaunchKernelGGL((k_gradmax_batch<S, Tile<wx, rx>>), dim3(blocks), dim3(TY, wx), 0, st, A, work, s, skip); \
+    return;                                                                                                  \
+  }
+  QP3D_TILES(QP3D_L)
+#undef QP3D_L
+}
+
+template <class S>
+void launch_gbound_batch(int tile, int blocks, int nshots, const int4 *work, int nwork, const int *first,
+                         hipStream_t st, const GradArgs<S> *A, Step s)
+{
+  int i = 0;
+#define QP3D_L(wx, rx)                                                                                   \
+  if (tile == i++) {                                                                                     \
+    hipLaunchKernelGGL((k_gbound_batch<S, Tile<wx, rx>>), dim3(blocks, nshots), dim3(TY, wx), 0, st, A, work, \
+                       nwork, first, s);                                                                 \
+    return;                                                                                              \
   }
   QP3D_TILES(QP3D_L)
 #undef QP3D_L
@@ -95,7 +111,10 @@ template void launch_gradmax<QP3D_STORE>(int, int, hipStream_t, const GradArgs<Q
 template void launch_mark<QP3D_STORE>(int, int, hipStream_t, const GradArgs<QP3D_STORE> &);
 template void launch_gbound<QP3D_STORE>(int, int, hipStream_t, const GradArgs<QP3D_STORE> &);
 template int occupancy_gradmax<QP3D_STORE>(int);
-template void launch_gradmax_batch<QP3D_STORE>(int, int, const int4 *, hipStream_t, const GradArgs<QP3D_STORE> *, Step);
+template void launch_gradmax_batch<QP3D_STORE>(int, int, const int4 *, hipStream_t, const GradArgs<QP3D_STORE> *, Step,
+                                                int);
+template void launch_gbound_batch<QP3D_STORE>(int, int, int, const int4 *, int, const int *, hipStream_t,
+                                              const GradArgs<QP3D_STORE> *, Step);
 template int occupancy_gradmax_batch<QP3D_STORE>(int);
 
 }  // namespace qp3d

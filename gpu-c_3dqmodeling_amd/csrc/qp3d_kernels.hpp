@@ -917,11 +917,10 @@ __global__ __launch_bounds__(T::NT) void k_gradmax(GradArgs<S> a)
 // gradient component at step k-1, split in z over the grid, evaluated exactly, so that the
 // skipping pass starts from a G_k close to the final one. It also clears the block maxima that
 // the updates of step k+1 fill ((k+2) % 3) and the key of this step's largest tile.
+// The bound pass's work on one workgroup: the clearing share and one z piece of tile w.
 template <class S, class T>
-__global__ __launch_bounds__(T::NT) void k_gbound(GradArgs<S> a)
+__device__ __forceinline__ void gbound_tile(const GradArgs<S> &a, int4 w, int kstep, TileLds<T> sm)
 {
-  __shared__ float sm[2][T::LX + 1][T::LY];
-  const int kstep = a.step.get();
   const unsigned tid = threadIdx.y * TY + threadIdx.x;
   unsigned *clr = a.amp.buf(kstep + 2), *clrd = a.amp.dbuf(kstep + 2), *clrc = a.amp.cbuf(kstep + 2);
   for (unsigned i = blockIdx.x * T::NT + tid; i < a.amp.nblk; i += gridDim.x * T::NT) {
@@ -933,10 +932,6 @@ __global__ __launch_bounds__(T::NT) void k_gbound(GradArgs<S> a)
     a.argkey[kstep & 1] = 0ull;
     a.gseed[(kstep + 1) & 1] = 0u;  // the next step's seed; this step's was cleared by step k-1
   }
-  const unsigned long long key = a.argkey[(kstep + 1) & 1];  // written by step k-1
-  int item = (int)(key & 0xffffffffull);
-  if (item < 0 || item >= a.nwork) item = 0;
-  int4 w = a.work[item];
   const int len = w.w - w.z;
   const int c = (len + (int)gridDim.x - 1) / (int)gridDim.x;
   const int s0 = w.z + (int)blockIdx.x * c, s1 = min(w.w, s0 + c);
@@ -950,6 +945,36 @@ __global__ __launch_bounds__(T::NT) void k_gbound(GradArgs<S> a)
   gradmax_tile<S, T>(b, w, kstep, sm);
 }
 
+template <class S, class T>
+__global__ __launch_bounds__(T::NT) void k_gbound(GradArgs<S> a)
+{
+  __shared__ float sm[2][T::LX + 1][T::LY];
+  const int kstep = a.step.get();
+  const unsigned long long key = a.argkey[(kstep + 1) & 1];  // written by step k-1
+  int item = (int)(key & 0xffffffffull);
+  if (item < 0 || item >= a.nwork) item = 0;
+  gbound_tile<S, T>(a, a.work[item], kstep, sm);
+}
+
+// The bound pass of every shot of a batch (blockIdx.y: shot) over the batch's shared gradient
+// list, whose items carry their shot: a key that names another shot's item (or none) falls back
+// to one of the shot's own. Shots without block maxima leave at once.
+template <class S, class T>
+__global__ __launch_bounds__(T::NT) void k_gbound_batch(const GradArgs<S> *A, const int4 *work, int nwork,
+                                                        const int *first, Step st)
+{
+  __shared__ float sm[2][T::LX + 1][T::LY];
+  const int shot = (int)blockIdx.y;
+  const GradArgs<S> a = A[shot];
+  if (!a.amp.b || !a.argkey) return;
+  const int kstep = st.get();
+  const unsigned long long key = a.argkey[(kstep + 1) & 1];
+  int item = (int)(key & 0xffffffffull);
+  if (item < 0 || item >= nwork || work[item].w != shot) item = first[shot];
+  const int4 v = work[item];
+  gbound_tile<S, T>(a, make_int4(v.x, v.y, v.z & 0xffff, (int)((unsigned)v.z >> 16)), kstep, sm);
+}
+
 // Marking pass of a step whose predicted threshold missed (gmax_mode 2); see gradmax_tile.
 template <class S, class T>
 __global__ __launch_bounds__(T::NT) void k_spec_mark(GradArgs<S> a)
@@ -958,14 +983,17 @@ __global__ __launch_bounds__(T::NT) void k_spec_mark(GradArgs<S> a)
   gradmax_tile<S, T, true>(a, a.work[blockIdx.x], a.step.get(), sm);
 }
 
+// skip 0: every tile evaluated (the shots' block maxima do not cover u); otherwise each shot's
+// own argument block decides (its maxima and seed from k_gbound_batch)
 template <class S, class T>
-__global__ __launch_bounds__(T::NT) void k_gradmax_batch(const GradArgs<S> *A, const int4 *work, Step st)
+__global__ __launch_bounds__(T::NT) void k_gradmax_batch(const GradArgs<S> *A, const int4 *work, Step st, int skip)
 {
   int shot;
   const int4 w = batch_item(work, shot);
-  const GradArgs<S> a = A[shot];
+  GradArgs<S> a = A[shot];
+  if (!skip) a.skip = 0;
   __shared__ float sm[2][T::LX + 1][T::LY];
-  gradmax_tile<S, T>(a, w, st.get(), sm);
+  gradmax_tile<S, T>(a, w, st.get(), sm, (int)blockIdx.x);
 }
 
 

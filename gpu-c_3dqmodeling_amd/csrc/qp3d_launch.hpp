@@ -30,7 +30,12 @@ int occupancy_update(int tile, bool spec);
 
 // batched forms (argument blocks in device memory, items carry the grid index)
 template <class S>
-void launch_gradmax_batch(int tile, int blocks, const int4 *work, hipStream_t st, const GradArgs<S> *A, Step s);
+void launch_gradmax_batch(int tile, int blocks, const int4 *work, hipStream_t st, const GradArgs<S> *A, Step s,
+                          int skip);
+// bound pass of every shot of a batch: `blocks` workgroups per shot
+template <class S>
+void launch_gbound_batch(int tile, int blocks, int nshots, const int4 *work, int nwork, const int *first,
+                         hipStream_t st, const GradArgs<S> *A, Step s);
 template <class S, int PML>
 void launch_update_batch(int tile, int blocks, const int4 *work, hipStream_t st, const UpdArgs<S> *A, Step s);
 template <class S>
