@@ -43,8 +43,15 @@ struct qp3d_batch {
   // and k_gbound_batch seeds each shot's G; amp_ok: the maxima cover every shot's u (false after
   // persistent or per-shot steps, which do not record them)
   bool amp_any = false;
-  bool amp_ok = true;
+  bool amp_ok = false;  // until the first reset
+  // autotuner: every stage does its full work (no tile skipped, no map), so that the timings
+  // compare the shapes and modes at a fully active wavefield, not at rest
+  bool tuning = false;
   int *first = nullptr;  // [nshots] an item of each shot in the gradient list (the bound pass's fallback)
+  // quiet and rest maps of every shot (k_quiet_batch after the gradient pass; the updates read
+  // them on steps whose maxima cover u)
+  QuietArgs *qargs = nullptr;
+  unsigned qblocks = 0;  // the largest shot's block count
   int *kdev = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -136,11 +143,12 @@ static int upload_args(qp3d_batch *b)
       make_args<S>(p, par, Step{nullptr, 0}, g[i], u[i]);
       g[i].gcount = nullptr;
       g[i].gseed_out = nullptr;
-      u[i].amp.q = nullptr;  // no quiet or rest map in batches
+      u[i].amp.q = nullptr;
       const bool amp = p->ampg.b && p->amp_on;
       if (amp) {
-        // the shot's block maxima, key and seed: tiles under the bound are skipped
+        // the shot's block maxima, key and seed: tiles under the bound are skipped; its maps
         g[i].skip = 1;
+        u[i].amp.q = p->ampg.q;
         b->amp_any = true;
       } else {
         g[i].amp = AmpGrid{};
@@ -166,6 +174,21 @@ static int upload_args(qp3d_batch *b)
     HIP_TRY(hipMemcpy(b->args_gp[par], gp.data(), sizeof(GradArgs<S>) * n, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b->args_up[par], up.data(), sizeof(UpdArgs<S>) * n, hipMemcpyHostToDevice));
   }
+  std::vector<QuietArgs> qa(n);
+  b->qblocks = 0;
+  for (size_t i = 0; i < n; i++) {
+    qp3d_plan *p = b->plans[i];
+    QuietArgs &q = qa[i];
+    q.ag = p->ampg.b && p->amp_on ? p->ampg : AmpGrid{};
+    q.g = p->geo;
+    q.c = p->coef;
+    q.gmax = p->gmax;
+    q.hist = p->rflag;
+    q.sb = SrcBox{p->src_on ? 1 : 0, p->sb[0], p->sb[1], p->sb[2], p->sb[3], p->sb[4], p->sb[5]};
+    if (q.ag.b) b->qblocks = std::max(b->qblocks, q.ag.nblk);
+  }
+  HIP_TRY(hipMalloc((void **)&b->qargs, sizeof(QuietArgs) * n));
+  HIP_TRY(hipMemcpy(b->qargs, qa.data(), sizeof(QuietArgs) * n, hipMemcpyHostToDevice));
   return QP3D_OK;
 }
 
@@ -278,6 +301,13 @@ static int batch_step_s(qp3d_batch *b, int k, Step st)
     }
     return QP3D_OK;
   }
+  // this step's maps: only where the block maxima cover u (as the gradient pass's skipping);
+  // a step without them ends the shots' rest histories (never inside a capture, which starts
+  // with the maxima covering u)
+  const int maps = b->amp_any && b->amp_ok && !b->tuning ? 1 : 0;
+  if (b->amp_any && !maps)
+    for (qp3d_plan *p : b->plans)
+      if (p->rflag) HIP_TRY(hipMemsetAsync(p->rflag, 0, sizeof(unsigned), b->stream));
   for (int s = 0; s < 4; s++) {
     const bool tm = b->timing && s == b->timed_stage;
     if (tm) HIP_TRY(hipEventRecord(b->ev[0], b->stream));
@@ -292,17 +322,23 @@ static int batch_step_s(qp3d_batch *b, int k, Step st)
                                  b->first, b->stream, G, st);
           HIP_TRY(hipGetLastError());
         }
-        launch_gradmax_batch<S>(b->tile[0], b->wl[0].n, b->wl[0].d, b->stream, G, st, b->amp_ok ? 1 : 0);
+        launch_gradmax_batch<S>(b->tile[0], b->wl[0].n, b->wl[0].d, b->stream, G, st, b->amp_ok && !b->tuning ? 1 : 0);
+        if (maps) {
+          // G_k of every shot is complete: its quiet and rest maps for the updates
+          HIP_TRY(hipGetLastError());
+          hipLaunchKernelGGL(k_quiet_batch, dim3((b->qblocks + 255) / 256, (unsigned)b->plans.size()), dim3(256), 0,
+                             b->stream, (const QuietArgs *)b->qargs, st);
+        }
       } else if (s == 1)
-        launch_update_batch<S, 0>(b->tile[1], b->wl[1].n, b->wl[1].d, b->stream, A, st);
+        launch_update_batch<S, 0>(b->tile[1], b->wl[1].n, b->wl[1].d, b->stream, A, st, maps);
       else if (s == 2)
-        launch_update_batch<S, 1>(b->tile[2], b->wl[2].n, b->wl[2].d, b->stream, A, st);
+        launch_update_batch<S, 1>(b->tile[2], b->wl[2].n, b->wl[2].d, b->stream, A, st, maps);
       else
-        launch_update_batch<S, 2>(b->tile[3], b->wl[3].n, b->wl[3].d, b->stream, A, st);
+        launch_update_batch<S, 2>(b->tile[3], b->wl[3].n, b->wl[3].d, b->stream, A, st, maps);
       HIP_TRY(hipGetLastError());
     }
     if (s == 3 && b->wl[4].n) {
-      launch_update_batch<S, 1>(b->tile[3], b->wl[4].n, b->wl[4].d, b->stream, A, st);
+      launch_update_batch<S, 1>(b->tile[3], b->wl[4].n, b->wl[4].d, b->stream, A, st, maps);
       HIP_TRY(hipGetLastError());
     }
     if (tm) {
@@ -350,6 +386,7 @@ static void batch_free(qp3d_batch *b)
     (void)hipFree(b->args_up[i]);
   }
   (void)hipFree(b->first);
+  (void)hipFree(b->qargs);
   (void)hipFree(b->kdev);
   (void)hipFree(b->pitems);
   (void)hipFree(b->pnitems);
@@ -448,6 +485,9 @@ extern "C" int qp3d_batch_reset(qp3d_batch *b)
     if (p->ampg.b) {
       HIP_TRY(hipMemsetAsync(p->ampg.b, 0, sizeof(unsigned) * 9 * (size_t)p->ampg.nblk, b->stream));
       HIP_TRY(hipMemsetAsync(p->argkey, 0, ARG_WORDS * sizeof(unsigned long long), b->stream));
+      // every block at rest, and a rest history from here
+      HIP_TRY(hipMemsetAsync(p->ampg.q, 0, (size_t)p->ampg.nblk, b->stream));
+      HIP_TRY(hipMemsetAsync(p->rflag, 1, sizeof(unsigned), b->stream));
     }
     p->steps_done = 0;
   }
@@ -502,7 +542,11 @@ extern "C" int qp3d_batch_run(qp3d_batch *b, int k0, int k1)
       if (rc) return rc;
     }
     if (k1 > k0) {
-      b->amp_ok = false;  // the persistent launch does not record the block maxima
+      // the persistent launch records no block maxima and keeps no map: the shots' rest
+      // histories end here
+      b->amp_ok = false;
+      for (qp3d_plan *p : b->plans)
+        if (p->rflag) HIP_TRY(hipMemsetAsync(p->rflag, 0, sizeof(unsigned), b->stream));
       int rc = QP3D_OK;
       with_storage(b->storage, [&](auto sp) { rc = persist_run_s<decltype(sp)>(b, k0, k1); });
       if (rc) return rc;
@@ -623,6 +667,7 @@ extern "C" int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, dou
   HIP_TRY(hipSetDevice(b->device));
   const bool graph = b->use_graph;
   b->use_graph = false;
+  b->tuning = true;
   int tiles[2][4] = {}, rc = QP3D_OK;
   double ms[2][4] = {}, t[3] = {0, 0, 0};
   b->per_shot = false;
@@ -648,6 +693,7 @@ extern "C" int qp3d_batch_autotune(qp3d_batch *b, int steps, int *tiles_out, dou
       b->persist = false;
     }
   b->use_graph = graph;
+  b->tuning = false;
   if (rc) return rc;
   const int best = t[2] < std::min(t[0], t[1]) ? 2 : (t[1] < t[0] ? 1 : 0);
   const int sh = best == 1 ? 1 : 0;  // the shared launches' form kept (also behind per-shot mode)

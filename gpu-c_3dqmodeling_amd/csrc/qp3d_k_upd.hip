@@ -53,13 +53,14 @@ int occupancy_update(int tile, bool spec)
 }
 
 template <class S, int PML>
-void launch_update_batch(int tile, int blocks, const int4 *work, hipStream_t st, const UpdArgs<S> *A, Step s)
+void launch_update_batch(int tile, int blocks, const int4 *work, hipStream_t st, const UpdArgs<S> *A, Step s,
+                         int maps)
 {
   int i = 0;
-#define QP3D_L(wx, rx)                                                                                       \
-  if (tile == i++) {                                                                                         \
-    hipLaunchKernelGGL((k_update_batch<S, Tile<wx, rx>, PML>), dim3(blocks), dim3(TY, wx), 0, st, A, work, s); \
-    return;                                                                                                  \
+#define QP3D_L(wx, rx)                                                                                             \
+  if (tile == i++) {                                                                                               \
+    hipLaunchKernelGGL((k_update_batch<S, Tile<wx, rx>, PML>), dim3(blocks), dim3(TY, wx), 0, st, A, work, s, maps); \
+    return;                                                                                                        \
   }
   QP3D_TILES(QP3D_L)
 #undef QP3D_L
@@ -79,7 +80,7 @@ int occupancy_update_batch(int tile)
 template void launch_update<QP3D_STORE, QP3D_PML>(int, int, hipStream_t, const UpdArgs<QP3D_STORE> &, bool);
 template int occupancy_update<QP3D_STORE, QP3D_PML>(int, bool);
 template void launch_update_batch<QP3D_STORE, QP3D_PML>(int, int, const int4 *, hipStream_t,
-                                                        const UpdArgs<QP3D_STORE> *, Step);
+                                                        const UpdArgs<QP3D_STORE> *, Step, int);
 template int occupancy_update_batch<QP3D_STORE, QP3D_PML>(int);
 
 }  // namespace qp3d

@@ -864,9 +864,11 @@ struct SrcBox {
   int on, z0, z1, x0, x1, y0, y1;  // inclusive bounds
 };
 
-static __global__ void k_quiet(AmpGrid ag, Geo g, Coef c, const unsigned *gmax, Step st, const unsigned *hist, SrcBox sb)
+// The quiet and rest maps of one grid at step kstep, blocks b = first, first + stride, ...
+__device__ __forceinline__ void quiet_blocks(const AmpGrid &ag, const Geo &g, const Coef &c, const unsigned *gmax,
+                                             int kstep, const unsigned *hist, const SrcBox &sb, unsigned first,
+                                             unsigned stride)
 {
-  const int kstep = st.get();
   const unsigned *bb = ag.buf(kstep);
   const float thr = s_threshold(__uint_as_float(gmax[kstep]));
   const bool whole = *hist != 0u;
@@ -875,7 +877,7 @@ static __global__ void k_quiet(AmpGrid ag, Geo g, Coef c, const unsigned *gmax, 
   for (int k = 0; k < H; k++) s1 += fabsf(c.c1[k]);
   const float fac = 2.f * s1 * fmaxf(c.ix, fmaxf(c.iy, c.iz));
   const unsigned *db = ag.dbuf(kstep), *cb = ag.cbuf(kstep);
-  for (unsigned b = blockIdx.x * blockDim.x + threadIdx.x; b < ag.nblk; b += gridDim.x * blockDim.x) {
+  for (unsigned b = first; b < ag.nblk; b += stride) {
     const int by = (int)(b % ag.nby), t = (int)(b / ag.nby), bx = t % ag.nbx, bz = t / ag.nbx;
     unsigned m = 0u, d = 0u;
     for (int z = max(0, bz - 1); z <= min(ag.nbz - 1, bz + 1); z++)
@@ -895,6 +897,29 @@ static __global__ void k_quiet(AmpGrid ag, Geo g, Coef c, const unsigned *gmax, 
     const bool quiet = gb < thr;
     ag.q[qb] = (unsigned char)((quiet ? 1 : 0) | (touched ? 2 : 0));
   }
+}
+
+static __global__ void k_quiet(AmpGrid ag, Geo g, Coef c, const unsigned *gmax, Step st, const unsigned *hist, SrcBox sb)
+{
+  quiet_blocks(ag, g, c, gmax, st.get(), hist, sb, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+// The maps of every shot of a batch (blockIdx.y: shot; shots without them leave at once).
+struct QuietArgs {
+  AmpGrid ag;  // ag.q == nullptr: no maps for this shot
+  Geo g;
+  Coef c;
+  const unsigned *gmax;
+  const unsigned *hist;
+  SrcBox sb;
+};
+
+static __global__ void k_quiet_batch(const QuietArgs *A, Step st)
+{
+  const QuietArgs &a = A[blockIdx.y];
+  if (!a.ag.q || !a.ag.b) return;
+  quiet_blocks(a.ag, a.g, a.c, a.gmax, st.get(), a.hist, a.sb, blockIdx.x * blockDim.x + threadIdx.x,
+               gridDim.x * blockDim.x);
 }
 
 // Batched launches (several grids in one launch): item .z packs z0 | z1 << 16 and .w is the
@@ -1410,12 +1435,14 @@ __global__ __launch_bounds__(T::NT) void k_update(UpdArgs<S> a)
   update_tile<S, T, PML, SPEC>(a, a.work[blockIdx.x], a.step.get(), sm);
 }
 
+// maps 0: the shots' quiet and rest maps are not valid for this step (not read)
 template <class S, class T, int PML>
-__global__ __launch_bounds__(T::NT) void k_update_batch(const UpdArgs<S> *A, const int4 *work, Step st)
+__global__ __launch_bounds__(T::NT) void k_update_batch(const UpdArgs<S> *A, const int4 *work, Step st, int maps)
 {
   int shot;
   const int4 w = batch_item(work, shot);
-  const UpdArgs<S> a = A[shot];  // one copy: stores to the fields must not force reloads
+  UpdArgs<S> a = A[shot];  // one copy: stores to the fields must not force reloads
+  if (!maps) a.amp.q = nullptr;
   __shared__ float sm[2][T::LX + 1][T::LY];
   update_tile<S, T, PML>(a, w, st.get(), sm);
 }

@@ -37,7 +37,8 @@ template <class S>
 void launch_gbound_batch(int tile, int blocks, int nshots, const int4 *work, int nwork, const int *first,
                          hipStream_t st, const GradArgs<S> *A, Step s);
 template <class S, int PML>
-void launch_update_batch(int tile, int blocks, const int4 *work, hipStream_t st, const UpdArgs<S> *A, Step s);
+void launch_update_batch(int tile, int blocks, const int4 *work, hipStream_t st, const UpdArgs<S> *A, Step s,
+                         int maps);
 template <class S>
 int occupancy_gradmax_batch(int tile);
 template <class S, int PML>
