@@ -47,6 +47,7 @@ struct qp3d_batch {
   // autotuner: every stage does its full work (no tile skipped, no map), so that the timings
   // compare the shapes and modes at a fully active wavefield, not at rest
   bool tuning = false;
+  bool hist_ended = false;  // the shots' rest histories were ended since the last reset
   int *first = nullptr;  // [nshots] an item of each shot in the gradient list (the bound pass's fallback)
   // quiet and rest maps of every shot (k_quiet_batch after the gradient pass; the updates read
   // them on steps whose maxima cover u)
@@ -305,9 +306,11 @@ static int batch_step_s(qp3d_batch *b, int k, Step st)
   // a step without them ends the shots' rest histories (never inside a capture, which starts
   // with the maxima covering u)
   const int maps = b->amp_any && b->amp_ok && !b->tuning ? 1 : 0;
-  if (b->amp_any && !maps)
+  if (b->amp_any && !maps && !b->hist_ended) {
     for (qp3d_plan *p : b->plans)
       if (p->rflag) HIP_TRY(hipMemsetAsync(p->rflag, 0, sizeof(unsigned), b->stream));
+    b->hist_ended = true;
+  }
   for (int s = 0; s < 4; s++) {
     const bool tm = b->timing && s == b->timed_stage;
     if (tm) HIP_TRY(hipEventRecord(b->ev[0], b->stream));
@@ -493,6 +496,7 @@ extern "C" int qp3d_batch_reset(qp3d_batch *b)
   }
   b->steps_done = 0;
   b->amp_ok = true;  // u = 0 everywhere, and so are the block maxima
+  b->hist_ended = false;
   return QP3D_OK;
 }
 
@@ -545,8 +549,11 @@ extern "C" int qp3d_batch_run(qp3d_batch *b, int k0, int k1)
       // the persistent launch records no block maxima and keeps no map: the shots' rest
       // histories end here
       b->amp_ok = false;
-      for (qp3d_plan *p : b->plans)
-        if (p->rflag) HIP_TRY(hipMemsetAsync(p->rflag, 0, sizeof(unsigned), b->stream));
+      if (!b->hist_ended) {
+        for (qp3d_plan *p : b->plans)
+          if (p->rflag) HIP_TRY(hipMemsetAsync(p->rflag, 0, sizeof(unsigned), b->stream));
+        b->hist_ended = true;
+      }
       int rc = QP3D_OK;
       with_storage(b->storage, [&](auto sp) { rc = persist_run_s<decltype(sp)>(b, k0, k1); });
       if (rc) return rc;
