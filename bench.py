@@ -536,6 +536,11 @@ def main():
                    "update_cells_quiet_frac": round(quiet_frac, 4) if counted else None,
                    "update_cells_at_rest_frac_interior": round(int_c[1] / max(1, int_c[0]), 4) if counted else None,
                    "ms_update_interior_by_twelfths": int_bins,
+                   # the last twelfth of the shot: every interior cell active (the wave fills the
+                   # grid long before), so each launch moves its full 24 B per cell
+                   "roofline_interior_last_twelfth": ({"achieved": round(kb["update_interior"] / (int_bins[-1] * 1e-3) / 1e9, 1),
+                                                       "frac": round(kb["update_interior"] / (int_bins[-1] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                                                      if int_bins and int_bins[-1] > 0 else None),
                    "roofline_kernel": f"k_update<interior> ({b_int:.0f} B per interior cell updated, "
                                       f"{b_int - 4:.0f} B without eps - del, 0 B at rest; "
                                       f"{moved_int:.0f} B per launch over the shot, "
