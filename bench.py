@@ -483,16 +483,23 @@ def main():
     upd_total = int(int_c[0] + pml_c[0])
     rest_frac = float(int_c[1] + pml_c[1]) / max(1, upd_total)
     quiet_frac = float(int_c[2] + pml_c[2]) / max(1, upd_total)
+    # The counter passes (FETCH_SIZE, WRITE_SIZE) measured a launch that processes every cell,
+    # on an earlier build: they give the kernels' re-read ratio at full load, not the traffic of
+    # this build's launches (which leave out cells at rest), so they go into config, and
+    # roofline.traffic stays null.
     traffic = step_traffic = None
+    traffic_full = None
     if os.path.exists(TRAFFIC_FILE):
         try:
             with open(TRAFFIC_FILE) as f:
                 tr = json.load(f)
             if tr.get("nx") == s.nx and tr.get("storage", "fp32") == args.storage:
-                traffic = tr.get("bytes_per_launch")
-                step_traffic = tr.get("bytes_per_step") or sum(k["bytes"] for k in tr.get("per_kernel", {}).values())
+                traffic_full = {"interior_bytes_per_launch": tr.get("bytes_per_launch"),
+                                "ratio_to_algorithmic": {k: v.get("ratio") for k, v in tr.get("per_kernel", {}).items()},
+                                "step_bytes": tr.get("bytes_per_step"),
+                                "source": "profiles/k_update_576cube_traffic.md (every cell processed; earlier build)"}
         except (OSError, ValueError):
-            traffic = step_traffic = None
+            traffic_full = None
     cpu = cpu1 = None
     if not args.no_cpu and args.cpu_steps > 0:
         runs = [(0, args.cpu_steps)] + ([(1, args.cpu_steps_1)] if args.cpu_steps_1 > 0 else [])
@@ -545,6 +552,7 @@ def main():
                                       f"{b_int - 4:.0f} B without eps - del, 0 B at rest; "
                                       f"{moved_int:.0f} B per launch over the shot, "
                                       f"{kb['cells_interior']:.0f} cells per launch)",
+                   "traffic_full_load": traffic_full,
                    "roofline_all_cells": {"achieved": round(achieved_all, 1), "frac": round(achieved_all / HBM_PEAK_GBS, 4),
                                           "step_achieved": round(step_gbs_all, 1),
                                           "step_frac": round(step_gbs_all / HBM_PEAK_GBS, 4), "step_bytes": step_bytes},
