@@ -32,6 +32,11 @@ struct qp3d_fwi {
   int interval = 0, nseg = 0;
   int ckpt_storage = QP3D_STORAGE_FP32;
   std::vector<void *> ckpt;      // state block at the start of each segment
+  // FP32 checkpoints: the plan's block maxima of the segment's first u (one slot: max |u|, the
+  // increment, |grad u|), its map and rest-history word, so that the recomputation runs with the
+  // same maps as the forward run (none: recomputation without them)
+  std::vector<void *> camp;
+  std::vector<char> amp_saved;  // [nseg] the record of segment s holds the forward run's
   float *ubuf = nullptr;         // [interval][N] u_k of the segment being back-propagated
   float *adj = nullptr;          // adjoint state: u[2], w1[2], p[2], w3[2], w21 (9 N)
   float *prod = nullptr;         // Pxx Pyy Pzz Qx Qy Qz (6 N)
@@ -46,10 +51,17 @@ struct qp3d_fwi {
   double ms_forward = 0, ms_recompute = 0, ms_adjoint = 0;
 };
 
+// Bytes of one checkpoint's block-maximum record: three arrays of one slot, the map, the word.
+static size_t amp_bytes(const qp3d_plan *p)
+{
+  return sizeof(unsigned) * (3 * (size_t)p->ampg.nblk + 1) + (size_t)p->ampg.nblk;
+}
+
 static void fwi_free(qp3d_fwi *f)
 {
   if (!f) return;
   for (void *c : f->ckpt) (void)hipFree(c);
+  for (void *c : f->camp) (void)hipFree(c);
   (void)hipFree(f->ubuf);
   (void)hipFree(f->adj);
   (void)hipFree(f->prod);
@@ -107,6 +119,11 @@ extern "C" int qp3d_fwi_create(qp3d_plan *p, int interval, int ckpt_storage, qp3
     void *d = nullptr;
     if ((rc = dmalloc(&d, cb, "checkpoint"))) return fail(rc);
     f->ckpt.push_back(d);
+    if (ckpt_storage == QP3D_STORAGE_FP32 && p->ampg.b && p->rflag) {
+      void *m = nullptr;
+      if ((rc = dmalloc(&m, amp_bytes(p), "checkpoint block maxima"))) return fail(rc);
+      f->camp.push_back(m);
+    }
   }
   if ((rc = dmalloc((void **)&f->ubuf, sizeof(float) * N * f->interval, "segment wavefields"))) return fail(rc);
   if ((rc = dmalloc((void **)&f->adj, sizeof(float) * 9 * N, "adjoint state"))) return fail(rc);
@@ -146,12 +163,65 @@ extern "C" int qp3d_fwi_destroy(qp3d_fwi *f)
 
 static int grid_of(long n) { return (int)std::min<long>((n + 255) / 256, 8192); }
 
+// The block-maximum record at the start of segment s (step k0 = s * interval): slot k0 % 3 of
+// the three arrays (the maxima of u_k0 that step k0's map reads), the map, the history word.
+static int save_amp(qp3d_fwi *f, int s)
+{
+  qp3d_plan *p = f->p;
+  if (f->camp.empty() || !p->amp_ok) return QP3D_OK;
+  const size_t nb = p->ampg.nblk, sl = (size_t)((s * f->interval) % 3);
+  unsigned *d = (unsigned *)f->camp[s - 1];
+  for (int a = 0; a < 3; a++)
+    HIP_TRY(hipMemcpyAsync(d + a * nb, p->ampg.b + (3 * a + sl) * nb, sizeof(unsigned) * nb, hipMemcpyDeviceToDevice,
+                           p->stream));
+  HIP_TRY(hipMemcpyAsync(d + 3 * nb, p->rflag, sizeof(unsigned), hipMemcpyDeviceToDevice, p->stream));
+  HIP_TRY(hipMemcpyAsync(d + 3 * nb + 1, p->ampg.q, nb, hipMemcpyDeviceToDevice, p->stream));
+  return QP3D_OK;
+}
+
+// Restores the record of segment s (s = 0: the reset's), with the other two slots cleared;
+// false when the segment has none (the recomputation then runs without the maps).
+static bool load_amp(qp3d_fwi *f, int s, int &rc)
+{
+  qp3d_plan *p = f->p;
+  rc = QP3D_OK;
+  if (f->camp.empty() || !f->amp_saved[s]) return false;
+  const size_t nb = p->ampg.nblk, sl = (size_t)((s * f->interval) % 3);
+  auto fail = [&](hipError_t e) {
+    qp3d_set_error("fwi_gradient: restoring the block maxima failed: %s", hipGetErrorString(e));
+    rc = QP3D_ERR_HIP;
+    return false;
+  };
+  hipError_t e = hipMemsetAsync(p->ampg.b, 0, sizeof(unsigned) * 9 * nb, p->stream);
+  if (e != hipSuccess) return fail(e);
+  if (s == 0) {
+    if ((e = hipMemsetAsync(p->ampg.q, 0, nb, p->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(p->rflag, 1, sizeof(unsigned), p->stream)) != hipSuccess)
+      return fail(e);
+  } else {
+    const unsigned *d = (const unsigned *)f->camp[s - 1];
+    for (int a = 0; a < 3; a++)
+      if ((e = hipMemcpyAsync(p->ampg.b + (3 * a + sl) * nb, d + a * nb, sizeof(unsigned) * nb,
+                              hipMemcpyDeviceToDevice, p->stream)) != hipSuccess)
+        return fail(e);
+    if ((e = hipMemcpyAsync(p->rflag, d + 3 * nb, sizeof(unsigned), hipMemcpyDeviceToDevice, p->stream)) !=
+            hipSuccess ||
+        (e = hipMemcpyAsync(p->ampg.q, d + 3 * nb + 1, nb, hipMemcpyDeviceToDevice, p->stream)) != hipSuccess)
+      return fail(e);
+  }
+  p->amp_ok = true;
+  return true;
+}
+
 static int save_ckpt(qp3d_fwi *f, int s)
 {
   qp3d_plan *p = f->p;
   const long n = (long)(p->state_bytes / sizeof(float));  // the whole state block, pads included
   if (f->ckpt_storage == QP3D_STORAGE_FP32) {
     HIP_TRY(hipMemcpyAsync(f->ckpt[s - 1], p->state, p->state_bytes, hipMemcpyDeviceToDevice, p->stream));
+    f->amp_saved[s] = !f->camp.empty() && p->amp_ok;
+    int rc = save_amp(f, s);
+    if (rc) return rc;
   } else {
     hipLaunchKernelGGL(k_pack_bf16, dim3(grid_of(n)), dim3(256), 0, p->stream, (unsigned short *)f->ckpt[s - 1],
                        (const float *)p->state, n);
@@ -190,6 +260,9 @@ extern "C" int qp3d_fwi_forward(qp3d_fwi *f)
   int rc = qp3d_plan_reset(p);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(f->ev[0], p->stream));
+  // segment 0 starts from the reset (its record: zeros, an empty map, a whole history)
+  f->amp_saved.assign((size_t)f->nseg, 0);
+  f->amp_saved[0] = !f->camp.empty() && p->amp_ok;
   const int klast = (f->nseg - 1) * f->interval;  // steps of the last segment keep u_k
   const size_t N = (size_t)p->N;
   for (int k = 0; k < p->cfg.nt; k++) {
@@ -285,11 +358,14 @@ extern "C" int qp3d_fwi_gradient(qp3d_fwi *f, const float *resid, double *g_vp, 
     HIP_TRY(hipEventRecord(f->ev[0], p->stream));
     if (s != f->ubuf_seg) {
       if ((rc = load_ckpt(f, s))) return rc;
+      const bool maps = load_amp(f, s, rc);
+      if (rc) return rc;
       for (int k = k0; k < k1; k++) {
         const int cur = (k & 1) ^ 1;  // slot of u_k in the forward state
         HIP_TRY(hipMemcpyAsync(f->ubuf + (size_t)(k - k0) * N, p->u[cur], sizeof(float) * N,
                                hipMemcpyDeviceToDevice, p->stream));
-        if ((rc = qp3d_forward_step_fwi(p, k, true))) return rc;  // G_k from the forward run
+        // G_k from the forward run; the forward run's maps when its record was restored
+        if ((rc = qp3d_forward_step_fwi(p, k, true, maps))) return rc;
       }
       f->ubuf_seg = s;
     }

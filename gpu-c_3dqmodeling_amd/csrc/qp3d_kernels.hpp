@@ -903,6 +903,19 @@ __device__ __forceinline__ void quiet_blocks(const AmpGrid &ag, const Geo &g, co
   }
 }
 
+// Clears the slot of block maxima that the updates of step k+1 fill (for steps without a bound
+// pass, whose k_gbound does it otherwise).
+static __global__ void k_amp_clear(AmpGrid ag, Step st)
+{
+  const int kstep = st.get();
+  unsigned *clr = ag.buf(kstep + 2), *clrd = ag.dbuf(kstep + 2), *clrc = ag.cbuf(kstep + 2);
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < ag.nblk; i += gridDim.x * blockDim.x) {
+    clr[i] = 0u;
+    clrd[i] = 0u;
+    clrc[i] = 0u;
+  }
+}
+
 static __global__ void k_quiet(AmpGrid ag, Geo g, Coef c, const unsigned *gmax, Step st, const unsigned *hist, SrcBox sb)
 {
   quiet_blocks(ag, g, c, gmax, st.get(), hist, sb, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);

@@ -1003,6 +1003,17 @@ static int launch_step_s(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on
       HIP_TRY(hipGetLastError());
       ua.amp.q = p->ampg.q;
     }
+  } else if (gknown && use_amp && p->amp_ok) {
+    // G_k known from an earlier run of the same step (FWI recomputation from a checkpoint whose
+    // block maxima and maps were restored): no gradient pass, the same maps
+    const SrcBox sb{p->src_on ? 1 : 0, p->sb[0], p->sb[1], p->sb[2], p->sb[3], p->sb[4], p->sb[5]};
+    const dim3 grid((p->ampg.nblk + 255) / 256);
+    hipLaunchKernelGGL(k_amp_clear, grid, dim3(256), 0, s0, p->ampg, st);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_quiet, grid, dim3(256), 0, s0, p->ampg, p->geo, p->coef, (const unsigned *)p->gmax, st,
+                       (const unsigned *)p->rflag, sb);
+    HIP_TRY(hipGetLastError());
+    ua.amp.q = p->ampg.q;
   }
   // a step without the map breaks the rest history until the next reset
   if (!ua.amp.q) ua.rest_off = p->rflag;
@@ -1087,11 +1098,13 @@ static int launch_step(qp3d_plan *p, int k, Step st, bool bare, hipStream_t on, 
 
 // FWI: the first forward run with the amplitude bound, or a recomputed step whose G_k the first
 // run already left in gmax[k] (no gradient pass; the update reads it)
-// (on the plan's own stream only: the checkpoints and the adjoint follow on it)
-int qp3d_forward_step_fwi(qp3d_plan *p, int k, bool recompute)
+// (on the plan's own stream only: the checkpoints and the adjoint follow on it). maps: a
+// recomputed step whose checkpoint restored the block maxima, maps and rest history of the
+// first run uses them as that run did.
+int qp3d_forward_step_fwi(qp3d_plan *p, int k, bool recompute, bool maps)
 {
   if (recompute && p->spec) return launch_step(p, k, Step{nullptr, k}, true, p->stream, false);
-  return recompute ? launch_step(p, k, Step{nullptr, k}, true, p->stream, false, true)
+  return recompute ? launch_step(p, k, Step{nullptr, k}, true, p->stream, maps, true)
                    : launch_step(p, k, Step{nullptr, k}, true, p->stream, true);
 }
 

@@ -254,7 +254,7 @@ int qp3d_cu_count(const qp3d_plan *p);
 int qp3d_forward_step(qp3d_plan *p, int k);
 // FWI steps: recompute = false, the forward run with the amplitude bound; true, a recomputed
 // step that takes G_k from gmax[k] (left there by the forward run) instead of a gradient pass.
-int qp3d_forward_step_fwi(qp3d_plan *p, int k, bool recompute);
+int qp3d_forward_step_fwi(qp3d_plan *p, int k, bool recompute, bool maps = false);
 // The same on stream `on` (one stream), with the step index the kernels read given by `st`.
 int qp3d_forward_step_on(qp3d_plan *p, int k, Step st, hipStream_t on);
 // Validates a configuration (no device access); `who` prefixes the error message.
