@@ -1013,7 +1013,10 @@ __global__ __launch_bounds__(T::NT) void k_gbound_batch(const GradArgs<S> *A, co
   const unsigned long long key = a.argkey[(kstep + 1) & 1];
   int item = (int)(key & 0xffffffffull);
   if (item < 0 || item >= nwork || work[item].w != shot) item = first[shot];
-  const int4 v = work[item];
+  // a shot without an item of its own gets no seed (nothing skipped), but its slot and next
+  // seed are cleared all the same (an empty z range)
+  const bool own = item >= 0 && item < nwork && work[item].w == shot;
+  const int4 v = own ? work[item] : make_int4(0, 0, 0, 0);
   gbound_tile<S, T>(a, make_int4(v.x, v.y, v.z & 0xffff, (int)((unsigned)v.z >> 16)), kstep, sm);
 }
 
