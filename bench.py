@@ -117,12 +117,16 @@ def bench_fwi(args):
     log(f"plain forward shot {t_fwd:.3f} s")
     runs = {}
     ref = None
-    cases = [("ckpt50_fp32", 50, "fp32"), ("ckpt50_bf16", 50, "bf16")]
+    # the adjoint sweep leaves out the planes its touched map shows at rest, except in the
+    # rest_map_off run (same gradient, every plane swept)
+    cases = [("ckpt50_fp32", 50, "fp32", True), ("ckpt50_fp32_rest_map_off", 50, "fp32", False),
+             ("ckpt50_bf16", 50, "bf16", True)]
     if args.fwi_full:
-        cases.append(("full_storage", nt, "fp32"))
-    for key, interval, ck in cases:
+        cases.append(("full_storage", nt, "fp32", True))
+    for key, interval, ck, rest in cases:
         try:
             with qp3d.Plan(s0, gmax_mode=args.gmax_mode) as p, qp3d.Fwi(p, interval=interval, ckpt_storage=ck) as f:
+                f.set_rest_map(rest)
                 f.forward()
                 p.sync()
                 t = time.perf_counter()

@@ -28,9 +28,70 @@
 
 namespace qp3d {
 
+// Touched map of an adjoint sweep, one byte per block of 8 planes x 8 rows x 32 cells (z, x, y),
+// z fastest: t[(bx * nby + by) * nbz + bz] != 0 when the adjoint state of the block (the adjoint
+// of u, the PML adjoint states, the product fields) may have been nonzero at some step of the
+// sweep. Before the sweep every adjoint array is zero and the receiver plane's blocks are
+// marked; k_adj_apply marks the blocks where the completed adjoint of u it writes is nonzero;
+// nothing clears a mark during the sweep. The outputs of k_adj_prod at a cell vanish with the
+// cell's adjoint of u (and residual) and PML adjoint states, which are zero outside the marked
+// blocks, so it leaves those planes out (writing zeros over zeros); k_adj_apply adds zero to the
+// cells more than 5 cells from every marked block (the product fields are zero outside them) and
+// leaves those planes out too. Both narrow an item's z range only: the adjoint wave enters
+// through the whole receiver plane and travels in depth. t == nullptr: off.
+struct AdjMap {
+  unsigned char *t;
+  int nbz, nbx, nby;
+};
+
+// Narrows [z0, z1) of a work item to the planes within HALO cells (along z) of a marked block
+// among the item's x and y blocks widened by HALO cells (a folded tile: both of its y strips);
+// z1 = z0 when there is none. Wave 0 reads the map and hands the range to the other wavefronts
+// through LDS: k_adj_apply marks blocks while other workgroups read the map, and every
+// wavefront of a workgroup must run the same planes (the march has a barrier per plane). A mark
+// read late only widens the range; the marks an item needs were set by earlier launches.
+template <int HALO>
+__device__ __forceinline__ void adj_trim(const AdjMap &m, const Geo &g, int x0, int x1, int y0, int y1, bool fold,
+                                         int &z0, int &z1, int *zr)
+{
+  if (threadIdx.y == 0) {
+    const int xb0 = max(0, x0 - HALO) >> AB_X, xb1 = min(g.nx - 1, x1 - 1 + HALO) >> AB_X;
+    const int yh = g.ny - g.pml;  // a folded tile's second strip starts here
+    const int ya0 = max(0, y0 - HALO) >> AB_Y, ya1 = min(g.ny - 1, y1 - 1 + HALO) >> AB_Y;
+    const int yc0 = max(0, yh - HALO) >> AB_Y, yc1 = min(g.ny - 1, yh + (y1 - y0) - 1 + HALO) >> AB_Y;
+    const int zb0 = max(0, z0 - HALO) >> AB_Z, zb1 = min(g.nz - 1, z1 - 1 + HALO) >> AB_Z;
+    int first = 0x7fffffff, last = -1;
+    for (int zb = zb0; zb <= zb1; zb += TY) {  // uniform
+      const int b = zb + (int)__lane_id();
+      bool hit = false;
+      if (b <= zb1) {
+        for (int st = 0; st < (fold ? 2 : 1); st++) {
+          const int yb0 = st ? yc0 : ya0, yb1 = st ? yc1 : ya1;
+          for (int xb = xb0; xb <= xb1; xb++)
+            for (int yb = yb0; yb <= yb1; yb++)
+              hit |= m.t[((unsigned)xb * (unsigned)m.nby + (unsigned)yb) * (unsigned)m.nbz + (unsigned)b] != 0;
+        }
+      }
+      const unsigned long long v = __ballot(hit);
+      if (v) {
+        if (first == 0x7fffffff) first = zb + __builtin_ctzll(v);
+        last = zb + 63 - __builtin_clzll(v);
+      }
+    }
+    if (__lane_id() == 0) {
+      zr[0] = last < 0 ? z0 : max(z0, (first << AB_Z) - HALO);
+      zr[1] = last < 0 ? z0 : min(z1, ((last + 1) << AB_Z) + HALO);
+    }
+  }
+  __syncthreads();
+  z0 = zr[0];
+  z1 = zr[1];
+}
+
 struct AdjArgs {
   Geo g;
   Coef c;
+  AdjMap m;
   const float *u;                         // forward u_k
   const float *vdt2, *eps, *epsd;         // model
   const float4 *pmltab;
@@ -137,6 +198,11 @@ __global__ __launch_bounds__(T::NT) void k_adj_prod(AdjArgs a)
   bool fold;
   unpack(a.work[blockIdx.x], x0, x1, y0, y1, z0, z1, fold);
   if (z0 >= z1) return;
+  __shared__ int zr[2];
+  if (a.m.t) {
+    adj_trim<0>(a.m, g, x0, x1, y0, y1, fold, z0, z1, zr);
+    if (z0 >= z1) return;
+  }
   Lane<T> L(g, x0, x1, y0, y1, fold);
   const int y = L.y;
   const int nx = g.nx, ny = g.ny, nz = g.nz, pml = g.pml;
@@ -339,6 +405,7 @@ struct ApplyArgs {
   Coef c;
   const float *pxx, *pyy, *pzz, *qx, *qy, *qz;
   float *ac;
+  AdjMap m;  // marked where the completed adjoint of u is nonzero
   const int4 *work;
 };
 
@@ -354,10 +421,28 @@ __global__ __launch_bounds__(T::NT) void k_adj_apply(ApplyArgs a)
   bool fold;
   unpack(a.work[blockIdx.x], x0, x1, y0, y1, z0, z1, fold);
   if (z0 >= z1) return;
+  __shared__ int zr[2];
+  const bool map_on = a.m.t != nullptr;
+  if (map_on) {
+    adj_trim<H>(a.m, g, x0, x1, y0, y1, fold, z0, z1, zr);
+    if (z0 >= z1) return;
+  }
   Lane<T> L(g, x0, x1, y0, y1, fold);
   bool rows_on[T::RX];
 #pragma unroll
   for (int r = 0; r < T::RX; r++) rows_on[r] = L.yon && x0 + L.row(r) < x1;
+  bool nzl = false;  // a nonzero adjoint of u written in the current z block (map on)
+  // marks z block zb for the tile's x and y blocks (both strips of a folded tile)
+  auto mark = [&](int zb) {
+    const int xb0 = x0 >> AB_X, xb1 = (x1 - 1) >> AB_X;
+    const int yh = g.ny - g.pml;
+    for (int st = 0; st < (fold ? 2 : 1); st++) {
+      const int yb0 = (st ? yh : y0) >> AB_Y, yb1 = ((st ? yh : y0) + (y1 - y0) - 1) >> AB_Y;
+      for (int xb = xb0; xb <= xb1; xb++)
+        for (int yb = yb0; yb <= yb1; yb++)
+          a.m.t[((unsigned)xb * (unsigned)a.m.nby + (unsigned)yb) * (unsigned)a.m.nbz + (unsigned)zb] = 1;
+    }
+  };
   float qzz[T::RX][QN], qqz[T::RX][QN];
   prime_queue<StoreF32, T>(qzz, a.pzz, g, L, z0);
   prime_queue<StoreF32, T>(qqz, a.qz, g, L, z0);
@@ -428,7 +513,13 @@ __global__ __launch_bounds__(T::NT) void k_adj_apply(ApplyArgs a)
       acc = fmaf(dx, a.c.ix, acc);
       acc = fmaf(dy, a.c.iy, acc);
       acc = fmaf(dz, a.c.iz, acc);
-      at(a.ac, zo, L.off[r]) = acv[r] + acc;
+      const float lam = acv[r] + acc;
+      at(a.ac, zo, L.off[r]) = lam;
+      nzl |= lam != 0.f;
+    }
+    if (map_on && ((z & ((1 << AB_Z) - 1)) == (1 << AB_Z) - 1 || !more)) {  // uniform
+      if (__ballot(nzl) && __lane_id() == 0) mark(z >> AB_Z);
+      nzl = false;
     }
     if (more) {
 #pragma unroll

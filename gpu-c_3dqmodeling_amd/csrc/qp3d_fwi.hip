@@ -44,6 +44,12 @@ struct qp3d_fwi {
   double *gsum = nullptr;        // FP64 sums (3 N)
   float *resid = nullptr;        // [nt][hx][hy]
   WorkList wl_i, wl_p, wl_py, wl_a;  // interior, PML x and z faces, PML y faces, all of [5,n-5)^3
+  // touched map of the adjoint sweep (AdjMap) and its state before the sweep (the receiver
+  // plane's blocks marked); rest_map: the adjoint launches leave out the planes it shows at rest
+  unsigned char *tmap = nullptr;
+  std::vector<unsigned char> tmap0;
+  int nbz = 0, nbx = 0, nby = 0;
+  bool rest_map = true;
   bool forward_done = false;
   int ubuf_seg = -1;             // segment whose u_k ubuf holds (the forward run keeps the last)
   unsigned gen = 0;              // the plan's upload count at the forward run
@@ -68,6 +74,7 @@ static void fwi_free(qp3d_fwi *f)
   (void)hipFree(f->gacc);
   (void)hipFree(f->gsum);
   (void)hipFree(f->resid);
+  (void)hipFree(f->tmap);
   (void)hipFree(f->wl_i.d);
   (void)hipFree(f->wl_p.d);
   (void)hipFree(f->wl_py.d);
@@ -133,6 +140,15 @@ extern "C" int qp3d_fwi_create(qp3d_plan *p, int interval, int ckpt_storage, qp3
   if ((rc = dmalloc((void **)&f->resid, sizeof(float) * (size_t)c.nt * p->hx * p->hy, "residuals"))) return fail(rc);
   // product fields stay zero outside [5,n-5)^3, where k_adj_prod never writes
   HIP_TRY(hipMemset(f->prod, 0, sizeof(float) * 6 * N));
+  f->nbz = (c.nz + (1 << AB_Z) - 1) >> AB_Z;
+  f->nbx = (c.nx + (1 << AB_X) - 1) >> AB_X;
+  f->nby = (c.ny + (1 << AB_Y) - 1) >> AB_Y;
+  const size_t nblk = (size_t)f->nbz * f->nbx * f->nby;
+  if ((rc = dmalloc((void **)&f->tmap, nblk, "adjoint touched map"))) return fail(rc);
+  f->tmap0.assign(nblk, 0);
+  for (int xb = c.pml >> AB_X; xb <= (c.nx - c.pml - 1) >> AB_X; xb++)
+    for (int yb = c.pml >> AB_Y; yb <= (c.ny - c.pml - 1) >> AB_Y; yb++)
+      f->tmap0[((size_t)xb * f->nby + yb) * f->nbz + (c.sz >> AB_Z)] = 1;
   const int oi = occupancy(reinterpret_cast<const void *>(&k_adj_prod<TP, false>), TP::NT);
   const int op = occupancy(reinterpret_cast<const void *>(&k_adj_prod<TP, true>), TP::NT);
   const int oa = occupancy(reinterpret_cast<const void *>(&k_adj_apply<TA>), TA::NT);
@@ -294,6 +310,7 @@ static int adjoint_step(qp3d_fwi *f, int k, const float *uk)
   AdjArgs a;
   a.g = p->geo;
   a.c = p->coef;
+  a.m = AdjMap{f->rest_map ? f->tmap : nullptr, f->nbz, f->nbx, f->nby};
   a.u = uk;
   a.vdt2 = p->vdt2; a.eps = p->eps; a.epsd = p->epsd;
   a.pmltab = p->pmltab;
@@ -324,6 +341,7 @@ static int adjoint_step(qp3d_fwi *f, int k, const float *uk)
   b.c = p->coef;
   b.pxx = a.pxx; b.pyy = a.pyy; b.pzz = a.pzz; b.qx = a.qx; b.qy = a.qy; b.qz = a.qz;
   b.ac = a.ac;
+  b.m = a.m;
   b.work = f->wl_a.d;
   hipLaunchKernelGGL((k_adj_apply<TA>), dim3(f->wl_a.n), dim3(TY, TA::WX), 0, p->stream, b);
   HIP_TRY(hipGetLastError());
@@ -351,6 +369,12 @@ extern "C" int qp3d_fwi_gradient(qp3d_fwi *f, const float *resid, double *g_vp, 
                          p->stream));
   HIP_TRY(hipMemsetAsync(f->adj, 0, sizeof(float) * 9 * N, p->stream));
   HIP_TRY(hipMemsetAsync(f->gsum, 0, sizeof(double) * 3 * N, p->stream));
+  if (f->rest_map) {
+    // the map's invariant: every nonzero adjoint value lies in a marked block (the product
+    // fields hold the last sweep's values until cleared)
+    HIP_TRY(hipMemsetAsync(f->prod, 0, sizeof(float) * 6 * N, p->stream));
+    HIP_TRY(hipMemcpyAsync(f->tmap, f->tmap0.data(), f->tmap0.size(), hipMemcpyHostToDevice, p->stream));
+  }
   double ms_re = 0, ms_adj = 0;
   int rc;
   for (int s = f->nseg - 1; s >= 0; s--) {
@@ -396,6 +420,15 @@ extern "C" int qp3d_fwi_gradient(qp3d_fwi *f, const float *resid, double *g_vp, 
   f->ms_adjoint = ms_adj;
   // the plan's state holds the end of the last segment recomputed (or of the forward run)
   if (f->nseg > 1) p->steps_done = std::min(f->interval, c.nt);
+  return QP3D_OK;
+}
+
+// Whether the adjoint sweep leaves out the planes that its touched map shows at rest (on by
+// default; the gradient is the same either way, up to the sign of zeros).
+extern "C" int qp3d_fwi_set_rest_map(qp3d_fwi *f, int on)
+{
+  if (!f) return QP3D_ERR_ARG;
+  f->rest_map = on != 0;
   return QP3D_OK;
 }
 

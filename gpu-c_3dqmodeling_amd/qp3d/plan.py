@@ -212,6 +212,10 @@ class Fwi:
         check(self.lib.qp3d_fwi_gradient(self.h, fptr(r), *[_dptr(o) for o in out]))
         return tuple(out)
 
+    def set_rest_map(self, on):
+        """Whether the adjoint sweep leaves out the planes its touched map shows at rest (default on)."""
+        check(self.lib.qp3d_fwi_set_rest_map(self.h, 1 if on else 0))
+
     def times(self):
         ms = (C.c_double * 3)()
         check(self.lib.qp3d_fwi_times(self.h, ms))

@@ -256,6 +256,11 @@ int qp3d_fwi_forward(qp3d_fwi *f);
 int qp3d_fwi_gradient(qp3d_fwi *f, const float *resid, double *g_vp, double *g_eps, double *g_del);
 /* ms[0] checkpointed forward run, ms[1] recomputation, ms[2] adjoint sweep (device time). */
 int qp3d_fwi_times(const qp3d_fwi *f, double *ms);
+/* on != 0 (the default): the adjoint sweep keeps a per-block map of where its state has been
+ * nonzero (from the receiver plane on) and leaves out the planes of each work item outside it,
+ * where its launches would write zeros over zeros. The gradient is the same either way (up to
+ * the sign of zeros). */
+int qp3d_fwi_set_rest_map(qp3d_fwi *f, int on);
 
 /* ---- batched shots ------------------------------------------------------------------------
  * No reference counterpart (the reference runs its shots one after another, Main.c:179-247).
