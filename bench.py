@@ -119,14 +119,19 @@ def bench_fwi(args):
     ref = None
     # the adjoint sweep leaves out the planes its touched map shows at rest, except in the
     # rest_map_off run (same gradient, every plane swept)
-    cases = [("ckpt50_fp32", 50, "fp32", True), ("ckpt50_fp32_rest_map_off", 50, "fp32", False),
-             ("ckpt50_bf16", 50, "bf16", True)]
+    # rest map off: the same gradient, every plane swept; resident: the forward run keeps the
+    # u_k of as many of the last segments as fit in HBM, which the gradient does not recompute
+    cases = [("ckpt50_fp32", 50, "fp32", True, 1), ("ckpt50_fp32_rest_map_off", 50, "fp32", False, 1),
+             ("ckpt50_bf16", 50, "bf16", True, 1), ("ckpt50_fp32_resident", 50, "fp32", True, -1)]
     if args.fwi_full:
-        cases.append(("full_storage", nt, "fp32", True))
-    for key, interval, ck, rest in cases:
+        cases.append(("full_storage", nt, "fp32", True, 1))
+    for key, interval, ck, rest, resident in cases:
+        kept = 1
         try:
             with qp3d.Plan(s0, gmax_mode=args.gmax_mode) as p, qp3d.Fwi(p, interval=interval, ckpt_storage=ck) as f:
                 f.set_rest_map(rest)
+                if resident != 1:
+                    kept = f.set_resident_segments(resident)
                 f.forward()
                 p.sync()
                 t = time.perf_counter()
@@ -140,7 +145,8 @@ def bench_fwi(args):
             continue
         r = {"s_per_shot": round(wall, 3), "x_forward": round(wall / t_fwd, 2),
              "ms_forward": round(tm["forward"], 1), "ms_recompute": round(tm["recompute"], 1),
-             "ms_adjoint": round(tm["adjoint"], 1), "misfit": float(0.5 * np.sum((rec.astype(np.float64) - obs) ** 2))}
+             "ms_adjoint": round(tm["adjoint"], 1), "misfit": float(0.5 * np.sum((rec.astype(np.float64) - obs) ** 2)),
+             "resident_segments": kept}
         if ref is None:
             ref = g
         else:

@@ -37,7 +37,13 @@ struct qp3d_fwi {
   // same maps as the forward run (none: recomputation without them)
   std::vector<void *> camp;
   std::vector<char> amp_saved;  // [nseg] the record of segment s holds the forward run's
-  float *ubuf = nullptr;         // [interval][N] u_k of the segment being back-propagated
+  // [keep][interval][N] u_k of segments: slot i holds the u_k of segment slot_seg[i] (-1: none).
+  // The forward run fills slot i with segment nseg - keep + i (the last `keep` segments stay
+  // resident in HBM and are back-propagated without recomputation); a recomputed segment goes
+  // to its own slot when it has one, else to slot 0.
+  float *ubuf = nullptr;
+  int keep = 1;
+  std::vector<int> slot_seg;
   float *adj = nullptr;          // adjoint state: u[2], w1[2], p[2], w3[2], w21 (9 N)
   float *prod = nullptr;         // Pxx Pyy Pzz Qx Qy Qz (6 N)
   float *gacc = nullptr;         // per-segment FP32 sums w.r.t. v2, eps, eps-del (3 N)
@@ -51,7 +57,6 @@ struct qp3d_fwi {
   int nbz = 0, nbx = 0, nby = 0;
   bool rest_map = true;
   bool forward_done = false;
-  int ubuf_seg = -1;             // segment whose u_k ubuf holds (the forward run keeps the last)
   unsigned gen = 0;              // the plan's upload count at the forward run
   hipEvent_t ev[4] = {};
   double ms_forward = 0, ms_recompute = 0, ms_adjoint = 0;
@@ -133,6 +138,7 @@ extern "C" int qp3d_fwi_create(qp3d_plan *p, int interval, int ckpt_storage, qp3
     }
   }
   if ((rc = dmalloc((void **)&f->ubuf, sizeof(float) * N * f->interval, "segment wavefields"))) return fail(rc);
+  f->slot_seg.assign(1, -1);
   if ((rc = dmalloc((void **)&f->adj, sizeof(float) * 9 * N, "adjoint state"))) return fail(rc);
   if ((rc = dmalloc((void **)&f->prod, sizeof(float) * 6 * N, "product fields"))) return fail(rc);
   if ((rc = dmalloc((void **)&f->gacc, sizeof(float) * 3 * N, "segment gradient"))) return fail(rc);
@@ -272,6 +278,10 @@ extern "C" int qp3d_fwi_forward(qp3d_fwi *f)
     qp3d_set_error("fwi_forward: upload the model first");
     return QP3D_ERR_STATE;
   }
+  if (!f->ubuf || f->keep < 1) {
+    qp3d_set_error("fwi_forward: no segment buffer (qp3d_fwi_set_resident_segments failed)");
+    return QP3D_ERR_STATE;
+  }
   HIP_TRY(hipSetDevice(p->cfg.device));
   int rc = qp3d_plan_reset(p);
   if (rc) return rc;
@@ -279,7 +289,7 @@ extern "C" int qp3d_fwi_forward(qp3d_fwi *f)
   // segment 0 starts from the reset (its record: zeros, an empty map, a whole history)
   f->amp_saved.assign((size_t)f->nseg, 0);
   f->amp_saved[0] = !f->camp.empty() && p->amp_ok;
-  const int klast = (f->nseg - 1) * f->interval;  // steps of the last segment keep u_k
+  const int klast = (f->nseg - f->keep) * f->interval;  // steps of the resident segments keep u_k
   const size_t N = (size_t)p->N;
   for (int k = 0; k < p->cfg.nt; k++) {
     if (k % f->interval == 0 && k > 0 && (rc = save_ckpt(f, k / f->interval))) return rc;
@@ -288,7 +298,7 @@ extern "C" int qp3d_fwi_forward(qp3d_fwi *f)
                              hipMemcpyDeviceToDevice, p->stream));
     if ((rc = qp3d_forward_step_fwi(p, k, false))) return rc;
   }
-  f->ubuf_seg = f->nseg - 1;
+  for (int i = 0; i < f->keep; i++) f->slot_seg[i] = f->nseg - f->keep + i;
   f->gen = p->upload_gen;
   p->steps_done = p->cfg.nt;
   HIP_TRY(hipEventRecord(f->ev[1], p->stream));
@@ -376,27 +386,30 @@ extern "C" int qp3d_fwi_gradient(qp3d_fwi *f, const float *resid, double *g_vp, 
     HIP_TRY(hipMemcpyAsync(f->tmap, f->tmap0.data(), f->tmap0.size(), hipMemcpyHostToDevice, p->stream));
   }
   double ms_re = 0, ms_adj = 0;
-  int rc;
+  int rc, last_re = -1;  // the last segment recomputed
   for (int s = f->nseg - 1; s >= 0; s--) {
     const int k0 = s * f->interval, k1 = std::min(k0 + f->interval, c.nt);
     HIP_TRY(hipEventRecord(f->ev[0], p->stream));
-    if (s != f->ubuf_seg) {
+    const int slot = s >= f->nseg - f->keep ? s - (f->nseg - f->keep) : 0;
+    float *const ub = f->ubuf + (size_t)slot * f->interval * N;
+    if (f->slot_seg[slot] != s) {
       if ((rc = load_ckpt(f, s))) return rc;
       const bool maps = load_amp(f, s, rc);
       if (rc) return rc;
       for (int k = k0; k < k1; k++) {
         const int cur = (k & 1) ^ 1;  // slot of u_k in the forward state
-        HIP_TRY(hipMemcpyAsync(f->ubuf + (size_t)(k - k0) * N, p->u[cur], sizeof(float) * N,
+        HIP_TRY(hipMemcpyAsync(ub + (size_t)(k - k0) * N, p->u[cur], sizeof(float) * N,
                                hipMemcpyDeviceToDevice, p->stream));
         // G_k from the forward run; the forward run's maps when its record was restored
         if ((rc = qp3d_forward_step_fwi(p, k, true, maps))) return rc;
       }
-      f->ubuf_seg = s;
+      f->slot_seg[slot] = s;
+      last_re = s;
     }
     HIP_TRY(hipEventRecord(f->ev[1], p->stream));
     HIP_TRY(hipMemsetAsync(f->gacc, 0, sizeof(float) * 3 * N, p->stream));
     for (int k = k1 - 1; k >= k0; k--)
-      if ((rc = adjoint_step(f, k, f->ubuf + (size_t)(k - k0) * N))) return rc;
+      if ((rc = adjoint_step(f, k, ub + (size_t)(k - k0) * N))) return rc;
     hipLaunchKernelGGL(k_fold, dim3(grid_of(3 * (long)N)), dim3(256), 0, p->stream, f->gsum, (const float *)f->gacc,
                        3 * (long)N);
     HIP_TRY(hipGetLastError());
@@ -419,7 +432,54 @@ extern "C" int qp3d_fwi_gradient(qp3d_fwi *f, const float *resid, double *g_vp, 
   f->ms_recompute = ms_re;
   f->ms_adjoint = ms_adj;
   // the plan's state holds the end of the last segment recomputed (or of the forward run)
-  if (f->nseg > 1) p->steps_done = std::min(f->interval, c.nt);
+  if (last_re >= 0) p->steps_done = std::min((last_re + 1) * f->interval, c.nt);
+  return QP3D_OK;
+}
+
+// Segments whose u_k the forward run keeps resident in HBM (the last n; 1 by default), so that
+// the gradient back-propagates them without recomputation; n < 0: as many as fit in the free
+// device memory less 2 GiB. Reallocates the segment buffer; the next gradient needs a forward
+// run after it. *kept (may be null) receives the number chosen.
+extern "C" int qp3d_fwi_set_resident_segments(qp3d_fwi *f, int n, int *kept)
+{
+  if (!f) return QP3D_ERR_ARG;
+  qp3d_plan *p = f->p;
+  HIP_TRY(hipSetDevice(p->cfg.device));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  const size_t seg = sizeof(float) * (size_t)p->N * f->interval;
+  if (n < 0) {
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot));
+    const size_t margin = (size_t)2 << 30, have = fr + seg * f->keep;
+    n = have > margin ? (int)std::min<size_t>((have - margin) / seg, (size_t)f->nseg) : 1;
+  }
+  if (n < 1 || n > f->nseg) {
+    qp3d_set_error("fwi_set_resident_segments: need 1 <= n <= %d segments (got %d)", f->nseg, n);
+    return QP3D_ERR_ARG;
+  }
+  if (n != f->keep) {
+    // the old buffer goes first: both may not fit at once
+    (void)hipFree(f->ubuf);
+    f->ubuf = nullptr;
+    hipError_t e = hipMalloc((void **)&f->ubuf, seg * n);
+    if (e != hipSuccess) {
+      f->ubuf = nullptr;
+      (void)hipGetLastError();
+      const int want = n;
+      n = 1;
+      e = hipMalloc((void **)&f->ubuf, seg);
+      f->keep = e == hipSuccess ? 1 : 0;
+      f->slot_seg.assign(1, -1);
+      f->forward_done = false;
+      qp3d_set_error("fwi_set_resident_segments: hipMalloc of %zu bytes for %d segments failed%s", seg * want, want,
+                     e == hipSuccess ? " (one segment kept)" : "; no segment buffer left");
+      return QP3D_ERR_ALLOC;
+    }
+    f->keep = n;
+  }
+  f->slot_seg.assign((size_t)f->keep, -1);
+  f->forward_done = false;
+  if (kept) *kept = f->keep;
   return QP3D_OK;
 }
 

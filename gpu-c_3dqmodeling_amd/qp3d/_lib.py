@@ -90,6 +90,7 @@ _SIGS = [
     ("qp3d_fwi_gradient", C.c_int, [C.c_void_p, _F] + [C.POINTER(C.c_double)] * 3),
     ("qp3d_fwi_times", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     ("qp3d_fwi_set_rest_map", C.c_int, [C.c_void_p, C.c_int]),
+    ("qp3d_fwi_set_resident_segments", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     ("qp3d_batch_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     ("qp3d_batch_destroy", C.c_int, [C.c_void_p]),
     ("qp3d_batch_reset", C.c_int, [C.c_void_p]),

@@ -216,6 +216,13 @@ class Fwi:
         """Whether the adjoint sweep leaves out the planes its touched map shows at rest (default on)."""
         check(self.lib.qp3d_fwi_set_rest_map(self.h, 1 if on else 0))
 
+    def set_resident_segments(self, n):
+        """Keeps the u_k of the last n segments in HBM after the forward run (n < 0: as many as
+        fit); returns the number kept. Run forward() again afterwards."""
+        kept = C.c_int(0)
+        check(self.lib.qp3d_fwi_set_resident_segments(self.h, int(n), C.byref(kept)))
+        return kept.value
+
     def times(self):
         ms = (C.c_double * 3)()
         check(self.lib.qp3d_fwi_times(self.h, ms))

@@ -261,6 +261,13 @@ int qp3d_fwi_times(const qp3d_fwi *f, double *ms);
  * where its launches would write zeros over zeros. The gradient is the same either way (up to
  * the sign of zeros). */
 int qp3d_fwi_set_rest_map(qp3d_fwi *f, int on);
+/* The forward run keeps the u_k of the last n segments resident in HBM (1 by default), and the
+ * gradient back-propagates those without recomputing them (n = number of segments: no
+ * recomputation at all, the checkpoints still taken); n < 0: as many as fit in the free device
+ * memory less 2 GiB. Reallocates the segment buffer (interval * nx * ny * nz * 4 B per segment,
+ * rows padded); run qp3d_fwi_forward again afterwards. *kept (may be NULL) = the number kept. On
+ * an allocation failure one segment is kept where it fits (QP3D_ERR_ALLOC either way). */
+int qp3d_fwi_set_resident_segments(qp3d_fwi *f, int n, int *kept);
 
 /* ---- batched shots ------------------------------------------------------------------------
  * No reference counterpart (the reference runs its shots one after another, Main.c:179-247).

@@ -160,6 +160,32 @@ def test_rest_map_sweep_equals_full_sweep(rec_plane):
         assert np.array_equal(a, b), float(np.abs(a - b).max())
 
 
+def test_resident_segments_equal_recomputed_ones():
+    """Segments whose u_k stay resident after the forward run give the gradient of recomputed
+    ones, on the first and on a second gradient (which recomputes the resident segments that a
+    recomputation overwrote); changing the count asks for a new forward run."""
+    s = small_setup(elliptic=False, nt=120)
+    obs = observed(s)
+    grads = {}
+    with qp3d.Plan(s) as p, qp3d.Fwi(p, interval=25) as f:  # 5 segments
+        for n in (1, 3, 5, -1):
+            kept = f.set_resident_segments(n)
+            assert kept == (5 if n < 0 else n)
+            with pytest.raises(qp3d.Qp3dError):
+                f.gradient(np.zeros((s.nt, s.hx, s.hy), np.float32))
+            rec = f.forward().astype(np.float64)
+            resid = (rec - obs).astype(np.float32)
+            grads[n] = [f.gradient(resid), f.gradient(resid)]
+            say(f"resident {kept}: {f.times()}")
+        with pytest.raises(qp3d.Qp3dError):
+            f.set_resident_segments(6)
+    ref = grads[1][0]
+    for n, gs in grads.items():
+        for g in gs:
+            for a, b in zip(g, ref):
+                assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(b), (n, float(np.abs(a - b).max()))
+
+
 def test_predicted_threshold_gradient():
     """With gmax_mode 2 (predicted threshold and fix-up) the checkpoint recomputation
     reproduces the forward run's steps, so the checkpointed gradient matches full storage to
