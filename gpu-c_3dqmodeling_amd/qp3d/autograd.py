@@ -47,13 +47,17 @@ def hip_runtimes():
 
 
 class Engine:
-    """A plan and its gradient object for one grid, source and set of PML profiles."""
+    """A plan and its gradient object for one grid, source and set of PML profiles; `resident`
+    segments of u_k stay in HBM after the forward pass (qp3d_fwi_set_resident_segments; -1: as
+    many as fit), which the backward pass does not recompute."""
 
-    def __init__(self, setup, interval=50, ckpt_storage="fp32", device=0):
+    def __init__(self, setup, interval=50, ckpt_storage="fp32", device=0, resident=1):
         self.s = setup
         self.device = int(device)
         self.plan = Plan(setup, device=self.device)
         self.fwi = Fwi(self.plan, interval=interval, ckpt_storage=ckpt_storage)
+        if resident != 1:
+            self.fwi.set_resident_segments(resident)
         self._hold = []
 
     def _on_gpu(self, t):

@@ -204,15 +204,17 @@ def test_predicted_threshold_gradient():
             assert err <= tol
 
 
-def test_autograd_shim_matches_direct_gradient():
-    """The shim with host tensors against the Fwi object's gradient."""
+@pytest.mark.parametrize("resident", [1, -1])
+def test_autograd_shim_matches_direct_gradient(resident):
+    """The shim with host tensors against the Fwi object's gradient (with one segment of u_k
+    kept after the forward pass, and with all of them)."""
     import torch
 
     import qp3d.autograd
 
     s = small_setup(elliptic=False, nt=100)
     obs = observed(s)
-    eng = qp3d.autograd.Engine(s, interval=30)
+    eng = qp3d.autograd.Engine(s, interval=30, resident=resident)
     try:
         vp = torch.tensor(s.vp, requires_grad=True)
         eps = torch.tensor(s.eps, requires_grad=True)
