@@ -415,7 +415,9 @@ struct ApplyArgs {
 template <class T>
 __global__ __launch_bounds__(T::NT) void k_adj_apply(ApplyArgs a)
 {
-  __shared__ float sm[2][4][T::LX + 1][T::LY];  // Pxx, Qx, Pyy, Qy
+  // Pxx, Qx, Pyy, Qy of one plane: a single buffer (a second barrier per plane) halves the
+  // workgroup's LDS, so that more workgroups stay resident on a CU
+  __shared__ float sm[1][4][T::LX + 1][T::LY];
   const Geo &g = a.g;
   int x0, x1, y0, y1, z0, z1;
   bool fold;
@@ -490,8 +492,7 @@ __global__ __launch_bounds__(T::NT) void k_adj_apply(ApplyArgs a)
     L.refresh();
     if (more) issue(z + 1);
     const size_t zo = (size_t)z * g.pln;
-    float (*sxx)[T::LY] = sm[it & 1][0], (*sqx)[T::LY] = sm[it & 1][1], (*syy)[T::LY] = sm[it & 1][2],
-          (*sqy)[T::LY] = sm[it & 1][3];
+    float (*sxx)[T::LY] = sm[0][0], (*sqx)[T::LY] = sm[0][1], (*syy)[T::LY] = sm[0][2], (*sqy)[T::LY] = sm[0][3];
 #pragma unroll
     for (int r = 0; r < T::RX; r++) {
       if (!rows_on[r]) continue;
@@ -528,7 +529,8 @@ __global__ __launch_bounds__(T::NT) void k_adj_apply(ApplyArgs a)
         qqz[r][it % QN] = fqz[r];
         acv[r] = acn[r];
       }
-      stage((it + 1) & 1);
+      __syncthreads();  // every wavefront is done with plane z
+      stage(0);
       __syncthreads();
     }
   };
