@@ -59,6 +59,10 @@ struct qp3d_fwi {
   bool forward_done = false;
   unsigned gen = 0;              // the plan's upload count at the forward run
   hipEvent_t ev[4] = {};
+  // the forward run's copies of u_k into the segment buffer, on a stream of their own so that
+  // each overlaps the step that reads u_k: ce[0] u_k complete, ce[1] its copy done
+  hipStream_t cs = nullptr;
+  hipEvent_t ce[2] = {};
   double ms_forward = 0, ms_recompute = 0, ms_adjoint = 0;
 };
 
@@ -86,6 +90,9 @@ static void fwi_free(qp3d_fwi *f)
   (void)hipFree(f->wl_a.d);
   for (hipEvent_t e : f->ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : f->ce)
+    if (e) (void)hipEventDestroy(e);
+  if (f->cs) (void)hipStreamDestroy(f->cs);
   delete f;
 }
 
@@ -170,6 +177,9 @@ extern "C" int qp3d_fwi_create(qp3d_plan *p, int interval, int ckpt_storage, qp3
   if ((rc = qp3d_build_uniform_list(p, TILE_APPLY, oa, f->wl_a))) return fail(rc);
   for (hipEvent_t &e : f->ev)
     if (hipEventCreate(&e) != hipSuccess) return fail(QP3D_ERR_HIP);
+  for (hipEvent_t &e : f->ce)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail(QP3D_ERR_HIP);
+  if (hipStreamCreateWithFlags(&f->cs, hipStreamNonBlocking) != hipSuccess) return fail(QP3D_ERR_HIP);
   *out = f;
   return QP3D_OK;
 }
@@ -291,13 +301,23 @@ extern "C" int qp3d_fwi_forward(qp3d_fwi *f)
   f->amp_saved[0] = !f->camp.empty() && p->amp_ok;
   const int klast = (f->nseg - f->keep) * f->interval;  // steps of the resident segments keep u_k
   const size_t N = (size_t)p->N;
+  bool copying = false;  // a copy of u_{k-1} is under way on the copy stream
   for (int k = 0; k < p->cfg.nt; k++) {
     if (k % f->interval == 0 && k > 0 && (rc = save_ckpt(f, k / f->interval))) return rc;
-    if (k >= klast)
+    // step k writes u_{k+1} over u_{k-1}, whose copy must have finished
+    if (copying) HIP_TRY(hipStreamWaitEvent(p->stream, f->ce[1], 0));
+    if (k >= klast) {
+      // u_k is complete once the work queued so far has run; its copy overlaps step k
+      HIP_TRY(hipEventRecord(f->ce[0], p->stream));
+      HIP_TRY(hipStreamWaitEvent(f->cs, f->ce[0], 0));
       HIP_TRY(hipMemcpyAsync(f->ubuf + (size_t)(k - klast) * N, p->u[(k & 1) ^ 1], sizeof(float) * N,
-                             hipMemcpyDeviceToDevice, p->stream));
+                             hipMemcpyDeviceToDevice, f->cs));
+      HIP_TRY(hipEventRecord(f->ce[1], f->cs));
+      copying = true;
+    }
     if ((rc = qp3d_forward_step_fwi(p, k, false))) return rc;
   }
+  if (copying) HIP_TRY(hipStreamWaitEvent(p->stream, f->ce[1], 0));
   for (int i = 0; i < f->keep; i++) f->slot_seg[i] = f->nseg - f->keep + i;
   f->gen = p->upload_gen;
   p->steps_done = p->cfg.nt;
