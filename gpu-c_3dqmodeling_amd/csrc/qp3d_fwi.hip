@@ -278,6 +278,19 @@ static int load_ckpt(qp3d_fwi *f, int s)
   return QP3D_OK;
 }
 
+// Copies u_k (the plan's current level before step k) to dst on the copy stream, once the work
+// queued on the plan's stream so far has run, so that it overlaps step k; step k + 1, which
+// writes u_{k+2} over u_k, and every reader of dst wait for f->ce[1] on the plan's stream.
+static int copy_uk(qp3d_fwi *f, float *dst, int k)
+{
+  qp3d_plan *p = f->p;
+  HIP_TRY(hipEventRecord(f->ce[0], p->stream));
+  HIP_TRY(hipStreamWaitEvent(f->cs, f->ce[0], 0));
+  HIP_TRY(hipMemcpyAsync(dst, p->u[(k & 1) ^ 1], sizeof(float) * (size_t)p->N, hipMemcpyDeviceToDevice, f->cs));
+  HIP_TRY(hipEventRecord(f->ce[1], f->cs));
+  return QP3D_OK;
+}
+
 // Forward run from rest with a checkpoint at the start of every segment; the plan then holds
 // the record, the per-step maxima and the final state as after qp3d_plan_run.
 extern "C" int qp3d_fwi_forward(qp3d_fwi *f)
@@ -307,12 +320,7 @@ extern "C" int qp3d_fwi_forward(qp3d_fwi *f)
     // step k writes u_{k+1} over u_{k-1}, whose copy must have finished
     if (copying) HIP_TRY(hipStreamWaitEvent(p->stream, f->ce[1], 0));
     if (k >= klast) {
-      // u_k is complete once the work queued so far has run; its copy overlaps step k
-      HIP_TRY(hipEventRecord(f->ce[0], p->stream));
-      HIP_TRY(hipStreamWaitEvent(f->cs, f->ce[0], 0));
-      HIP_TRY(hipMemcpyAsync(f->ubuf + (size_t)(k - klast) * N, p->u[(k & 1) ^ 1], sizeof(float) * N,
-                             hipMemcpyDeviceToDevice, f->cs));
-      HIP_TRY(hipEventRecord(f->ce[1], f->cs));
+      if ((rc = copy_uk(f, f->ubuf + (size_t)(k - klast) * N, k))) return rc;
       copying = true;
     }
     if ((rc = qp3d_forward_step_fwi(p, k, false))) return rc;
@@ -417,12 +425,13 @@ extern "C" int qp3d_fwi_gradient(qp3d_fwi *f, const float *resid, double *g_vp, 
       const bool maps = load_amp(f, s, rc);
       if (rc) return rc;
       for (int k = k0; k < k1; k++) {
-        const int cur = (k & 1) ^ 1;  // slot of u_k in the forward state
-        HIP_TRY(hipMemcpyAsync(ub + (size_t)(k - k0) * N, p->u[cur], sizeof(float) * N,
-                               hipMemcpyDeviceToDevice, p->stream));
+        // step k writes u_{k+1} over u_{k-1}, whose copy must have finished
+        if (k > k0) HIP_TRY(hipStreamWaitEvent(p->stream, f->ce[1], 0));
+        if ((rc = copy_uk(f, ub + (size_t)(k - k0) * N, k))) return rc;
         // G_k from the forward run; the forward run's maps when its record was restored
         if ((rc = qp3d_forward_step_fwi(p, k, true, maps))) return rc;
       }
+      HIP_TRY(hipStreamWaitEvent(p->stream, f->ce[1], 0));  // the adjoint sweep reads ub
       f->slot_seg[slot] = s;
       last_re = s;
     }
